@@ -1,0 +1,135 @@
+/*
+ * wpic.h -- C-ABI of the MI355X-native WumingPIC step loop (2D).
+ *
+ * One opaque context per simulation owns device-resident particles (structure
+ * of arrays, sorted by cell), fields and solver work arrays on one GPU and one
+ * HIP stream.  Host arrays cross this boundary only in the reference's own
+ * Fortran layouts, so the reference's module entry points can be rebound onto
+ * these calls (see wumingpic_amd/fortran/ for the ISO_C_BINDING shim):
+ *
+ *   up(ndim, np, nys:nye, nsp)        real(8), id bit-cast in slot 6
+ *   uf(6, nxgs-2:nxge+2, nys-2:nye+2) real(8)  Bx By Bz Ex Ey Ez
+ *   cumcnt(nxgs:nxge+1, nys:nye, nsp) integer
+ *   np2(nys:nye, nsp)                 integer
+ *
+ * Every call returns 0 on success and a negative WPIC_E* code on failure;
+ * wpic_last_error() gives the message (the reference's own text where the
+ * reference stops with one).  Calls on one context are not thread-safe.
+ */
+#ifndef WPIC_H
+#define WPIC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WPIC_MAXSP 4
+
+/* boundary kinds */
+#define WPIC_BC_PERIODIC 0 /* 2d/common/boundary_periodic.f90 */
+#define WPIC_BC_WALLX 1    /* 2d/proj/reconnection/boundary_reconnection.f90 */
+
+/* arithmetic modes of the FP64 path */
+#define WPIC_MODE_FAST 0   /* tiled deposit with atomics, tree dot products */
+#define WPIC_MODE_PARITY 1 /* reference summation order: bit-exact vs 1 thread */
+
+/* error codes */
+#define WPIC_OK 0
+#define WPIC_EINVAL -1   /* bad argument or call order */
+#define WPIC_EHIP -2     /* HIP runtime error */
+#define WPIC_ECG -3      /* "stop at cgm after ite_max" (field.f90:427-430) */
+#define WPIC_EMEM -4     /* "memory over (np2 > np)" (boundary_periodic.f90:231-234) */
+#define WPIC_EDOMAIN -5  /* particle outside the sortable domain */
+
+typedef struct wpic_ctx wpic_ctx;
+
+/* Union of the arguments of particle__init (2d/common/particle.f90:18-23),
+ * field__init (2d/common/field.f90:22-29), sort__init (2d/common/sort.f90:16)
+ * and boundary_*__init (2d/common/boundary_periodic.f90:26-33) at one rank:
+ * nys = nygs, nye = nyge. */
+typedef struct {
+  int ndim; /* 6 */
+  int nsp;
+  int nxgs, nxge, nygs, nyge;
+  double delx, delt, c, gfac;
+  double q[WPIC_MAXSP], r[WPIC_MAXSP];
+  int bc;   /* WPIC_BC_* */
+  int mode; /* WPIC_MODE_* */
+  long long capacity; /* particles per species held on the device */
+} wpic_params2d;
+
+/* context ----------------------------------------------------------------- */
+int wpic_create2d(wpic_ctx **ctx, const wpic_params2d *p);
+int wpic_destroy(wpic_ctx *ctx);
+const char *wpic_last_error(void);
+int wpic_set_mode(wpic_ctx *ctx, int mode);
+int wpic_sync(wpic_ctx *ctx);
+
+/* reference-layout transfers --------------------------------------------- */
+/* up: sorted particles with their cumcnt (the app's `up` after sort__bucket,
+ * 2d/proj/weibel/app.f90:107); only slots 1..np2(j,isp) of each row are read */
+int wpic_upload_up2d(wpic_ctx *ctx, const double *up, const int *np2, const int *cumcnt, int np);
+int wpic_download_up2d(wpic_ctx *ctx, double *up, int *np2, int *cumcnt, int np);
+/* gp: the particle array handed between particle__solv, bc__particle_x/y and
+ * sort__bucket (2d/proj/weibel/app.f90:102-107).  Uploading gp starts the
+ * state "rows ready for sort__bucket" (the restart path, app.f90:351-352). */
+int wpic_upload_gp2d(wpic_ctx *ctx, const double *gp, const int *np2, int np);
+int wpic_download_gp2d(wpic_ctx *ctx, double *gp, int *np2, int np);
+int wpic_upload_fields2d(wpic_ctx *ctx, const double *uf);
+int wpic_download_fields2d(wpic_ctx *ctx, double *uf);
+/* field.f90's saved module arrays: uj(3,...) after set_boundary_curre and
+ * df(6,...) after the last set_boundary_dfield of field__fdtd_i */
+int wpic_download_current2d(wpic_ctx *ctx, double *uj);
+int wpic_download_dfield2d(wpic_ctx *ctx, double *df);
+int wpic_upload_dfield2d(wpic_ctx *ctx, const double *df);
+
+/* reference entry points ------------------------------------------------- */
+/* particle__solv(gp, up, uf, cumcnt, nxs, nxe): 2d/common/particle.f90:48 */
+int wpic_particle_solv2d(wpic_ctx *ctx);
+/* field__fdtd_i(uf, up, gp, cumcnt, nxs, nxe, bc_dfield, bc_curre, bc_phi):
+ * 2d/common/field.f90:66.  The boundary procedures are applied on the device
+ * according to the context's boundary kind. */
+int wpic_field_fdtd_i2d(wpic_ctx *ctx);
+/* boundary_periodic__particle_x(gp, np2): boundary_periodic.f90:61, or
+ * boundary_reconnection__particle_x(gp, np2, nxs, nxe): :61 */
+int wpic_bc_particle_x2d(wpic_ctx *ctx);
+/* boundary_periodic__particle_y(gp, np2): boundary_periodic.f90:99 */
+int wpic_bc_particle_y2d(wpic_ctx *ctx);
+/* sort__bucket(up, gp, cumcnt, np2, nxs, nxe): 2d/common/sort.f90:36 */
+int wpic_sort_bucket2d(wpic_ctx *ctx);
+/* CG iteration counts of the last field__fdtd_i, one per component */
+int wpic_cg_iters2d(wpic_ctx *ctx, int iters[3]);
+
+/* the whole step loop in the app's order (2d/proj/weibel/app.f90:102-107 or
+ * 2d/proj/reconnection/app.f90:101-106), fused on the device */
+int wpic_step2d(wpic_ctx *ctx, int nsteps);
+
+/* diagnostics -------------------------------------------------------------- */
+/* energy_history sums (2d/proj/weibel/app.f90:503-534):
+ * out = {sum r(gamma-1), sum E^2/8pi, sum B^2/8pi} over interior cells */
+int wpic_energy2d(wpic_ctx *ctx, double out[3]);
+/* Gauss residual div E - 4 pi rho at cell centres, rho with the deposit's
+ * quadratic shape; G(nx, ny) column-major.  No reference counterpart. */
+int wpic_gauss2d(wpic_ctx *ctx, double *G);
+/* particle counts per species on the device */
+int wpic_counts2d(wpic_ctx *ctx, long long *counts);
+
+/* seeded initial state ----------------------------------------------------- */
+/* Counter-based seeded two-species beam state: n_ppc particles per cell per
+ * species at uniformly random in-cell positions, ions and electrons
+ * co-located, electrons u_x = +/-udrift (alternating by slot) + uth_e N(0,1),
+ * u_y, u_z = uth_e N(0,1); ions uth_i N(0,1); ids -1,-2,... per species as
+ * 2d/proj/weibel/app.f90:442-474 numbers them; uf = 0 except Bz = b0. */
+int wpic_ic_beams2d(wpic_ctx *ctx, unsigned long long seed, int n_ppc, double udrift,
+                    double uth_e, double uth_i, double b0);
+
+/* timing of named kernels with HIP events on the context's stream ---------- */
+int wpic_set_timing(wpic_ctx *ctx, int on);
+/* name: "push_deposit", "sort", "exchange", "field", "cg"; returns the summed
+ * milliseconds and the number of launches since timing was switched on */
+int wpic_timing(wpic_ctx *ctx, const char *name, double *ms, long long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,911 @@
+/*
+ * oracle2d.c -- CPU restatement of the 2D WumingPIC step (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY (see wpic_oracle.h).  Each routine follows the
+ * reference routine named in its comment, statement by statement, so that at
+ * one OpenMP thread the results are the reference's bit for bit (given the
+ * same compiler rules: IEEE double, no FMA contraction, correctly rounded
+ * sqrt and division).  With more threads the deposit reduction order changes
+ * (as it does in the reference's OMP REDUCTION), which is what the timed CPU
+ * baseline runs.
+ */
+#include "wpic_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static char g_err[256];
+const char *or_last_error(void) { return g_err; }
+static int set_err(const char *m) {
+  snprintf(g_err, sizeof g_err, "%s", m);
+  return -1;
+}
+
+void or_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* ---------------------------------------------------------------------------
+ * Directed rounding.  The reference sets ieee_down for the whole body of
+ * boundary_periodic__particle_x/_y (boundary_periodic.f90:74,124).  These
+ * return the correctly rounded-down result using round-to-nearest arithmetic
+ * plus an exact error term (TwoSum / FMA residual), so no FP mode switch and
+ * no reliance on the compiler honouring it.
+ * ------------------------------------------------------------------------- */
+double or_add_rd(double a, double b) {
+  double s = a + b;
+  if (s == 0.0) {
+    /* in round-down an exact zero sum is -0 unless both operands are +0 */
+    if (a == 0.0 && b == 0.0 && !signbit(a) && !signbit(b)) return 0.0;
+    return -0.0;
+  }
+  double bb = s - a;
+  double err = (a - (s - bb)) + (b - bb);
+  if (err < 0.0) s = nextafter(s, -INFINITY);
+  return s;
+}
+
+double or_mul_rd(double a, double b) {
+  double p = a * b;
+  double err = fma(a, b, -p);
+  if (err < 0.0) p = nextafter(p, -INFINITY);
+  return p;
+}
+
+double or_div_rd(double a, double b) {
+  double q = a / b;
+  /* residual a - q*b is exact with one fma */
+  double r = fma(-q, b, a);
+  /* true quotient = q + r/b; it lies below q when r/b < 0 */
+  if ((r < 0.0 && b > 0.0) || (r > 0.0 && b < 0.0)) q = nextafter(q, -INFINITY);
+  return q;
+}
+
+/* ---------------------------------------------------------------------------
+ * Layout helpers (Fortran column-major, reference bounds).
+ * ------------------------------------------------------------------------- */
+#define NX (p->nxge - p->nxgs + 1)
+#define NY (p->nye - p->nys + 1)
+
+static inline size_t iUP(const or2d_params *p, int idim, int ii, int j, int isp) {
+  return (size_t)(idim - 1) +
+         (size_t)p->ndim * ((size_t)(ii - 1) +
+                            (size_t)p->np * ((size_t)(j - p->nys) + (size_t)NY * (size_t)(isp - 1)));
+}
+static inline size_t iF6(const or2d_params *p, int c, int i, int j) {
+  return (size_t)(c - 1) +
+         6u * ((size_t)(i - (p->nxgs - 2)) + (size_t)(NX + 4) * (size_t)(j - (p->nys - 2)));
+}
+static inline size_t iF3(const or2d_params *p, int c, int i, int j) {
+  return (size_t)(c - 1) +
+         3u * ((size_t)(i - (p->nxgs - 2)) + (size_t)(NX + 4) * (size_t)(j - (p->nys - 2)));
+}
+static inline size_t iCC(const or2d_params *p, int i, int j, int isp) {
+  return (size_t)(i - p->nxgs) + (size_t)(NX + 1) * ((size_t)(j - p->nys) + (size_t)NY * (size_t)(isp - 1));
+}
+static inline size_t iNP(const or2d_params *p, int j, int isp) {
+  return (size_t)(j - p->nys) + (size_t)NY * (size_t)(isp - 1);
+}
+static inline size_t iGK(const or2d_params *p, int l, int i, int j) {
+  return (size_t)(l - 1) + 3u * ((size_t)(i - p->nxgs) + (size_t)NX * (size_t)(j - p->nys));
+}
+
+/* field.f90:53-59 */
+void or2d_field_consts(const or2d_params *p, double f[5]) {
+  const double pi = M_PI; /* 4.0D0*atan(1.0D0) */
+  double c = p->c, delt = p->delt, delx = p->delx, gfac = p->gfac;
+  double t = delx / (c * delt * gfac);
+  f[0] = c * delt / delx;          /* f1 */
+  f[1] = gfac * f[0] * f[0];       /* f2 */
+  f[2] = 4.0 * pi * delx / c;      /* f3 */
+  f[3] = 4.0 + t * t;              /* f4 = 4+(delx/(c*delt*gfac))**2 */
+  f[4] = t * t;                    /* f5 */
+}
+
+int or2d_fstate_init(const or2d_params *p, or2d_fstate *st) {
+  size_t n6 = 6u * (size_t)(NX + 4) * (size_t)(NY + 4);
+  size_t n3 = 3u * (size_t)(NX + 4) * (size_t)(NY + 4);
+  st->df = (double *)calloc(n6, sizeof(double));
+  st->gkl = (double *)calloc(3u * (size_t)NX * (size_t)NY, sizeof(double));
+  st->uj = (double *)calloc(n3, sizeof(double));
+  st->cg_iters[0] = st->cg_iters[1] = st->cg_iters[2] = 0;
+  if (!st->df || !st->gkl || !st->uj) return set_err("oracle: out of memory");
+  return 0;
+}
+
+void or2d_fstate_free(or2d_fstate *st) {
+  free(st->df);
+  free(st->gkl);
+  free(st->uj);
+  st->df = st->gkl = st->uj = NULL;
+}
+
+/* quadratic spline with nodes at cell centres (particle.f90:99-107) */
+static inline void shape3(double dh, double s[3]) {
+  s[0] = 0.5 * (0.5 - dh) * (0.5 - dh);
+  s[1] = 0.75 - dh * dh;
+  s[2] = 0.5 * (0.5 + dh) * (0.5 + dh);
+}
+
+/* ---------------------------------------------------------------------------
+ * particle__solv: 2d/common/particle.f90:48-179
+ * ------------------------------------------------------------------------- */
+void or2d_particle_solv(const or2d_params *p, double *gp, const double *up,
+                        const double *uf, const int *cumcnt) {
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int tw = NX + 2;
+  const double d_delx = 1. / p->delx;
+  const double delt = p->delt, c = p->c;
+  double *tmp = (double *)malloc(sizeof(double) * 6u * (size_t)tw * (size_t)(NY + 2));
+#define TMP(cc, i, j) tmp[(size_t)((cc)-1) + 6u * ((size_t)((i) - (nxs - 1)) + (size_t)tw * (size_t)((j) - (nys - 1)))]
+#define UF(cc, i, j) uf[iF6(p, cc, i, j)]
+
+  /* fields at (i+1/2, j+1/2): particle.f90:70-83 */
+#pragma omp parallel for
+  for (int j = nys - 1; j <= nye + 1; j++)
+    for (int i = nxs - 1; i <= nxe + 1; i++) {
+      TMP(1, i, j) = 0.5 * (+UF(1, i, j) + UF(1, i, j + 1));
+      TMP(2, i, j) = 0.5 * (+UF(2, i, j) + UF(2, i + 1, j));
+      TMP(3, i, j) = 0.25 * (+UF(3, i, j) + UF(3, i + 1, j) + UF(3, i, j + 1) + UF(3, i + 1, j + 1));
+      TMP(4, i, j) = 0.5 * (+UF(4, i, j) + UF(4, i + 1, j));
+      TMP(5, i, j) = 0.5 * (+UF(5, i, j) + UF(5, i, j + 1));
+      TMP(6, i, j) = UF(6, i, j);
+    }
+
+  /* particle.f90:85-171 */
+#pragma omp parallel for
+  for (int j = nys; j <= nye; j++)
+    for (int i = nxs; i <= nxe; i++)
+      for (int isp = 1; isp <= p->nsp; isp++) {
+        const double q = p->q[isp - 1], r = p->r[isp - 1];
+        const double fac1 = q / r * 0.5 * delt;
+        const double txxx = fac1 * fac1;
+        const double fac2 = q * delt / r;
+        for (int ii = cumcnt[iCC(p, i, j, isp)] + 1; ii <= cumcnt[iCC(p, i + 1, j, isp)]; ii++) {
+          const double *u = &up[iUP(p, 1, ii, j, isp)];
+          double *g = &gp[iUP(p, 1, ii, j, isp)];
+          double sx[3], sy[3];
+          shape3(u[0] * d_delx - 0.5 - i, sx);
+          shape3(u[1] * d_delx - 0.5 - j, sy);
+#define GATH(cc)                                                                                   \
+  (+(+TMP(cc, i - 1, j - 1) * sx[0] + TMP(cc, i, j - 1) * sx[1] + TMP(cc, i + 1, j - 1) * sx[2]) * sy[0] + \
+   (+TMP(cc, i - 1, j) * sx[0] + TMP(cc, i, j) * sx[1] + TMP(cc, i + 1, j) * sx[2]) * sy[1] +             \
+   (+TMP(cc, i - 1, j + 1) * sx[0] + TMP(cc, i, j + 1) * sx[1] + TMP(cc, i + 1, j + 1) * sx[2]) * sy[2])
+          const double bpx = GATH(1), bpy = GATH(2), bpz = GATH(3);
+          const double epx = GATH(4), epy = GATH(5), epz = GATH(6);
+#undef GATH
+          double uvm1 = u[2] + fac1 * epx;
+          double uvm2 = u[3] + fac1 * epy;
+          double uvm3 = u[4] + fac1 * epz;
+
+          double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+          const double igam = 1. / gam;
+          const double fac1r = fac1 * igam;
+          const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+
+          const double uvm4 = uvm1 + fac1r * (+uvm2 * bpz - uvm3 * bpy);
+          const double uvm5 = uvm2 + fac1r * (+uvm3 * bpx - uvm1 * bpz);
+          const double uvm6 = uvm3 + fac1r * (+uvm1 * bpy - uvm2 * bpx);
+
+          uvm1 = uvm1 + fac2r * (+uvm5 * bpz - uvm6 * bpy);
+          uvm2 = uvm2 + fac2r * (+uvm6 * bpx - uvm4 * bpz);
+          uvm3 = uvm3 + fac2r * (+uvm4 * bpy - uvm5 * bpx);
+
+          g[2] = uvm1 + fac1 * epx;
+          g[3] = uvm2 + fac1 * epy;
+          g[4] = uvm3 + fac1 * epz;
+
+          gam = 1.0 / sqrt(1.0 + (+g[2] * g[2] + g[3] * g[3] + g[4] * g[4]) / (c * c));
+
+          g[0] = u[0] + g[2] * delt * gam;
+          g[1] = u[1] + g[3] * delt * gam;
+        }
+      }
+#undef TMP
+#undef UF
+  free(tmp);
+
+  /* particle.f90:173-177: id slot copied over the full capacity, as raw bits */
+  if (p->ndim == 6) {
+    size_t n = (size_t)p->np * (size_t)NY * (size_t)p->nsp;
+#pragma omp parallel for
+    for (size_t k = 0; k < n; k++) memcpy(&gp[6 * k + 5], &up[6 * k + 5], sizeof(double));
+  }
+}
+
+/* ---------------------------------------------------------------------------
+ * ele_cur (Esirkepov density decomposition): 2d/common/field.f90:189-316
+ * ------------------------------------------------------------------------- */
+static void ele_cur_cell(const or2d_params *p, double *uj, const double *up, const double *gp,
+                         const int *cumcnt, int i, int j) {
+  const double fac = 1.0 / 3.0;
+  const double d_delx = 1. / p->delx, d_delt = 1. / p->delt, delx = p->delx, c = p->c;
+  double pjx[5][5], pjy[5][5], pjz[5][5], pjtmp[5][5]; /* [jp+2][ip+2] */
+  memset(pjx, 0, sizeof pjx);
+  memset(pjy, 0, sizeof pjy);
+  memset(pjz, 0, sizeof pjz);
+  for (int isp = 1; isp <= p->nsp; isp++) {
+    const double q = p->q[isp - 1];
+    for (int ii = cumcnt[iCC(p, i, j, isp)] + 1; ii <= cumcnt[iCC(p, i + 1, j, isp)]; ii++) {
+      const double *u = &up[iUP(p, 1, ii, j, isp)];
+      const double *g = &gp[iUP(p, 1, ii, j, isp)];
+      double s0[2][5], ds[2][5], sh[3];
+      double dh;
+      dh = u[0] * d_delx - 0.5 - i;
+      shape3(dh, sh);
+      s0[0][0] = 0.0; s0[0][1] = sh[0]; s0[0][2] = sh[1]; s0[0][3] = sh[2]; s0[0][4] = 0.0;
+      dh = u[1] * d_delx - 0.5 - j;
+      shape3(dh, sh);
+      s0[1][0] = 0.0; s0[1][1] = sh[0]; s0[1][2] = sh[1]; s0[1][3] = sh[2]; s0[1][4] = 0.0;
+
+      for (int d = 0; d < 2; d++) {
+        const int cell = d == 0 ? i : j;
+        const int i2 = (int)(g[d] * d_delx);
+        dh = g[d] * d_delx - 0.5 - i2;
+        const int inc = i2 - cell;
+        const double s1_1 = 0.5 * (0.5 - dh) * (0.5 - dh);
+        const double s1_2 = 0.75 - dh * dh;
+        const double s1_3 = 0.5 * (0.5 + dh) * (0.5 + dh);
+        /* smo_1 = -(inc-abs(inc))*0.5+0 etc.: exactly 0 or 1, never -0 */
+        const double smo_1 = (inc - abs(inc)) != 0 ? (double)(-(float)(inc - abs(inc)) * 0.5f) + 0.0 : 0.0;
+        const double smo_2 = (double)(-abs(inc) + 1);
+        const double smo_3 = (double)((float)(inc + abs(inc)) * 0.5f + 0.0f);
+        ds[d][0] = s1_1 * smo_1;
+        ds[d][1] = s1_1 * smo_2 + s1_2 * smo_1;
+        ds[d][2] = s1_2 * smo_2 + s1_3 * smo_1 + s1_1 * smo_3;
+        ds[d][3] = s1_3 * smo_2 + s1_2 * smo_3;
+        ds[d][4] = s1_3 * smo_3;
+      }
+      for (int d = 0; d < 2; d++)
+        for (int k = 0; k < 5; k++) ds[d][k] = ds[d][k] - s0[d][k];
+
+      const double gvz = g[4] / sqrt(1. + (+g[2] * g[2] + g[3] * g[3] + g[4] * g[4]) / (c * c));
+
+      memset(pjtmp, 0, sizeof pjtmp);
+      for (int jp = 0; jp < 5; jp++)
+        for (int ip = 0; ip < 4; ip++)
+          pjtmp[jp][ip + 1] = pjtmp[jp][ip] - q * delx * d_delt * ds[0][ip] * (s0[1][jp] + 0.5 * ds[1][jp]);
+      for (int jp = 0; jp < 5; jp++)
+        for (int ip = 0; ip < 5; ip++) pjx[jp][ip] = pjx[jp][ip] + pjtmp[jp][ip];
+
+      memset(pjtmp, 0, sizeof pjtmp);
+      for (int jp = 0; jp < 4; jp++)
+        for (int ip = 0; ip < 5; ip++)
+          pjtmp[jp + 1][ip] = pjtmp[jp][ip] - q * delx * d_delt * ds[1][jp] * (s0[0][ip] + 0.5 * ds[0][ip]);
+      for (int jp = 0; jp < 5; jp++)
+        for (int ip = 0; ip < 5; ip++) pjy[jp][ip] = pjy[jp][ip] + pjtmp[jp][ip];
+
+      for (int jp = 0; jp < 5; jp++)
+        for (int ip = 0; ip < 5; ip++)
+          pjz[jp][ip] = pjz[jp][ip] + q * gvz *
+                                          (+s0[0][ip] * s0[1][jp] + 0.5 * ds[0][ip] * s0[1][jp] +
+                                           0.5 * s0[0][ip] * ds[1][jp] + fac * ds[0][ip] * ds[1][jp]);
+    }
+  }
+  for (int jp = -2; jp <= 2; jp++)
+    for (int ip = -2; ip <= 2; ip++) {
+      uj[iF3(p, 1, i + ip, j + jp)] = uj[iF3(p, 1, i + ip, j + jp)] + pjx[jp + 2][ip + 2];
+      uj[iF3(p, 2, i + ip, j + jp)] = uj[iF3(p, 2, i + ip, j + jp)] + pjy[jp + 2][ip + 2];
+      uj[iF3(p, 3, i + ip, j + jp)] = uj[iF3(p, 3, i + ip, j + jp)] + pjz[jp + 2][ip + 2];
+    }
+}
+
+void or2d_ele_cur(const or2d_params *p, double *uj, const double *up, const double *gp,
+                  const int *cumcnt) {
+  const size_t n3 = 3u * (size_t)(NX + 4) * (size_t)(NY + 4);
+  memset(uj, 0, n3 * sizeof(double));
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  if (nth <= 1) {
+    for (int j = p->nys; j <= p->nye; j++)
+      for (int i = p->nxgs; i <= p->nxge; i++) ele_cur_cell(p, uj, up, gp, cumcnt, i, j);
+    return;
+  }
+  /* OMP REDUCTION(+:uj): private copies summed at the end */
+#pragma omp parallel
+  {
+    double *priv = (double *)calloc(n3, sizeof(double));
+#pragma omp for schedule(static)
+    for (int j = p->nys; j <= p->nye; j++)
+      for (int i = p->nxgs; i <= p->nxge; i++) ele_cur_cell(p, priv, up, gp, cumcnt, i, j);
+#pragma omp critical
+    for (size_t k = 0; k < n3; k++) uj[k] = uj[k] + priv[k];
+    free(priv);
+  }
+}
+
+/* ---------------------------------------------------------------------------
+ * Boundary routines at a single rank (nup = ndown = self).
+ * ------------------------------------------------------------------------- */
+
+/* boundary_periodic__curre: boundary_periodic.f90:357-508 (y part shared with
+ * boundary_reconnection__curre, which has no x fold: :364-502) */
+void or2d_bc_curre(const or2d_params *p, double *uj) {
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int w = nxe - nxs + 5;
+  double *snd = (double *)malloc(sizeof(double) * 6u * (size_t)w);
+#define UJ(cc, i, j) uj[iF3(p, cc, i, j)]
+  /* send rows nys-2, nys-1 to rank-1 (self) and add into nye-1, nye */
+  for (int i = nxs - 2; i <= nxe + 2; i++)
+    for (int cc = 1; cc <= 3; cc++) {
+      snd[6 * (i - (nxs - 2)) + cc - 1] = UJ(cc, i, nys - 2);
+      snd[6 * (i - (nxs - 2)) + cc + 2] = UJ(cc, i, nys - 1);
+    }
+  for (int i = nxs - 2; i <= nxe + 2; i++)
+    for (int cc = 1; cc <= 3; cc++) {
+      UJ(cc, i, nye - 1) = UJ(cc, i, nye - 1) + snd[6 * (i - (nxs - 2)) + cc - 1];
+      UJ(cc, i, nye) = UJ(cc, i, nye) + snd[6 * (i - (nxs - 2)) + cc + 2];
+    }
+  /* send rows nye+1, nye+2 to rank+1 (self) and add into nys, nys+1 */
+  for (int i = nxs - 2; i <= nxe + 2; i++)
+    for (int cc = 1; cc <= 3; cc++) {
+      snd[6 * (i - (nxs - 2)) + cc - 1] = UJ(cc, i, nye + 1);
+      snd[6 * (i - (nxs - 2)) + cc + 2] = UJ(cc, i, nye + 2);
+    }
+  for (int i = nxs - 2; i <= nxe + 2; i++)
+    for (int cc = 1; cc <= 3; cc++) {
+      UJ(cc, i, nys) = UJ(cc, i, nys) + snd[6 * (i - (nxs - 2)) + cc - 1];
+      UJ(cc, i, nys + 1) = UJ(cc, i, nys + 1) + snd[6 * (i - (nxs - 2)) + cc + 2];
+    }
+  /* margins: rows nys, nys+1 -> nye+1, nye+2; rows nye-1, nye -> nys-2, nys-1 */
+  for (int i = nxs - 2; i <= nxe + 2; i++)
+    for (int cc = 1; cc <= 3; cc++) {
+      snd[6 * (i - (nxs - 2)) + cc - 1] = UJ(cc, i, nys);
+      snd[6 * (i - (nxs - 2)) + cc + 2] = UJ(cc, i, nys + 1);
+    }
+  for (int i = nxs - 2; i <= nxe + 2; i++)
+    for (int cc = 1; cc <= 3; cc++) {
+      UJ(cc, i, nye + 1) = snd[6 * (i - (nxs - 2)) + cc - 1];
+      UJ(cc, i, nye + 2) = snd[6 * (i - (nxs - 2)) + cc + 2];
+    }
+  for (int i = nxs - 2; i <= nxe + 2; i++)
+    for (int cc = 1; cc <= 3; cc++) {
+      snd[6 * (i - (nxs - 2)) + cc - 1] = UJ(cc, i, nye - 1);
+      snd[6 * (i - (nxs - 2)) + cc + 2] = UJ(cc, i, nye);
+    }
+  for (int i = nxs - 2; i <= nxe + 2; i++)
+    for (int cc = 1; cc <= 3; cc++) {
+      UJ(cc, i, nys - 2) = snd[6 * (i - (nxs - 2)) + cc - 1];
+      UJ(cc, i, nys - 1) = snd[6 * (i - (nxs - 2)) + cc + 2];
+    }
+  free(snd);
+  if (p->bc == OR_BC_WALLX) return;
+  /* x fold then x copy over all rows (boundary_periodic.f90:495-506) */
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 3; cc++) UJ(cc, nxe - 1, j) = UJ(cc, nxe - 1, j) + UJ(cc, nxs - 2, j);
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 3; cc++) UJ(cc, nxe, j) = UJ(cc, nxe, j) + UJ(cc, nxs - 1, j);
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 3; cc++) UJ(cc, nxs, j) = UJ(cc, nxs, j) + UJ(cc, nxe + 1, j);
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 3; cc++) UJ(cc, nxs + 1, j) = UJ(cc, nxs + 1, j) + UJ(cc, nxe + 2, j);
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 3; cc++) {
+      UJ(cc, nxs - 2, j) = UJ(cc, nxe - 1, j);
+    }
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 3; cc++) UJ(cc, nxs - 1, j) = UJ(cc, nxe, j);
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 3; cc++) UJ(cc, nxe + 1, j) = UJ(cc, nxs, j);
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 3; cc++) UJ(cc, nxe + 2, j) = UJ(cc, nxs + 1, j);
+#undef UJ
+}
+
+/* boundary_periodic__dfield: boundary_periodic.f90:251-354;
+ * boundary_reconnection__dfield x walls: boundary_reconnection.f90:350-359 */
+void or2d_bc_dfield(const or2d_params *p, double *df) {
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int w = nxe - nxs + 1;
+  double *snd = (double *)malloc(sizeof(double) * 12u * (size_t)w);
+#define DF(cc, i, j) df[iF6(p, cc, i, j)]
+  for (int i = nxs; i <= nxe; i++)
+    for (int cc = 1; cc <= 6; cc++) {
+      snd[12 * (i - nxs) + cc - 1] = DF(cc, i, nys);
+      snd[12 * (i - nxs) + cc + 5] = DF(cc, i, nys + 1);
+    }
+  for (int i = nxs; i <= nxe; i++)
+    for (int cc = 1; cc <= 6; cc++) {
+      DF(cc, i, nye + 1) = snd[12 * (i - nxs) + cc - 1];
+      DF(cc, i, nye + 2) = snd[12 * (i - nxs) + cc + 5];
+    }
+  for (int i = nxs; i <= nxe; i++)
+    for (int cc = 1; cc <= 6; cc++) {
+      snd[12 * (i - nxs) + cc - 1] = DF(cc, i, nye - 1);
+      snd[12 * (i - nxs) + cc + 5] = DF(cc, i, nye);
+    }
+  for (int i = nxs; i <= nxe; i++)
+    for (int cc = 1; cc <= 6; cc++) {
+      DF(cc, i, nys - 2) = snd[12 * (i - nxs) + cc - 1];
+      DF(cc, i, nys - 1) = snd[12 * (i - nxs) + cc + 5];
+    }
+  free(snd);
+  if (p->bc == OR_BC_WALLX) {
+    for (int j = nys - 2; j <= nye + 2; j++) {
+      DF(1, nxs - 1, j) = -DF(1, nxs, j);
+      for (int cc = 2; cc <= 4; cc++) DF(cc, nxs - 1, j) = DF(cc, nxs + 1, j);
+      for (int cc = 5; cc <= 6; cc++) DF(cc, nxs - 1, j) = -DF(cc, nxs, j);
+      DF(1, nxe, j) = -DF(1, nxe - 1, j);
+      for (int cc = 2; cc <= 4; cc++) DF(cc, nxe + 1, j) = DF(cc, nxe - 1, j);
+      for (int cc = 5; cc <= 6; cc++) DF(cc, nxe, j) = -DF(cc, nxe - 1, j);
+    }
+    return;
+  }
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 6; cc++) DF(cc, nxs - 2, j) = DF(cc, nxe - 1, j);
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 6; cc++) DF(cc, nxs - 1, j) = DF(cc, nxe, j);
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 6; cc++) DF(cc, nxe + 1, j) = DF(cc, nxs, j);
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int cc = 1; cc <= 6; cc++) DF(cc, nxe + 2, j) = DF(cc, nxs + 1, j);
+#undef DF
+}
+
+/* boundary_periodic__phi: boundary_periodic.f90:511-568;
+ * boundary_reconnection__phi walls: boundary_reconnection.f90:557-577.
+ * phi(nxs-1:nxe+1, nys-1:nye+1) */
+void or2d_bc_phi(const or2d_params *p, double *phi, int l) {
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int pw = NX + 2;
+#define PH(i, j) phi[(size_t)((i) - (nxs - 1)) + (size_t)pw * (size_t)((j) - (nys - 1))]
+  double *snd = (double *)malloc(sizeof(double) * (size_t)NX);
+  for (int i = nxs; i <= nxe; i++) snd[i - nxs] = PH(i, nys);
+  for (int i = nxs; i <= nxe; i++) PH(i, nye + 1) = snd[i - nxs];
+  for (int i = nxs; i <= nxe; i++) snd[i - nxs] = PH(i, nye);
+  for (int i = nxs; i <= nxe; i++) PH(i, nys - 1) = snd[i - nxs];
+  free(snd);
+  if (p->bc == OR_BC_WALLX) {
+    if (l == 1) {
+      for (int j = nys - 1; j <= nye + 1; j++) {
+        PH(nxs - 1, j) = -PH(nxs, j);
+        PH(nxe + 1, j) = -PH(nxe - 2, j);
+      }
+    } else {
+      for (int j = nys - 1; j <= nye + 1; j++) {
+        PH(nxs - 1, j) = PH(nxs + 1, j);
+        PH(nxe + 1, j) = PH(nxe - 1, j);
+      }
+    }
+    return;
+  }
+  for (int j = nys - 1; j <= nye + 1; j++) PH(nxs - 1, j) = PH(nxe, j);
+  for (int j = nys - 1; j <= nye + 1; j++) PH(nxe + 1, j) = PH(nxs, j);
+#undef PH
+}
+
+/* ---------------------------------------------------------------------------
+ * cgm: 2d/common/field.f90:319-461.  Serial dot products in (j, i) order, as
+ * the reference computes them at one thread.
+ * ------------------------------------------------------------------------- */
+int or2d_cgm(const or2d_params *p, or2d_fstate *st) {
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int ite_max = 100;
+  const double err = 1e-6;
+  double f[5];
+  or2d_field_consts(p, f);
+  const double f4 = f[3], f5 = f[4];
+  const int pw = NX + 2;
+  double *phi = (double *)calloc((size_t)pw * (size_t)(NY + 2), sizeof(double));
+  double *pp = (double *)calloc((size_t)pw * (size_t)(NY + 2), sizeof(double));
+  double *r = (double *)calloc((size_t)NX * (size_t)NY, sizeof(double));
+  double *b = (double *)calloc((size_t)NX * (size_t)NY, sizeof(double));
+  double *ap = (double *)calloc((size_t)NX * (size_t)NY, sizeof(double));
+#define PH(a, i, j) a[(size_t)((i) - (nxs - 1)) + (size_t)pw * (size_t)((j) - (nys - 1))]
+#define IN(a, i, j) a[(size_t)((i) - nxs) + (size_t)NX * (size_t)((j) - nys)]
+#define DB(l, i, j) st->df[iF6(p, l, i, j)]
+  int rc = 0;
+  for (int l = 1; l <= 3; l++) {
+    int ite = 0;
+    double sum = 0.0;
+    for (int j = nys; j <= nye; j++)
+      for (int i = nxs; i <= nxe; i++) {
+        PH(phi, i, j) = DB(l, i, j);
+        IN(b, i, j) = f5 * st->gkl[iGK(p, l, i, j)];
+        sum = sum + IN(b, i, j) * IN(b, i, j);
+      }
+    double sum_g = sum;
+    const double eps = sqrt(sum_g) * err;
+    or2d_bc_phi(p, phi, l);
+    double sumr = 0.0;
+    for (int j = nys; j <= nye; j++)
+      for (int i = nxs; i <= nxe; i++) {
+        IN(r, i, j) = IN(b, i, j) + PH(phi, i, j - 1) + PH(phi, i - 1, j) - f4 * PH(phi, i, j) +
+                      PH(phi, i + 1, j) + PH(phi, i, j + 1);
+        PH(pp, i, j) = IN(r, i, j);
+        sumr = sumr + IN(r, i, j) * IN(r, i, j);
+      }
+    double sumr_g = sumr;
+    if (sqrt(sumr_g) > eps) {
+      while (sum_g > eps) {
+        ite = ite + 1;
+        or2d_bc_phi(p, pp, l);
+        double s_r = 0.0, sum2 = 0.0;
+        for (int j = nys; j <= nye; j++)
+          for (int i = nxs; i <= nxe; i++) {
+            IN(ap, i, j) = -PH(pp, i, j - 1) - PH(pp, i - 1, j) + f4 * PH(pp, i, j) - PH(pp, i + 1, j) -
+                           PH(pp, i, j + 1);
+            s_r = s_r + IN(r, i, j) * IN(r, i, j);
+            sum2 = sum2 + PH(pp, i, j) * IN(ap, i, j);
+          }
+        sumr_g = s_r;
+        const double sum2_g = sum2;
+        const double av = sumr_g / sum2_g;
+        for (int j = nys; j <= nye; j++)
+          for (int i = nxs; i <= nxe; i++) {
+            PH(phi, i, j) = PH(phi, i, j) + av * PH(pp, i, j);
+            IN(r, i, j) = IN(r, i, j) - av * IN(ap, i, j);
+          }
+        sum_g = sqrt(sumr_g);
+        if (ite >= ite_max) {
+          set_err("********** stop at cgm after ite_max **********");
+          rc = -1;
+          break;
+        }
+        double sum1 = 0.0;
+        for (int j = nys; j <= nye; j++)
+          for (int i = nxs; i <= nxe; i++) sum1 = sum1 + IN(r, i, j) * IN(r, i, j);
+        const double bv = sum1 / sumr_g;
+        for (int j = nys; j <= nye; j++)
+          for (int i = nxs; i <= nxe; i++) PH(pp, i, j) = IN(r, i, j) + bv * PH(pp, i, j);
+      }
+    }
+    st->cg_iters[l - 1] = ite;
+    if (rc) break;
+    for (int j = nys; j <= nye; j++)
+      for (int i = nxs; i <= nxe; i++) DB(l, i, j) = PH(phi, i, j);
+  }
+#undef PH
+#undef IN
+#undef DB
+  free(phi);
+  free(pp);
+  free(r);
+  free(b);
+  free(ap);
+  return rc;
+}
+
+/* ---------------------------------------------------------------------------
+ * field__fdtd_i: 2d/common/field.f90:66-186
+ * ------------------------------------------------------------------------- */
+int or2d_field_fdtd_i(const or2d_params *p, or2d_fstate *st, double *uf, const double *up,
+                      const double *gp, const int *cumcnt) {
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const double pi = M_PI, delt = p->delt, gfac = p->gfac;
+  double f[5];
+  or2d_field_consts(p, f);
+  const double f1 = f[0], f2 = f[1], f3 = f[2];
+  double *uj = st->uj, *df = st->df;
+
+  or2d_ele_cur(p, uj, up, gp, cumcnt);
+  or2d_bc_curre(p, uj);
+
+#define UF(cc, i, j) uf[iF6(p, cc, i, j)]
+#define UJ(cc, i, j) uj[iF3(p, cc, i, j)]
+#define DF(cc, i, j) df[iF6(p, cc, i, j)]
+#pragma omp parallel for
+  for (int j = nys; j <= nye; j++)
+    for (int i = nxs; i <= nxe; i++) {
+      st->gkl[iGK(p, 1, i, j)] =
+          +f2 * (+UF(1, i, j - 1) + UF(1, i - 1, j) - 4. * UF(1, i, j) + UF(1, i + 1, j) + UF(1, i, j + 1) +
+                 f3 * (-UJ(3, i, j - 1) + UJ(3, i, j))) -
+          f1 * (-UF(6, i, j - 1) + UF(6, i, j));
+      st->gkl[iGK(p, 2, i, j)] =
+          +f2 * (+UF(2, i, j - 1) + UF(2, i - 1, j) - 4. * UF(2, i, j) + UF(2, i + 1, j) + UF(2, i, j + 1) -
+                 f3 * (-UJ(3, i - 1, j) + UJ(3, i, j))) +
+          f1 * (-UF(6, i - 1, j) + UF(6, i, j));
+      st->gkl[iGK(p, 3, i, j)] =
+          +f2 * (+UF(3, i, j - 1) + UF(3, i - 1, j) - 4. * UF(3, i, j) + UF(3, i + 1, j) + UF(3, i, j + 1) +
+                 f3 * (-UJ(2, i - 1, j) + UJ(2, i, j) + UJ(1, i, j - 1) - UJ(1, i, j))) -
+          f1 * (-UF(5, i - 1, j) + UF(5, i, j) + UF(4, i, j - 1) - UF(4, i, j));
+    }
+
+  if (or2d_cgm(p, st)) return -1;
+
+  or2d_bc_dfield(p, df);
+
+#pragma omp parallel for
+  for (int j = nys; j <= nye; j++)
+    for (int i = nxs; i <= nxe; i++) {
+      DF(4, i, j) = +f1 * (+gfac * (-DF(3, i, j) + DF(3, i, j + 1)) + (-UF(3, i, j) + UF(3, i, j + 1))) -
+                    4. * pi * delt * UJ(1, i, j);
+      DF(5, i, j) = -f1 * (+gfac * (-DF(3, i, j) + DF(3, i + 1, j)) + (-UF(3, i, j) + UF(3, i + 1, j))) -
+                    4. * pi * delt * UJ(2, i, j);
+      DF(6, i, j) = +f1 * (+gfac * (-DF(2, i, j) + DF(2, i + 1, j) + DF(1, i, j) - DF(1, i, j + 1)) +
+                           (-UF(2, i, j) + UF(2, i + 1, j) + UF(1, i, j) - UF(1, i, j + 1))) -
+                    4. * pi * delt * UJ(3, i, j);
+    }
+
+  or2d_bc_dfield(p, df);
+
+#pragma omp parallel for
+  for (int j = nys - 2; j <= nye + 2; j++)
+    for (int i = nxs - 2; i <= nxe + 2; i++)
+      for (int ieq = 1; ieq <= 6; ieq++) UF(ieq, i, j) = UF(ieq, i, j) + DF(ieq, i, j);
+#undef UF
+#undef UJ
+#undef DF
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * boundary_periodic__particle_x (boundary_periodic.f90:61-96), round-down;
+ * boundary_reconnection__particle_x (boundary_reconnection.f90:61-99),
+ * reflecting walls, round-to-nearest.
+ * ------------------------------------------------------------------------- */
+void or2d_bc_particle_x(const or2d_params *p, double *up, const int *np2) {
+  const int nxgs = p->nxgs, nxge = p->nxge;
+  const double delx = p->delx;
+  for (int isp = 1; isp <= p->nsp; isp++) {
+#pragma omp parallel for
+    for (int j = p->nys; j <= p->nye; j++)
+      for (int ii = 1; ii <= np2[iNP(p, j, isp)]; ii++) {
+        double *u = &up[iUP(p, 1, ii, j, isp)];
+        if (p->bc == OR_BC_WALLX) {
+          const int nxs = nxgs, nxe = nxge;
+          const int ipos = (int)(u[0] / delx);
+          if (ipos < nxs + 1) {
+            u[0] = (double)(2.0f * (float)(nxs + 1)) * delx - u[0];
+            u[2] = -u[2];
+            u[3] = -u[3];
+            u[4] = -u[4];
+          } else if (ipos >= nxe - 1) {
+            u[0] = (double)(2.0f * (float)(nxe - 1)) * delx - u[0];
+            u[2] = -u[2];
+            u[3] = -u[3];
+            u[4] = -u[4];
+          }
+        } else {
+          const int ipos = (int)or_div_rd(u[0], delx);
+          const double span = or_mul_rd((double)(nxge - nxgs + 1), delx);
+          if (ipos < nxgs)
+            u[0] = or_add_rd(u[0], span);
+          else if (ipos >= nxge + 1)
+            u[0] = or_add_rd(u[0], -span);
+        }
+      }
+  }
+}
+
+/* ---------------------------------------------------------------------------
+ * boundary_periodic__particle_y: boundary_periodic.f90:99-248, at one rank
+ * and one thread (the row-bucket fill order is then deterministic).
+ * ------------------------------------------------------------------------- */
+int or2d_bc_particle_y(const or2d_params *p, double *up, int *np2) {
+  const int nys = p->nys, nye = p->nye, nygs = p->nygs, nyge = p->nyge;
+  const int ndim = p->ndim, np = p->np;
+  const double delx = p->delx;
+  const int nrow = NY + 2; /* buckets nys-1 .. nye+1 */
+  int *flag = (int *)malloc(sizeof(int) * (size_t)np * (size_t)NY);
+  double *bff = (double *)malloc(sizeof(double) * (size_t)ndim * (size_t)np * (size_t)nrow);
+  int *cnt = (int *)malloc(sizeof(int) * (size_t)nrow);
+  int *cnt2 = (int *)malloc(sizeof(int) * (size_t)NY);
+  int rc = 0;
+  if (!flag || !bff || !cnt || !cnt2) {
+    rc = set_err("oracle: out of memory in bc_particle_y");
+    goto done;
+  }
+#define FLAG(k, j) flag[(size_t)((k)-1) + (size_t)np * (size_t)((j)-nys)]
+#define BFF(k, j) bff[(size_t)((k)-1) + (size_t)ndim * (size_t)np * (size_t)((j) - (nys - 1))]
+#define CNT(j) cnt[(j) - (nys - 1)]
+#define CNT2(j) cnt2[(j)-nys]
+  for (int isp = 1; isp <= p->nsp; isp++) {
+    for (int j = nys - 1; j <= nye + 1; j++) CNT(j) = 0;
+    for (int j = nys; j <= nye; j++) CNT2(j) = 0;
+    const double span = or_mul_rd((double)(nyge - nygs + 1), delx);
+    for (int j = nys; j <= nye; j++)
+      for (int ii = 1; ii <= np2[iNP(p, j, isp)]; ii++) {
+        double *u = &up[iUP(p, 1, ii, j, isp)];
+        const int jpos = (int)or_div_rd(u[1], delx);
+        if (jpos != j) {
+          if (jpos <= nygs - 1)
+            u[1] = or_add_rd(u[1], span);
+          else if (jpos >= nyge + 1)
+            u[1] = or_add_rd(u[1], -span);
+          if (jpos < nys - 1 || jpos > nye + 1 || CNT(jpos) >= np) {
+            rc = set_err("oracle: particle moved more than one row");
+            goto done;
+          }
+          memcpy(&BFF(1 + ndim * CNT(jpos), jpos), u, sizeof(double) * (size_t)ndim);
+          CNT(jpos) = CNT(jpos) + 1;
+          CNT2(j) = CNT2(j) + 1;
+          FLAG(CNT2(j), j) = ii;
+        }
+      }
+    /* transfer to rank-1 (self): bucket nys-1 appended to bucket nye */
+    {
+      const int cnt_tmp = CNT(nys - 1);
+      memmove(&BFF(ndim * CNT(nye) + 1, nye), &BFF(1, nys - 1), sizeof(double) * (size_t)ndim * (size_t)cnt_tmp);
+      CNT(nye) = CNT(nye) + cnt_tmp;
+    }
+    /* transfer to rank+1 (self): bucket nye+1 appended to bucket nys */
+    {
+      const int cnt_tmp = CNT(nye + 1);
+      memmove(&BFF(ndim * CNT(nys) + 1, nys), &BFF(1, nye + 1), sizeof(double) * (size_t)ndim * (size_t)cnt_tmp);
+      CNT(nys) = CNT(nys) + cnt_tmp;
+    }
+    for (int j = nys; j <= nye; j++) {
+      int *n2 = &np2[iNP(p, j, isp)];
+      int iii = 0;
+      int cnt_tmp = CNT2(j);
+      for (int ii = 1; ii <= CNT2(j); ii++) {
+        if (CNT(j) == 0) {
+          if (*n2 < FLAG(ii, j)) break;
+          int out = 0;
+          while (*n2 == FLAG(cnt_tmp, j)) {
+            *n2 = *n2 - 1;
+            if (*n2 < FLAG(ii, j)) {
+              out = 1;
+              break;
+            }
+            cnt_tmp = cnt_tmp - 1;
+          }
+          if (out) break;
+          memcpy(&up[iUP(p, 1, FLAG(ii, j), j, isp)], &up[iUP(p, 1, *n2, j, isp)], sizeof(double) * (size_t)ndim);
+          *n2 = *n2 - 1;
+        } else {
+          memcpy(&up[iUP(p, 1, FLAG(ii, j), j, isp)], &BFF(1 + ndim * iii, j), sizeof(double) * (size_t)ndim);
+          iii = iii + 1;
+          CNT(j) = CNT(j) - 1;
+        }
+      }
+      if (CNT(j) > 0) {
+        if (*n2 + CNT(j) > np) {
+          rc = set_err("memory over (np2 > np)");
+          goto done;
+        }
+        for (int ii = 1; ii <= CNT(j); ii++)
+          memcpy(&up[iUP(p, 1, *n2 + ii, j, isp)], &BFF(ndim * iii + 1 + ndim * (ii - 1), j),
+                 sizeof(double) * (size_t)ndim);
+      }
+    }
+    for (int j = nys; j <= nye; j++) {
+      np2[iNP(p, j, isp)] = np2[iNP(p, j, isp)] + CNT(j);
+      if (np2[iNP(p, j, isp)] > np) {
+        rc = set_err("memory over (np2 > np)");
+        goto done;
+      }
+    }
+  }
+#undef FLAG
+#undef BFF
+#undef CNT
+#undef CNT2
+done:
+  free(flag);
+  free(bff);
+  free(cnt);
+  free(cnt2);
+  return rc;
+}
+
+/* ---------------------------------------------------------------------------
+ * sort__bucket: 2d/common/sort.f90:36-82 (cell = int(x), no 1/delx)
+ * ------------------------------------------------------------------------- */
+void or2d_sort_bucket(const or2d_params *p, double *gp, const double *up, int *cumcnt, const int *np2) {
+  const int nxs = p->nxgs, nxe = p->nxge;
+  for (int isp = 1; isp <= p->nsp; isp++) {
+#pragma omp parallel for
+    for (int j = p->nys; j <= p->nye; j++) {
+      int *cnt = (int *)calloc((size_t)(nxe - nxs + 1), sizeof(int));
+      int *sum_cnt = (int *)calloc((size_t)(nxe - nxs + 2), sizeof(int));
+      const int n = np2[iNP(p, j, isp)];
+      for (int ii = 1; ii <= n; ii++) {
+        const int i = (int)up[iUP(p, 1, ii, j, isp)];
+        cnt[i - nxs] = cnt[i - nxs] + 1;
+      }
+      sum_cnt[0] = 0;
+      cumcnt[iCC(p, nxs, j, isp)] = 0;
+      for (int i = nxs + 1; i <= nxe + 1; i++) {
+        sum_cnt[i - nxs] = sum_cnt[i - 1 - nxs] + cnt[i - 1 - nxs];
+        cumcnt[iCC(p, i, j, isp)] = sum_cnt[i - nxs];
+      }
+      for (int ii = 1; ii <= n; ii++) {
+        const int i = (int)up[iUP(p, 1, ii, j, isp)];
+        memcpy(&gp[iUP(p, 1, sum_cnt[i - nxs] + 1, j, isp)], &up[iUP(p, 1, ii, j, isp)],
+               sizeof(double) * (size_t)p->ndim);
+        sum_cnt[i - nxs] = sum_cnt[i - nxs] + 1;
+      }
+      free(cnt);
+      free(sum_cnt);
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------------
+ * One or more steps in the app's call order.
+ * ------------------------------------------------------------------------- */
+int or2d_step(const or2d_params *p, or2d_fstate *st, double *up, double *gp, double *uf, int *cumcnt,
+              int *np2, int nsteps) {
+  for (int it = 0; it < nsteps; it++) {
+    or2d_particle_solv(p, gp, up, uf, cumcnt);
+    if (p->bc == OR_BC_WALLX) {
+      or2d_bc_particle_x(p, gp, np2);
+      if (or2d_field_fdtd_i(p, st, uf, up, gp, cumcnt)) return -1;
+    } else {
+      if (or2d_field_fdtd_i(p, st, uf, up, gp, cumcnt)) return -1;
+      or2d_bc_particle_x(p, gp, np2);
+    }
+    if (or2d_bc_particle_y(p, gp, np2)) return -1;
+    or2d_sort_bucket(p, up, gp, cumcnt, np2);
+  }
+  return 0;
+}
+
+/* energy_history sums: 2d/proj/weibel/app.f90:503-534 (serial order) */
+void or2d_energy(const or2d_params *p, const double *up, const double *uf, const int *np2, double out[3]) {
+  const double pi = M_PI, c = p->c;
+  double kin = 0.0;
+  for (int isp = 1; isp <= p->nsp; isp++) {
+    double vene = 0.0;
+    for (int j = p->nys; j <= p->nye; j++)
+      for (int ii = 1; ii <= np2[iNP(p, j, isp)]; ii++) {
+        const double *u = &up[iUP(p, 1, ii, j, isp)];
+        const double u2 = u[2] * u[2] + u[3] * u[3] + u[4] * u[4];
+        const double gam = sqrt(1 + u2 / (c * c));
+        vene = vene + p->r[isp - 1] * (gam - 1);
+      }
+    kin = kin + vene;
+  }
+  double efield = 0.0, bfield = 0.0;
+  for (int j = p->nys; j <= p->nye; j++)
+    for (int i = p->nxgs; i <= p->nxge; i++) {
+      const double *f = &uf[iF6(p, 1, i, j)];
+      bfield = bfield + f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+      efield = efield + f[3] * f[3] + f[4] * f[4] + f[5] * f[5];
+    }
+  out[0] = kin;
+  out[1] = efield / (8 * pi);
+  out[2] = bfield / (8 * pi);
+}
+
+/* Gauss residual at cell centres with the deposit's quadratic shape */
+void or2d_gauss(const or2d_params *p, double *G, const double *uf, const double *up, const int *np2) {
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const double pi = M_PI, d_delx = 1. / p->delx;
+  const int gw = NX + 4;
+  double *rho = (double *)calloc((size_t)gw * (size_t)(NY + 4), sizeof(double));
+#define RHO(i, j) rho[(size_t)((i) - (nxs - 2)) + (size_t)gw * (size_t)((j) - (nys - 2))]
+  for (int isp = 1; isp <= p->nsp; isp++)
+    for (int j = nys; j <= nye; j++)
+      for (int ii = 1; ii <= np2[iNP(p, j, isp)]; ii++) {
+        const double *u = &up[iUP(p, 1, ii, j, isp)];
+        const int i0 = (int)(u[0] * d_delx), j0 = (int)(u[1] * d_delx);
+        double sx[3], sy[3];
+        shape3(u[0] * d_delx - 0.5 - i0, sx);
+        shape3(u[1] * d_delx - 0.5 - j0, sy);
+        for (int b = 0; b < 3; b++)
+          for (int a = 0; a < 3; a++) RHO(i0 - 1 + a, j0 - 1 + b) += p->q[isp - 1] * sx[a] * sy[b];
+      }
+  if (p->bc == OR_BC_PERIODIC) {
+    /* fold the one-cell halo periodically */
+    for (int i = nxs - 1; i <= nxe + 1; i++) {
+      RHO(i, nye) += RHO(i, nys - 1);
+      RHO(i, nys) += RHO(i, nye + 1);
+    }
+    for (int j = nys; j <= nye; j++) {
+      RHO(nxe, j) += RHO(nxs - 1, j);
+      RHO(nxs, j) += RHO(nxe + 1, j);
+    }
+  }
+  for (int j = nys; j <= nye; j++)
+    for (int i = nxs; i <= nxe; i++) {
+      const double dive = (uf[iF6(p, 4, i + 1, j)] - uf[iF6(p, 4, i, j)]) * d_delx +
+                          (uf[iF6(p, 5, i, j + 1)] - uf[iF6(p, 5, i, j)]) * d_delx;
+      G[(size_t)(i - nxs) + (size_t)NX * (size_t)(j - nys)] = dive - 4.0 * pi * RHO(i, j);
+    }
+#undef RHO
+  free(rho);
+}

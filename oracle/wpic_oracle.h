@@ -1,0 +1,120 @@
+/*
+ * wpic_oracle.h -- CPU restatement of WumingPIC's per-timestep particle loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product path (the HIP library,
+ * the Fortran shim, bench.py's timed region) may include, link or call this.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+ * as the checker and as the timed CPU baseline.
+ *
+ * Every function restates one reference routine, in the reference's own
+ * operation order (Fortran left-to-right evaluation, parentheses respected,
+ * no FMA contraction: build with -ffp-contract=off), on arrays in the
+ * reference's own Fortran (column-major) layout:
+ *
+ *   up(ndim, np, nys:nye, nsp)        particles, padded rows of capacity np
+ *   uf(6, nxgs-2:nxge+2, nys-2:nye+2) fields Bx By Bz Ex Ey Ez
+ *   uj(3, nxgs-2:nxge+2, nys-2:nye+2) current Jx Jy Jz
+ *   cumcnt(nxgs:nxge+1, nys:nye, nsp) in-row cell offsets, np2(nys:nye, nsp)
+ *
+ * Pinning status of each function (what checks it, see tests/):
+ *   or2d_particle_solv    bit-exact vs the reference particle.f90 built with
+ *                         amdflang (oracle/_ref), plus closed forms.
+ *   or2d_sort_bucket      bit-exact vs the reference sort.f90 (oracle/_ref),
+ *                         plus brute-force histogram / stable permutation.
+ *   or2d_ele_cur          discrete continuity (rho^{n+1}-rho^n+dt div J = 0),
+ *                         single-particle brute force, J=0 at rest.
+ *   or2d_field_fdtd_i     CG solution vs an exact FFT solve of the periodic
+ *                         operator; vacuum wave energy; div B / Gauss law.
+ *   or2d_bc_*             brute-force periodic indexing on tiny grids, the
+ *                         round-down wrap edge case, wall sign tables.
+ *   or2d_bc_particle_y    particle multiset / id conservation, row membership;
+ *                         the exact in-row order after hole filling is
+ *                         "parity unpinned" (only a reference build with MPI
+ *                         could pin it, and MPI is absent from the image).
+ */
+#ifndef WPIC_ORACLE_H
+#define WPIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAXSP 4
+
+/* boundary kinds: boundary_periodic (2d/common/boundary_periodic.f90) and
+ * boundary_reconnection (2d/proj/reconnection/boundary_reconnection.f90) */
+enum { OR_BC_PERIODIC = 0, OR_BC_WALLX = 1 };
+
+typedef struct {
+  int ndim, np, nsp;
+  int nxgs, nxge, nygs, nyge, nys, nye;
+  double delx, delt, c, gfac;
+  double q[OR_MAXSP], r[OR_MAXSP];
+  int bc;
+} or2d_params;
+
+/* persistent module state of field.f90 (df, gkl, uj are `save`d) */
+typedef struct {
+  double *df;  /* df(6, nxgs-2:nxge+2, nys-2:nye+2) */
+  double *gkl; /* gkl(3, nxgs:nxge, nys:nye) */
+  double *uj;  /* uj(3, nxgs-2:nxge+2, nys-2:nye+2) */
+  int cg_iters[3];
+} or2d_fstate;
+
+int or2d_fstate_init(const or2d_params *p, or2d_fstate *st);
+void or2d_fstate_free(or2d_fstate *st);
+
+void or_set_threads(int n);
+const char *or_last_error(void);
+
+/* field.f90:22-63 */
+void or2d_field_consts(const or2d_params *p, double f[5]);
+/* particle.f90:48-179 */
+void or2d_particle_solv(const or2d_params *p, double *gp, const double *up,
+                        const double *uf, const int *cumcnt);
+/* field.f90:189-316 */
+void or2d_ele_cur(const or2d_params *p, double *uj, const double *up,
+                  const double *gp, const int *cumcnt);
+/* boundary_periodic.f90:357-508 / boundary_reconnection.f90:364-502 */
+void or2d_bc_curre(const or2d_params *p, double *uj);
+/* boundary_periodic.f90:251-354 / boundary_reconnection.f90:254-361 */
+void or2d_bc_dfield(const or2d_params *p, double *df);
+/* boundary_periodic.f90:511-568 / boundary_reconnection.f90:505-579 */
+void or2d_bc_phi(const or2d_params *p, double *phi, int l);
+/* field.f90:66-186 (calls ele_cur, bc_curre, cgm, bc_dfield) */
+int or2d_field_fdtd_i(const or2d_params *p, or2d_fstate *st, double *uf,
+                      const double *up, const double *gp, const int *cumcnt);
+/* field.f90:319-461 on st->df, st->gkl */
+int or2d_cgm(const or2d_params *p, or2d_fstate *st);
+/* boundary_periodic.f90:61-96 / boundary_reconnection.f90:61-99 */
+void or2d_bc_particle_x(const or2d_params *p, double *up, const int *np2);
+/* boundary_periodic.f90:99-248 (identical in boundary_reconnection.f90) */
+int or2d_bc_particle_y(const or2d_params *p, double *up, int *np2);
+/* sort.f90:36-82 */
+void or2d_sort_bucket(const or2d_params *p, double *gp, const double *up,
+                      int *cumcnt, const int *np2);
+/* one step in the app's order: 2d/proj/weibel/app.f90:102-107 (periodic) or
+ * 2d/proj/reconnection/app.f90:101-106 (walls) */
+int or2d_step(const or2d_params *p, or2d_fstate *st, double *up, double *gp,
+              double *uf, int *cumcnt, int *np2, int nsteps);
+/* energy_history sums: 2d/proj/weibel/app.f90:503-534.
+ * out = {kinetic(isp=1..nsp summed), E^2/8pi, B^2/8pi} */
+void or2d_energy(const or2d_params *p, const double *up, const double *uf,
+                 const int *np2, double out[3]);
+/* Gauss-law residual G(i,j) = div E - 4 pi rho at cell centres, rho with the
+ * same quadratic shape as the deposit (s0 x s0).  G(nxgs:nxge, nys:nye). */
+void or2d_gauss(const or2d_params *p, double *G, const double *uf,
+                const double *up, const int *np2);
+
+/* round-toward-minus-infinity arithmetic used by the periodic wraps
+ * (ieee_set_rounding_mode(ieee_down), boundary_periodic.f90:74,124) */
+double or_add_rd(double a, double b);
+double or_mul_rd(double a, double b);
+double or_div_rd(double a, double b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,63 @@
+"""The C-ABI library loads on a CPU-only host and exports every entry point
+include/wpic.h declares (no compute calls here: there is no GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "wpic.h")
+LIB = os.path.join(ROOT, "wumingpic_amd", "libwpic.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wpic_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_built():
+    assert os.path.exists(LIB), "build with `python -m wumingpic_amd.build` (or __graft_entry__.build())"
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared()
+    assert len(names) >= 25
+    lib = C.CDLL(LIB)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (wpic_[a-z0-9_]+)", out))
+    assert set(names) <= exported
+
+
+def test_python_binding_covers_the_header():
+    from wumingpic_amd import _lib
+    assert sorted(_lib.EXPORTS) == declared()
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    from wumingpic_amd import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", os.path.join(ROOT, "no-such-lib.so"))
+    with pytest.raises(ImportError):
+        _lib.lib()
+
+
+def test_invalid_parameters_rejected_without_gpu_work():
+    """parameter validation happens before any device call"""
+    from wumingpic_amd import _lib
+    L = _lib.lib()
+    p = _lib.Params2D()
+    p.ndim, p.nsp = 5, 2
+    h = C.c_void_p()
+    assert L.wpic_create2d(C.byref(h), C.byref(p)) == _lib.E_INVAL
+    assert b"ndim" in L.wpic_last_error()
+    p.ndim = 6
+    p.nxgs, p.nxge, p.nygs, p.nyge = 2, 65, 2, 65
+    p.delx, p.delt, p.c, p.gfac = 1.0, 1.5, 1.0, 0.501
+    p.capacity = 1000
+    assert L.wpic_create2d(C.byref(h), C.byref(p)) == _lib.E_INVAL
+    assert b"c*delt" in L.wpic_last_error()

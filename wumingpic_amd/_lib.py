@@ -1,0 +1,104 @@
+"""ctypes binding of libwpic.so (the C-ABI declared in include/wpic.h).
+
+The library is loaded from this package directory only; there is no CPU
+fallback: importing the bindings without the built library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwpic.so")
+
+MAXSP = 4
+BC_PERIODIC, BC_WALLX = 0, 1
+MODE_FAST, MODE_PARITY = 0, 1
+E_OK, E_INVAL, E_HIP, E_CG, E_MEM, E_DOMAIN = 0, -1, -2, -3, -4, -5
+
+# every symbol include/wpic.h declares
+EXPORTS = (
+    "wpic_create2d", "wpic_destroy", "wpic_last_error", "wpic_set_mode", "wpic_sync",
+    "wpic_upload_up2d", "wpic_download_up2d", "wpic_upload_gp2d", "wpic_download_gp2d",
+    "wpic_upload_fields2d", "wpic_download_fields2d", "wpic_download_current2d",
+    "wpic_download_dfield2d", "wpic_upload_dfield2d",
+    "wpic_particle_solv2d", "wpic_field_fdtd_i2d", "wpic_bc_particle_x2d", "wpic_bc_particle_y2d",
+    "wpic_sort_bucket2d", "wpic_cg_iters2d", "wpic_step2d",
+    "wpic_energy2d", "wpic_gauss2d", "wpic_counts2d", "wpic_ic_beams2d",
+    "wpic_set_timing", "wpic_timing",
+)
+
+
+class Params2D(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int), ("nsp", C.c_int),
+        ("nxgs", C.c_int), ("nxge", C.c_int), ("nygs", C.c_int), ("nyge", C.c_int),
+        ("delx", C.c_double), ("delt", C.c_double), ("c", C.c_double), ("gfac", C.c_double),
+        ("q", C.c_double * MAXSP), ("r", C.c_double * MAXSP),
+        ("bc", C.c_int), ("mode", C.c_int), ("capacity", C.c_longlong),
+    ]
+
+
+class WpicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libwpic.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m wumingpic_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    d = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+    i = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+    P = C.POINTER(Params2D)
+    sig = {
+        "wpic_create2d": [C.POINTER(vp), P],
+        "wpic_destroy": [vp],
+        "wpic_set_mode": [vp, C.c_int],
+        "wpic_sync": [vp],
+        "wpic_upload_up2d": [vp, d, i, i, C.c_int],
+        "wpic_download_up2d": [vp, d, i, i, C.c_int],
+        "wpic_upload_gp2d": [vp, d, i, C.c_int],
+        "wpic_download_gp2d": [vp, d, i, C.c_int],
+        "wpic_upload_fields2d": [vp, d],
+        "wpic_download_fields2d": [vp, d],
+        "wpic_download_current2d": [vp, d],
+        "wpic_download_dfield2d": [vp, d],
+        "wpic_upload_dfield2d": [vp, d],
+        "wpic_particle_solv2d": [vp],
+        "wpic_field_fdtd_i2d": [vp],
+        "wpic_bc_particle_x2d": [vp],
+        "wpic_bc_particle_y2d": [vp],
+        "wpic_sort_bucket2d": [vp],
+        "wpic_cg_iters2d": [vp, np.ctypeslib.ndpointer(np.int32)],
+        "wpic_step2d": [vp, C.c_int],
+        "wpic_energy2d": [vp, np.ctypeslib.ndpointer(np.float64)],
+        "wpic_gauss2d": [vp, d],
+        "wpic_counts2d": [vp, np.ctypeslib.ndpointer(np.int64)],
+        "wpic_ic_beams2d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
+        "wpic_set_timing": [vp, C.c_int],
+        "wpic_timing": [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.wpic_last_error.restype = C.c_char_p
+    L.wpic_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise WpicError(rc, lib().wpic_last_error().decode())
+    return rc

@@ -1,0 +1,320 @@
+// k_misc2d.hip -- layout conversion at the boundary (reference AoS <-> device
+// SoA), the seeded initial state, and diagnostics (energy sums, Gauss law).
+#include "wpic_internal.h"
+
+namespace wpic {
+
+// ---------------------------------------------------------------------------
+// particles: compact AoS rows (ndim = 6 doubles per particle, the reference's
+// up(:, ii, j, isp) contiguous) <-> SoA, over all rows of A (which = 0) or B
+// (which = 1).  The AoS buffer holds species after species, rows in order.
+// ---------------------------------------------------------------------------
+struct XferArgs {
+  Grid2D g;
+  Parts P;
+  uint64_t* id;
+  const int *rowbase, *np2;
+  const int* R;  // non-null: B seen through the exchange permutation
+  const long long* aosbase;  // [nsp*ny] start of each row in the AoS buffer
+};
+
+__global__ void k_aos_to_soa(XferArgs a, const double* __restrict__ aos) {
+  const int rw = blockIdx.x;
+  const int base = a.rowbase[rw], n = a.np2[rw];
+  const double* src = aos + a.aosbase[rw] * 6;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const double* s = src + (size_t)k * 6;
+    const int d = base + k;
+    a.P.x[d] = s[0];
+    a.P.y[d] = s[1];
+    a.P.ux[d] = s[2];
+    a.P.uy[d] = s[3];
+    a.P.uz[d] = s[4];
+    uint64_t b;
+    __builtin_memcpy(&b, &s[5], 8);
+    a.id[d] = b;
+  }
+}
+
+__global__ void k_soa_to_aos(XferArgs a, double* __restrict__ aos) {
+  const int rw = blockIdx.x;
+  const int base = a.rowbase[rw], n = a.np2[rw];
+  double* dst = aos + a.aosbase[rw] * 6;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    int s = a.R ? a.R[base + k] : base + k;
+    if ((unsigned)s >= (unsigned)(a.g.ncap * a.g.nsp)) s = 0;
+    double* d = dst + (size_t)k * 6;
+    d[0] = a.P.x[s];
+    d[1] = a.P.y[s];
+    d[2] = a.P.ux[s];
+    d[3] = a.P.uy[s];
+    d[4] = a.P.uz[s];
+    const uint64_t b = a.id[s];
+    __builtin_memcpy(&d[5], &b, 8);
+  }
+}
+
+// fields: reference uf(ncomp, i, j) AoS <-> planes
+__global__ void k_planes_from_aos(int n, int ncomp, const double* __restrict__ aos, double* __restrict__ pl) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (int c = 0; c < ncomp; c++) pl[(size_t)c * n + k] = aos[(size_t)k * ncomp + c];
+}
+
+__global__ void k_planes_to_aos(int n, int ncomp, const double* __restrict__ pl, double* __restrict__ aos) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (int c = 0; c < ncomp; c++) aos[(size_t)k * ncomp + c] = pl[(size_t)c * n + k];
+}
+
+hipError_t launch_planes_from_aos(wpic_ctx* c, const double* aos, double* planes, int ncomp) {
+  const int n = c->g.fw * c->g.fh;
+  hipLaunchKernelGGL(k_planes_from_aos, dim3((n + 255) / 256), dim3(256), 0, c->st, n, ncomp, aos, planes);
+  return hipGetLastError();
+}
+
+hipError_t launch_planes_to_aos(wpic_ctx* c, const double* planes, double* aos, int ncomp) {
+  const int n = c->g.fw * c->g.fh;
+  hipLaunchKernelGGL(k_planes_to_aos, dim3((n + 255) / 256), dim3(256), 0, c->st, n, ncomp, planes, aos);
+  return hipGetLastError();
+}
+
+// which: 0 = A rows, 1 = B rows (through R when the exchange is pending),
+// 2 = B rows in B's own order.  aosbase lives in the staging tail.
+hipError_t launch_up_to_soa(wpic_ctx* c, const double* aos, long long n, int which) {
+  (void)n;
+  XferArgs a;
+  a.g = c->g;
+  a.P = which == 0 ? c->A : c->B;
+  a.id = which == 0 ? c->id[c->ida] : c->id[c->idb];
+  a.rowbase = which == 0 ? c->rowbaseA : c->rowbaseB;
+  a.np2 = which == 0 ? c->np2A : c->np2B;
+  a.R = nullptr;
+  a.aosbase = c->aosbase;
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, aos);
+  return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(wpic_ctx* c, double* aos, int which) {
+  XferArgs a;
+  a.g = c->g;
+  a.P = which == 0 ? c->A : c->B;
+  a.id = which == 0 ? c->id[c->ida] : c->id[c->idb];
+  if (which == 0) {
+    a.rowbase = c->rowbaseA;
+    a.np2 = c->np2A;
+    a.R = nullptr;
+  } else if (which == 1) {
+    a.rowbase = c->rowbaseN;
+    a.np2 = c->np2N;
+    a.R = c->R;
+  } else {
+    a.rowbase = c->rowbaseB;
+    a.np2 = c->np2B;
+    a.R = nullptr;
+  }
+  a.aosbase = c->aosbase;
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, aos);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// seeded initial state: counter-based (splitmix64) draws keyed by
+// (seed, stream, row, cell, slot), so the state is independent of the launch
+// shape; Box-Muller normals.
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t smix(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__device__ inline uint64_t rkey(uint64_t seed, uint64_t stream, uint64_t a, uint64_t b, uint64_t c) {
+  return smix(smix(smix(smix(seed ^ 0x5851f42d4c957f2dull) ^ stream) ^ a) ^ ((b << 20) ^ c));
+}
+__device__ inline double u01(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
+__device__ inline void normal2(uint64_t h1, uint64_t h2, double& n0, double& n1) {
+  const double u1 = 1.0 - u01(h1);  // (0, 1]
+  const double u2 = u01(h2);
+  const double rr = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  n0 = rr * c;
+  n1 = rr * s;
+}
+
+struct IcArgs {
+  Grid2D g;
+  Parts A;
+  uint64_t* id;
+  unsigned long long seed;
+  int n_ppc;
+  double udrift, uth_e, uth_i;
+};
+
+__global__ void k_ic_beams(IcArgs a) {
+  const Grid2D& g = a.g;
+  const long long per = (long long)a.n_ppc * g.nx * g.ny;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= per * g.nsp) return;
+  const int s = (int)(t / per);
+  const long long k = t - s * per;
+  const long long rowlen = (long long)a.n_ppc * g.nx;
+  const int r = (int)(k / rowlen);
+  const long long m = k - r * rowlen;
+  const int ci = (int)(m / a.n_ppc), sl = (int)(m - (long long)ci * a.n_ppc);
+  const long long d = s * g.ncap + k;
+  // positions shared by all species (co-located: zero charge density)
+  const double u0 = u01(rkey(a.seed, 1, r, ci, sl)), u1 = u01(rkey(a.seed, 2, r, ci, sl));
+  const int i = g.nxgs + ci, j = g.nys + r;
+  double x = ((double)i + u0) * g.delx, y = ((double)j + u1) * g.delx;
+  if ((int)(x * g.d_delx) != i) x = next_down((double)(i + 1) * g.delx);
+  if ((int)(y * g.d_delx) != j) y = next_down((double)(j + 1) * g.delx);
+  a.A.x[d] = x;
+  a.A.y[d] = y;
+  double n0, n1, n2, n3;
+  normal2(rkey(a.seed, 10 + 4 * s, r, ci, sl), rkey(a.seed, 11 + 4 * s, r, ci, sl), n0, n1);
+  normal2(rkey(a.seed, 12 + 4 * s, r, ci, sl), rkey(a.seed, 13 + 4 * s, r, ci, sl), n2, n3);
+  const bool electron = s == g.nsp - 1 && g.nsp > 1;
+  const double th = electron ? a.uth_e : a.uth_i;
+  double ux = th * n0;
+  if (electron) ux += (sl & 1) ? -a.udrift : a.udrift;
+  a.A.ux[d] = ux;
+  a.A.uy[d] = th * n1;
+  a.A.uz[d] = th * n2;
+  // id = -(pid), pid counted row by row from 1 (app.f90:455-470)
+  const long long pid = k + 1;
+  a.id[d] = (uint64_t)(-pid);
+}
+
+hipError_t launch_ic_beams(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
+                           double uth_i) {
+  IcArgs a;
+  a.g = c->g;
+  a.A = c->A;
+  a.id = c->id[c->ida];
+  a.seed = seed;
+  a.n_ppc = n_ppc;
+  a.udrift = udrift;
+  a.uth_e = uth_e;
+  a.uth_i = uth_i;
+  const long long n = (long long)n_ppc * c->g.nx * c->g.ny * c->g.nsp;
+  hipLaunchKernelGGL(k_ic_beams, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------
+constexpr int RB = 256;   // reduction blocks
+__device__ inline double bsum(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return t;
+}
+
+// out[blk*3 + {0,1,2}] = partial {kinetic, E^2, B^2}
+__global__ __launch_bounds__(256) void k_energy(Grid2D g, Parts A, const int* rowbase, const int* np2,
+                                                const double* uf, double* out) {
+  __shared__ double sh[4];
+  double kin = 0.0, e2 = 0.0, b2 = 0.0;
+  const int nrow = g.nsp * g.ny;
+  for (int rw = blockIdx.x; rw < nrow; rw += gridDim.x) {
+    const int s = rw / g.ny;
+    const int base = rowbase[rw], n = np2[rw];
+    for (int k = threadIdx.x; k < n; k += 256) {
+      const int p = base + k;
+      const double u2 = A.ux[p] * A.ux[p] + A.uy[p] * A.uy[p] + A.uz[p] * A.uz[p];
+      const double gam = sqrt(1 + u2 / (g.c * g.c));
+      kin = kin + g.r[s] * (gam - 1);
+    }
+  }
+  const size_t pl = (size_t)g.fw * g.fh;
+  const int ncell = g.nx * g.ny;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < ncell; k += gridDim.x * 256) {
+    const int f = fidx(g, g.nxgs + k % g.nx, g.nys + k / g.nx);
+    b2 = b2 + uf[f] * uf[f] + uf[pl + f] * uf[pl + f] + uf[2 * pl + f] * uf[2 * pl + f];
+    e2 = e2 + uf[3 * pl + f] * uf[3 * pl + f] + uf[4 * pl + f] * uf[4 * pl + f] + uf[5 * pl + f] * uf[5 * pl + f];
+  }
+  kin = bsum(kin, sh);
+  e2 = bsum(e2, sh);
+  b2 = bsum(b2, sh);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x * 3 + 0] = kin;
+    out[blockIdx.x * 3 + 1] = e2;
+    out[blockIdx.x * 3 + 2] = b2;
+  }
+}
+
+hipError_t launch_energy(wpic_ctx* c, double* dout) {
+  hipLaunchKernelGGL(k_energy, dim3(RB), dim3(256), 0, c->st, c->g, c->A, c->rowbaseA, c->np2A, c->uf, dout);
+  return hipGetLastError();
+}
+
+// rho with the quadratic shape at cell centres, then G = div E - 4 pi rho
+__global__ __launch_bounds__(256) void k_rho(Grid2D g, Parts A, const int* rowbase, const int* np2, double* rho) {
+  const int rw = blockIdx.x;
+  const int s = rw / g.ny;
+  const int base = rowbase[rw], n = np2[rw];
+  const int gw = g.fw;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    const int p = base + k;
+    const double x = A.x[p], y = A.y[p];
+    const int i0 = (int)(x * g.d_delx), j0 = (int)(y * g.d_delx);
+    if (i0 < g.nxgs || i0 > g.nxge || j0 < g.nys || j0 > g.nye) continue;
+    double sx[3], sy[3];
+    shape3(x * g.d_delx - 0.5 - i0, sx[0], sx[1], sx[2]);
+    shape3(y * g.d_delx - 0.5 - j0, sy[0], sy[1], sy[2]);
+    for (int b = 0; b < 3; b++)
+      for (int a = 0; a < 3; a++)
+        unsafeAtomicAdd(&rho[(i0 - 1 + a - (g.nxgs - 2)) + gw * (j0 - 1 + b - (g.nys - 2))],
+                        g.q[s] * sx[a] * sy[b]);
+  }
+}
+
+__global__ void k_rho_fold_y(Grid2D g, double* rho) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = g.nxgs - 1 + t;
+  if (I > g.nxge + 1) return;
+  rho[fidx(g, I, g.nye)] += rho[fidx(g, I, g.nys - 1)];
+  rho[fidx(g, I, g.nys)] += rho[fidx(g, I, g.nye + 1)];
+}
+
+__global__ void k_rho_fold_x(Grid2D g, double* rho) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = g.nys + t;
+  if (J > g.nye) return;
+  rho[fidx(g, g.nxge, J)] += rho[fidx(g, g.nxgs - 1, J)];
+  rho[fidx(g, g.nxgs, J)] += rho[fidx(g, g.nxge + 1, J)];
+}
+
+__global__ void k_gauss(Grid2D g, const double* uf, const double* rho, double* G) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.nx * g.ny) return;
+  const int i = g.nxgs + k % g.nx, j = g.nys + k / g.nx;
+  const size_t pl = (size_t)g.fw * g.fh;
+  const double* ex = uf + 3 * pl;
+  const double* ey = uf + 4 * pl;
+  const double dive = (ex[fidx(g, i + 1, j)] - ex[fidx(g, i, j)]) * g.d_delx +
+                      (ey[fidx(g, i, j + 1)] - ey[fidx(g, i, j)]) * g.d_delx;
+  G[k] = dive - 4.0 * 3.141592653589793 * rho[fidx(g, i, j)];
+}
+
+hipError_t launch_gauss(wpic_ctx* c, double* G, double* rho) {
+  const size_t n = (size_t)c->g.fw * c->g.fh;
+  hipError_t e = launch_zero(c, rho, n);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_rho, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, c->g, c->A, c->rowbaseA, c->np2A, rho);
+  if (c->g.bc == WPIC_BC_PERIODIC) {
+    hipLaunchKernelGGL(k_rho_fold_y, dim3((c->g.nx + 2 + 255) / 256), dim3(256), 0, c->st, c->g, rho);
+    hipLaunchKernelGGL(k_rho_fold_x, dim3((c->g.ny + 255) / 256), dim3(256), 0, c->st, c->g, rho);
+  }
+  hipLaunchKernelGGL(k_gauss, dim3((c->g.nx * c->g.ny + 255) / 256), dim3(256), 0, c->st, c->g, c->uf, rho, G);
+  return hipGetLastError();
+}
+
+}  // namespace wpic

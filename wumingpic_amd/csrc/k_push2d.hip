@@ -1,0 +1,388 @@
+// k_push2d.hip -- Buneman-Boris push (2d/common/particle.f90:48-179) and
+// Esirkepov charge-conserving deposit (2d/common/field.f90:189-316) on tiles.
+//
+// One workgroup owns a TX x TY tile of cells.  It stages the cell-centred
+// fields of the tile plus a one-cell halo in LDS (computed from uf as
+// particle.f90:70-83 does) and, in fast mode, a current accumulator for the
+// tile plus a two-cell halo.  Lane groups of G lanes walk one cell at a time
+// (particles of a cell are contiguous, sorted by cell); each lane keeps the
+// cell's 65 current weights in registers, the group reduces them once per cell
+// and adds them to LDS; the tile is added to HBM with one atomic per node.
+//
+// Parity mode (G = 1): one lane per cell accumulates in the reference's slot
+// order and writes its 5x5x3 block to a per-cell buffer; k_gather_pj then adds
+// the blocks in the reference's (j, i) cell order, reproducing uj bit for bit.
+//
+// All arithmetic is written in the reference's evaluation order and the
+// library is built with -ffp-contract=off, so the push is bit-identical to
+// the reference at any thread count.
+#include "wpic_internal.h"
+
+namespace wpic {
+
+constexpr int TX = 32, TY = 4, NT = 256;
+constexpr int SFW = TX + 2, SFH = TY + 2;  // staged fields: one-cell halo
+constexpr int ACW = TX + 4, ACH = TY + 4;  // current accumulator: two-cell halo
+
+struct PushArgs {
+  Grid2D g;
+  Parts A, B;
+  const int* rowbase;
+  const int* cumcnt;
+  const double* uf;
+  double* uj;
+  double* pjbuf;
+  int* code;
+  int* err;
+  int ntx, nty;
+};
+
+// cell-centred field component c at (i, j) (particle.f90:74-80)
+__device__ inline double cell_field(const Grid2D& g, const double* uf, int c, int i, int j) {
+  const size_t pl = (size_t)g.fw * g.fh;
+  const double* u = uf + pl * c;
+  switch (c) {
+    case 0: return 0.5 * (+u[fidx(g, i, j)] + u[fidx(g, i, j + 1)]);
+    case 1: return 0.5 * (+u[fidx(g, i, j)] + u[fidx(g, i + 1, j)]);
+    case 2:
+      return 0.25 * (+u[fidx(g, i, j)] + u[fidx(g, i + 1, j)] + u[fidx(g, i, j + 1)] + u[fidx(g, i + 1, j + 1)]);
+    case 3: return 0.5 * (+u[fidx(g, i, j)] + u[fidx(g, i + 1, j)]);
+    case 4: return 0.5 * (+u[fidx(g, i, j)] + u[fidx(g, i, j + 1)]);
+    default: return u[fidx(g, i, j)];
+  }
+}
+
+// register layout of one cell's current block:
+//   [0,20)  pjx(ip,jp), ip = -1..2, jp = -2..2  -> (jp+2)*4 + ip+1
+//   [20,40) pjy(ip,jp), ip = -2..2, jp = -1..2  -> 20 + (jp+1)*5 + ip+2
+//   [40,65) pjz(ip,jp)                          -> 40 + (jp+2)*5 + ip+2
+// pjx(-2,.) and pjy(.,-2) are identically +0 in the reference and adding +0
+// to a never-negative-zero sum is the identity, so they are not carried.
+constexpr int NPJ = 65;
+
+template <int G, bool PUSH, bool DEPOSIT, bool FUSED, bool PARITY, int BC>
+__global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
+  __shared__ double sf[6][SFW * SFH];
+  __shared__ double acc[(DEPOSIT && !PARITY) ? 3 : 1][(DEPOSIT && !PARITY) ? ACW * ACH : 1];
+
+  const Grid2D& g = a.g;
+  const int ntiles = a.ntx * a.nty;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  const int i0 = g.nxgs + (tile % a.ntx) * TX;
+  const int j0 = g.nys + (tile / a.ntx) * TY;
+  const int tid = threadIdx.x;
+
+  if (PUSH) {
+    for (int k = tid; k < 6 * SFW * SFH; k += NT) {
+      const int c = k / (SFW * SFH), rem = k - c * (SFW * SFH);
+      const int i = i0 - 1 + rem % SFW, j = j0 - 1 + rem / SFW;
+      if (i <= g.nxge + 1 && j <= g.nye + 1) sf[c][rem] = cell_field(g, a.uf, c, i, j);
+    }
+  }
+  if (DEPOSIT && !PARITY) {
+    for (int k = tid; k < 3 * ACW * ACH; k += NT) (&acc[0][0])[k] = 0.0;
+  }
+  __syncthreads();
+
+  const int grp = tid / G, lg = tid % G;
+  constexpr int NGRP = NT / G;
+  const double d_delx = g.d_delx, delt = g.delt, c = g.c;
+
+  for (int cc = grp; cc < TX * TY; cc += NGRP) {
+    const int li = cc % TX, lj = cc / TX;
+    const int i = i0 + li, j = j0 + lj;
+    if (i > g.nxge || j > g.nye) continue;  // uniform within a group
+
+    double pj[DEPOSIT ? NPJ : 1];
+    if (DEPOSIT) {
+#pragma unroll
+      for (int k = 0; k < NPJ; k++) pj[k] = 0.0;
+    }
+
+    for (int s = 0; s < g.nsp; s++) {
+      const int rw = s * g.ny + (j - g.nys);
+      const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
+      const int base = a.rowbase[rw];
+      const int pbeg = base + cr[i - g.nxgs], pend = base + cr[i - g.nxgs + 1];
+      const double q = g.q[s], rm = g.r[s];
+      const double fac1 = q / rm * 0.5 * delt;
+      const double txxx = fac1 * fac1;
+      const double fac2 = q * delt / rm;
+      const double qk = q * g.delx * g.d_delt;
+
+      for (int pi = pbeg + lg; pi < pend; pi += G) {
+        const double x = a.A.x[pi], y = a.A.y[pi];
+        double nxp, nyp, g3, g4, g5;
+        if (PUSH) {
+          double sx0, sx1, sx2, sy0, sy1, sy2;
+          shape3(x * d_delx - 0.5 - i, sx0, sx1, sx2);
+          shape3(y * d_delx - 0.5 - j, sy0, sy1, sy2);
+          double fl[6];
+#pragma unroll
+          for (int cmp = 0; cmp < 6; cmp++) {
+            const double* f = &sf[cmp][(lj + 1) * SFW + (li + 1)];
+            fl[cmp] = +(+f[-SFW - 1] * sx0 + f[-SFW] * sx1 + f[-SFW + 1] * sx2) * sy0 +
+                      (+f[-1] * sx0 + f[0] * sx1 + f[1] * sx2) * sy1 +
+                      (+f[SFW - 1] * sx0 + f[SFW] * sx1 + f[SFW + 1] * sx2) * sy2;
+          }
+          const double bpx = fl[0], bpy = fl[1], bpz = fl[2], epx = fl[3], epy = fl[4], epz = fl[5];
+          double uvm1 = a.A.ux[pi] + fac1 * epx;
+          double uvm2 = a.A.uy[pi] + fac1 * epy;
+          double uvm3 = a.A.uz[pi] + fac1 * epz;
+          double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+          const double igam = 1. / gam;
+          const double fac1r = fac1 * igam;
+          const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+          const double uvm4 = uvm1 + fac1r * (+uvm2 * bpz - uvm3 * bpy);
+          const double uvm5 = uvm2 + fac1r * (+uvm3 * bpx - uvm1 * bpz);
+          const double uvm6 = uvm3 + fac1r * (+uvm1 * bpy - uvm2 * bpx);
+          uvm1 = uvm1 + fac2r * (+uvm5 * bpz - uvm6 * bpy);
+          uvm2 = uvm2 + fac2r * (+uvm6 * bpx - uvm4 * bpz);
+          uvm3 = uvm3 + fac2r * (+uvm4 * bpy - uvm5 * bpx);
+          g3 = uvm1 + fac1 * epx;
+          g4 = uvm2 + fac1 * epy;
+          g5 = uvm3 + fac1 * epz;
+          gam = 1.0 / sqrt(1.0 + (+g3 * g3 + g4 * g4 + g5 * g5) / (c * c));
+          nxp = x + g3 * delt * gam;
+          nyp = y + g4 * delt * gam;
+        } else {
+          nxp = a.B.x[pi];
+          nyp = a.B.y[pi];
+          g3 = a.B.ux[pi];
+          g4 = a.B.uy[pi];
+          g5 = a.B.uz[pi];
+        }
+
+        // reflecting walls act before the deposit in the reconnection order
+        // (2d/proj/reconnection/app.f90:101-103; boundary_reconnection.f90:79-91)
+        if (FUSED && BC == WPIC_BC_WALLX) {
+          const int ipos = (int)(nxp / g.delx);
+          if (ipos < g.nxgs + 1) {
+            nxp = g.wall_lo - nxp;
+            g3 = -g3; g4 = -g4; g5 = -g5;
+          } else if (ipos >= g.nxge - 1) {
+            nxp = g.wall_hi - nxp;
+            g3 = -g3; g4 = -g4; g5 = -g5;
+          }
+        }
+
+        if (DEPOSIT) {
+          double s0x[5], s0y[5], dsx[5], dsy[5];
+          {
+            double t0, t1, t2;
+            shape3(x * d_delx - 0.5 - i, t0, t1, t2);
+            s0x[0] = 0.0; s0x[1] = t0; s0x[2] = t1; s0x[3] = t2; s0x[4] = 0.0;
+            shape3(y * d_delx - 0.5 - j, t0, t1, t2);
+            s0y[0] = 0.0; s0y[1] = t0; s0y[2] = t1; s0y[3] = t2; s0y[4] = 0.0;
+          }
+#pragma unroll
+          for (int d = 0; d < 2; d++) {
+            const double gp = d == 0 ? nxp : nyp;
+            const int cell = d == 0 ? i : j;
+            const int i2 = (int)(gp * d_delx);
+            const double dh = gp * d_delx - 0.5 - i2;
+            const int inc = i2 - cell;
+            const double s1_1 = 0.5 * (0.5 - dh) * (0.5 - dh);
+            const double s1_2 = 0.75 - dh * dh;
+            const double s1_3 = 0.5 * (0.5 + dh) * (0.5 + dh);
+            // smo_1 = -(inc-abs(inc))*0.5+0 etc. (field.f90:244-246): 0 or 1
+            const double smo_1 = inc < 0 ? 1.0 : 0.0;
+            const double smo_2 = (double)(-abs(inc) + 1);
+            const double smo_3 = inc > 0 ? 1.0 : 0.0;
+            double* ds = d == 0 ? dsx : dsy;
+            const double* s0 = d == 0 ? s0x : s0y;
+            ds[0] = s1_1 * smo_1;
+            ds[1] = s1_1 * smo_2 + s1_2 * smo_1;
+            ds[2] = s1_2 * smo_2 + s1_3 * smo_1 + s1_1 * smo_3;
+            ds[3] = s1_3 * smo_2 + s1_2 * smo_3;
+            ds[4] = s1_3 * smo_3;
+#pragma unroll
+            for (int k = 0; k < 5; k++) ds[k] = ds[k] - s0[k];
+          }
+          const double gvz = g5 / sqrt(1. + (+g3 * g3 + g4 * g4 + g5 * g5) / (c * c));
+          const double qgvz = q * gvz;
+          const double fac = 1.0 / 3.0;
+          // pjx(ip+1,jp) = pjx(ip,jp) - q dx/dt ds_x(ip) (s0_y(jp) + ds_y(jp)/2)
+#pragma unroll
+          for (int jp = 0; jp < 5; jp++) {
+            const double wy = s0y[jp] + 0.5 * dsy[jp];
+            double run = 0.0;
+#pragma unroll
+            for (int ip = 0; ip < 4; ip++) {
+              run = run - qk * dsx[ip] * wy;
+              pj[jp * 4 + ip] = pj[jp * 4 + ip] + run;
+            }
+          }
+#pragma unroll
+          for (int ip = 0; ip < 5; ip++) {
+            const double wx = s0x[ip] + 0.5 * dsx[ip];
+            double run = 0.0;
+#pragma unroll
+            for (int jp = 0; jp < 4; jp++) {
+              run = run - qk * dsy[jp] * wx;
+              pj[20 + jp * 5 + ip] = pj[20 + jp * 5 + ip] + run;
+            }
+          }
+#pragma unroll
+          for (int jp = 0; jp < 5; jp++)
+#pragma unroll
+            for (int ip = 0; ip < 5; ip++)
+              pj[40 + jp * 5 + ip] =
+                  pj[40 + jp * 5 + ip] + qgvz * (+s0x[ip] * s0y[jp] + 0.5 * dsx[ip] * s0y[jp] +
+                                                 0.5 * s0x[ip] * dsy[jp] + fac * dsx[ip] * dsy[jp]);
+        }
+
+        if (FUSED) {
+          // boundary_periodic__particle_x / _y wraps, round-down (:74-88, :124-154)
+          if (BC == WPIC_BC_PERIODIC) {
+            const int ipos = (int)rd_div(nxp, g.delx);
+            if (ipos < g.nxgs)
+              nxp = rd_add(nxp, g.spanx);
+            else if (ipos >= g.nxge + 1)
+              nxp = rd_add(nxp, -g.spanx);
+          }
+          const int jpos = (int)rd_div(nyp, g.delx);
+          if (jpos != j) {
+            if (jpos <= g.nygs - 1)
+              nyp = rd_add(nyp, g.spany);
+            else if (jpos >= g.nyge + 1)
+              nyp = rd_add(nyp, -g.spany);
+          }
+          int ni = (int)nxp, dy = jpos - j;
+          if (ni < g.nxgs || ni > g.nxge || dy < -1 || dy > 1) {
+            atomicOr(a.err, ERR_DOMAIN);
+            ni = ni < g.nxgs ? g.nxgs : (ni > g.nxge ? g.nxge : ni);
+            dy = dy < -1 ? -1 : (dy > 1 ? 1 : dy);
+          }
+          a.code[pi] = ((ni - g.nxgs) << 2) | (dy + 1);
+        }
+        if (PUSH) {
+          a.B.x[pi] = nxp;
+          a.B.y[pi] = nyp;
+          a.B.ux[pi] = g3;
+          a.B.uy[pi] = g4;
+          a.B.uz[pi] = g5;
+        }
+      }
+    }
+
+    if (DEPOSIT) {
+      if (PARITY) {
+        double* dst = a.pjbuf + (size_t)cidx(g, i, j) * NPJ;
+#pragma unroll
+        for (int k = 0; k < NPJ; k++) dst[k] = pj[k];
+      } else {
+        if (G > 1) {
+#pragma unroll
+          for (int k = 0; k < NPJ; k++) {
+#pragma unroll
+            for (int off = G / 2; off >= 1; off >>= 1) pj[k] += __shfl_xor(pj[k], off, G);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < NPJ; k++) {
+          if ((k % G) != lg) continue;
+          int comp, ip, jp;
+          if (k < 20) {
+            comp = 0; jp = k / 4 - 2; ip = k % 4 - 1;
+          } else if (k < 40) {
+            comp = 1; jp = (k - 20) / 5 - 1; ip = (k - 20) % 5 - 2;
+          } else {
+            comp = 2; jp = (k - 40) / 5 - 2; ip = (k - 40) % 5 - 2;
+          }
+          atomicAdd(&acc[comp][(lj + 2 + jp) * ACW + (li + 2 + ip)], pj[k]);
+        }
+      }
+    }
+  }
+
+  if (DEPOSIT && !PARITY) {
+    __syncthreads();
+    const size_t pl = (size_t)g.fw * g.fh;
+    for (int k = tid; k < 3 * ACW * ACH; k += NT) {
+      const int comp = k / (ACW * ACH), rem = k - comp * (ACW * ACH);
+      const double v = acc[comp][rem];
+      if (v == 0.0) continue;
+      const int I = i0 - 2 + rem % ACW, J = j0 - 2 + rem / ACW;
+      if (I > g.nxge + 2 || J > g.nye + 2) continue;
+      unsafeAtomicAdd(a.uj + pl * comp + fidx(g, I, J), v);
+    }
+  }
+}
+
+// uj(c, I, J) = sum over source cells in the reference's (j, i) loop order of
+// their block entry (field.f90:212-310 at one thread), starting from +0.
+__global__ __launch_bounds__(256) void k_gather_pj(Grid2D g, const double* __restrict__ pjbuf,
+                                                  double* __restrict__ uj) {
+  const int n = g.fw * g.fh;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int I = g.nxgs - 2 + k % g.fw, J = g.nys - 2 + k / g.fw;
+  const size_t pl = (size_t)g.fw * g.fh;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  const int jlo = J - 2 > g.nys ? J - 2 : g.nys, jhi = J + 2 < g.nye ? J + 2 : g.nye;
+  const int ilo = I - 2 > g.nxgs ? I - 2 : g.nxgs, ihi = I + 2 < g.nxge ? I + 2 : g.nxge;
+  for (int j = jlo; j <= jhi; j++)
+    for (int i = ilo; i <= ihi; i++) {
+      const int ip = I - i, jp = J - j;
+      const double* b = pjbuf + (size_t)cidx(g, i, j) * NPJ;
+      if (ip >= -1) sx = sx + b[(jp + 2) * 4 + ip + 1];
+      if (jp >= -1) sy = sy + b[20 + (jp + 1) * 5 + ip + 2];
+      sz = sz + b[40 + (jp + 2) * 5 + ip + 2];
+    }
+  const int f = fidx(g, I, J);
+  uj[f] = sx;
+  uj[pl + f] = sy;
+  uj[2 * pl + f] = sz;
+}
+
+template <int G, bool PARITY, int BC>
+static hipError_t launch_t(wpic_ctx* c, bool push, bool deposit, bool fused, PushArgs& a, int nblk) {
+  dim3 grid(nblk), block(NT);
+#define L(P, D, F) hipLaunchKernelGGL((k_push_deposit<G, P, D, F, PARITY, BC>), grid, block, 0, c->st, a)
+  if (push && deposit && fused)
+    L(true, true, true);
+  else if (push && !deposit && !fused)
+    L(true, false, false);
+  else if (!push && deposit && !fused)
+    L(false, true, false);
+  else if (push && deposit && !fused)
+    L(true, true, false);
+  else
+    return hipErrorInvalidValue;
+#undef L
+  return hipGetLastError();
+}
+
+hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused) {
+  PushArgs a;
+  a.g = c->g;
+  a.A = c->A;
+  a.B = c->B;
+  a.rowbase = c->rowbaseA;
+  a.cumcnt = c->cumcnt;
+  a.uf = c->uf;
+  a.uj = c->uj;
+  a.pjbuf = c->pjbuf;
+  a.code = c->code;
+  a.err = c->err;
+  a.ntx = (c->g.nx + TX - 1) / TX;
+  a.nty = (c->g.ny + TY - 1) / TY;
+  const int nblk = xcd_grid(a.ntx * a.nty);
+  const bool parity = c->mode == WPIC_MODE_PARITY && deposit;
+  if (parity) {
+    return c->g.bc == WPIC_BC_WALLX ? launch_t<1, true, WPIC_BC_WALLX>(c, push, deposit, fused, a, nblk)
+                                    : launch_t<1, true, WPIC_BC_PERIODIC>(c, push, deposit, fused, a, nblk);
+  }
+  return c->g.bc == WPIC_BC_WALLX ? launch_t<8, false, WPIC_BC_WALLX>(c, push, deposit, fused, a, nblk)
+                                  : launch_t<8, false, WPIC_BC_PERIODIC>(c, push, deposit, fused, a, nblk);
+}
+
+hipError_t launch_gather_pj(wpic_ctx* c) {
+  const int n = c->g.fw * c->g.fh;
+  hipLaunchKernelGGL(k_gather_pj, dim3((n + 255) / 256), dim3(256), 0, c->st, c->g, c->pjbuf, c->uj);
+  return hipGetLastError();
+}
+
+}  // namespace wpic

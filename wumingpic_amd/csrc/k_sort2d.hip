@@ -1,0 +1,476 @@
+// k_sort2d.hip -- particle boundaries, row exchange and per-row counting sort
+// (2d/common/boundary_periodic.f90:61-248, 2d/common/sort.f90:36-82,
+// 2d/proj/reconnection/boundary_reconnection.f90:61-99) at one rank.
+//
+// Particles of species s live in rows j = nys..nye, each row a contiguous
+// segment of the species' structure-of-arrays block.  The reference's in-row
+// order after boundary_periodic__particle_y (leavers' slots refilled first by
+// arriving particles, then from the tail) followed by its stable bucket sort
+// is reproduced exactly, but computed in parallel:
+//   k_rowscan   one workgroup per row: leaver slots ("holes") and the up/down
+//               movers in slot order, by wavefront ballots and prefix counts;
+//   k_rowcount  new row lengths and bases (one scan over rows);
+//   k_rowbuild  one workgroup per row: R = for each slot of the post-exchange
+//               row, the index of the particle that lands there;
+//   k_rowsort   one workgroup per row: stable counting sort of R by cell
+//               (per-wave histograms, then a wave multi-split by ballots on
+//               the key bits), gathering the particles into sorted order.
+#include "wpic_internal.h"
+
+namespace wpic {
+
+constexpr int SB = 256;  // threads per row workgroup
+
+struct SortArgs {
+  Grid2D g;
+  Parts A, B;
+  const uint64_t* idsrc;
+  uint64_t* iddst;
+  int *code, *R, *holes, *ulist, *dlist;
+  int *rowbaseB, *np2B, *rowbaseN, *np2N, *rowbaseA, *np2A, *cumcnt, *rowcnt;
+  int* err;
+};
+
+__device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+// boundary_periodic__particle_x (round-down wrap) or
+// boundary_reconnection__particle_x (reflection, round-to-nearest) on B
+__global__ __launch_bounds__(SB) void k_wrapx(SortArgs a) {
+  const Grid2D& g = a.g;
+  const int rw = blockIdx.x;
+  const int base = a.rowbaseB[rw], n = a.np2B[rw];
+  for (int k = threadIdx.x; k < n; k += SB) {
+    const int p = base + k;
+    double x = a.B.x[p];
+    if (g.bc == WPIC_BC_WALLX) {
+      const int ipos = (int)(x / g.delx);
+      if (ipos < g.nxgs + 1) {
+        a.B.x[p] = g.wall_lo - x;
+        a.B.ux[p] = -a.B.ux[p];
+        a.B.uy[p] = -a.B.uy[p];
+        a.B.uz[p] = -a.B.uz[p];
+      } else if (ipos >= g.nxge - 1) {
+        a.B.x[p] = g.wall_hi - x;
+        a.B.ux[p] = -a.B.ux[p];
+        a.B.uy[p] = -a.B.uy[p];
+        a.B.uz[p] = -a.B.uz[p];
+      }
+    } else {
+      const int ipos = (int)rd_div(x, g.delx);
+      if (ipos < g.nxgs)
+        a.B.x[p] = rd_add(x, g.spanx);
+      else if (ipos >= g.nxge + 1)
+        a.B.x[p] = rd_add(x, -g.spanx);
+    }
+  }
+}
+
+// Per row: (YWRAP) the y wrap and destination code as particle_y computes
+// them (:143-167), then holes / up movers / down movers in slot order.
+template <bool YWRAP>
+__global__ __launch_bounds__(SB) void k_rowscan(SortArgs a) {
+  __shared__ int wt[SB / 64][3];
+  __shared__ int run[3];
+  const Grid2D& g = a.g;
+  const int rw = blockIdx.x;
+  const int j = g.nys + rw % g.ny;
+  const int base = a.rowbaseB[rw], n = a.np2B[rw];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid < 3) run[tid] = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += SB) {
+    const int k = c0 + tid;
+    int dy = 0;
+    if (k < n) {
+      const int p = base + k;
+      if (YWRAP) {
+        double y = a.B.y[p];
+        const int jpos = (int)rd_div(y, g.delx);
+        if (jpos != j) {
+          if (jpos <= g.nygs - 1)
+            a.B.y[p] = rd_add(y, g.spany);
+          else if (jpos >= g.nyge + 1)
+            a.B.y[p] = rd_add(y, -g.spany);
+        }
+        int ni = (int)a.B.x[p];
+        dy = jpos - j;
+        if (ni < g.nxgs || ni > g.nxge || dy < -1 || dy > 1) {
+          atomicOr(a.err, ERR_DOMAIN);
+          ni = ni < g.nxgs ? g.nxgs : (ni > g.nxge ? g.nxge : ni);
+          dy = dy < -1 ? -1 : (dy > 1 ? 1 : dy);
+        }
+        a.code[p] = ((ni - g.nxgs) << 2) | (dy + 1);
+      } else {
+        dy = (a.code[p] & 3) - 1;
+      }
+    }
+    const uint64_t bl = __ballot(dy != 0), bu = __ballot(dy == 1), bd = __ballot(dy == -1);
+    const uint64_t lt = lanemask_lt(lane);
+    if (lane == 0) {
+      wt[w][0] = __popcll(bl);
+      wt[w][1] = __popcll(bu);
+      wt[w][2] = __popcll(bd);
+    }
+    __syncthreads();
+    int ol = run[0], ou = run[1], od = run[2];
+    for (int v = 0; v < w; v++) {
+      ol += wt[v][0];
+      ou += wt[v][1];
+      od += wt[v][2];
+    }
+    if (dy != 0) a.holes[base + ol + __popcll(bl & lt)] = k + 1;
+    if (dy == 1) a.ulist[base + ou + __popcll(bu & lt)] = base + k;
+    if (dy == -1) a.dlist[base + od + __popcll(bd & lt)] = base + k;
+    __syncthreads();
+    if (tid < 3) {
+      int t = 0;
+      for (int v = 0; v < SB / 64; v++) t += wt[v][tid];
+      run[tid] += t;
+    }
+    __syncthreads();
+  }
+  if (tid < 3) a.rowcnt[rw * 4 + tid] = run[tid];
+}
+
+// incoming count m_J and new row length L = np2 - k + m; bases by one scan
+// per species.  bff(J) receives row J-1's up movers then row J+1's down
+// movers; at J = nys the down movers of nys+1 come first and nye's up movers
+// (sent through the periodic transfer, :173-189) are appended.
+__global__ __launch_bounds__(1024) void k_rowcount(SortArgs a) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const Grid2D& g = a.g;
+  const int ny = g.ny, tid = threadIdx.x;
+  for (int s = 0; s < g.nsp; s++) {
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int r0 = 0; r0 < ny; r0 += 1024) {
+      const int r = r0 + tid;
+      int L = 0;
+      if (r < ny) {
+        const int rw = s * ny + r;
+        const int rp = s * ny + (r == 0 ? ny - 1 : r - 1), rn = s * ny + (r == ny - 1 ? 0 : r + 1);
+        const int m = a.rowcnt[rp * 4 + 1] + a.rowcnt[rn * 4 + 2];
+        a.rowcnt[rw * 4 + 3] = m;
+        L = a.np2B[rw] - a.rowcnt[rw * 4 + 0] + m;
+        a.np2N[rw] = L;
+      }
+      // block exclusive scan of L
+      const int lane = tid & 63, w = tid >> 6;
+      int v = L;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+      }
+      if (lane == 63) wsum[w] = v;
+      __syncthreads();
+      if (tid == 0) {
+        int acc = 0;
+        for (int q = 0; q < 16; q++) {
+          const int t = wsum[q];
+          wsum[q] = acc;
+          acc += t;
+        }
+      }
+      __syncthreads();
+      const int excl = carry + wsum[w] + v - L;
+      if (r < ny) a.rowbaseN[s * ny + r] = (int)(s * g.ncap) + excl;
+      __syncthreads();
+      if (tid == 1023) carry = excl + L;
+      __syncthreads();
+    }
+  }
+}
+
+struct Incoming {
+  const int *l1, *l2;
+  int c1;
+  __device__ int operator()(int t) const { return t < c1 ? l1[t] : l2[t - c1]; }
+};
+
+// R for one row: slot -> source particle (boundary_periodic.f90:193-236)
+__global__ __launch_bounds__(SB) void k_rowbuild(SortArgs a) {
+  __shared__ int wt[SB / 64][2];
+  __shared__ int run[2];
+  __shared__ int sh_a;
+  const Grid2D& g = a.g;
+  const int rw = blockIdx.x;
+  const int s = rw / g.ny, r = rw % g.ny, ny = g.ny;
+  const int base = a.rowbaseB[rw], n = a.np2B[rw];
+  const int kk = a.rowcnt[rw * 4 + 0], m = a.rowcnt[rw * 4 + 3];
+  const int L = a.np2N[rw], nb = a.rowbaseN[rw];
+  const int rp = s * ny + (r == 0 ? ny - 1 : r - 1), rn = s * ny + (r == ny - 1 ? 0 : r + 1);
+  Incoming in;
+  if (r == 0) {
+    in.l1 = a.dlist + a.rowbaseB[rn];
+    in.c1 = a.rowcnt[rn * 4 + 2];
+    in.l2 = a.ulist + a.rowbaseB[rp];
+  } else {
+    in.l1 = a.ulist + a.rowbaseB[rp];
+    in.c1 = a.rowcnt[rp * 4 + 1];
+    in.l2 = a.dlist + a.rowbaseB[rn];
+  }
+  const int* holes = a.holes + base;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+
+  // non-leavers keep their slot when it survives
+  const int keep = n < L ? n : L;
+  for (int k = tid; k < keep; k += SB)
+    if ((a.code[base + k] & 3) == 1) a.R[nb + k] = base + k;
+  // the first min(k, m) holes take the arrivals in order
+  const int kf = kk < m ? kk : m;
+  for (int t = tid; t < kf; t += SB) {
+    const int h = holes[t];
+    if (h >= 1 && h <= L) a.R[nb + h - 1] = in(t);
+  }
+  // more arrivals than holes: the rest are appended
+  for (int t = tid; t < m - kk; t += SB) a.R[nb + n + t] = in(kk + t);
+  if (m >= kk) return;
+
+  // fewer arrivals than holes: tail particles fill the remaining holes, the
+  // last one into the first free hole (loop1, :197-216).  Tail = slots (L, n].
+  if (tid == 0) {
+    int lo = 0, hi = kk;  // a = number of holes <= L
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (holes[mid] <= L)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    sh_a = lo;
+    run[0] = 0;  // leavers seen in the tail so far
+    run[1] = 0;  // non-H' slots seen so far
+  }
+  __syncthreads();
+  const int aa = sh_a;
+  const int T = n - L;
+  // pass 1: count the tail slots that are not unfilled holes (non-H')
+  int totalq = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int c0 = 0; c0 < T; c0 += SB) {
+      const int k = c0 + tid;  // tail offset, slot = L + 1 + k
+      bool lv = false;
+      if (k < T) lv = (a.code[base + L + k] & 3) != 1;
+      const uint64_t bl = __ballot(lv);
+      if (lane == 0) wt[w][0] = __popcll(bl);
+      __syncthreads();
+      int ol = run[0];
+      for (int v = 0; v < w; v++) ol += wt[v][0];
+      const int rank = aa + ol + __popcll(bl & lanemask_lt(lane)) + 1;  // 1-based hole rank
+      const bool nonh = k < T && (!lv || rank <= m);
+      const uint64_t bn = __ballot(nonh);
+      if (lane == 0) wt[w][1] = __popcll(bn);
+      __syncthreads();
+      int on = run[1];
+      for (int v = 0; v < w; v++) on += wt[v][1];
+      if (pass == 1 && nonh) {
+        // descending position among non-H' tail slots
+        const int incl = on + __popcll(bn & lanemask_lt(lane)) + 1;
+        const int rr = totalq - incl + 1;
+        const int tgt = m + rr;  // 1-based hole index
+        const int src = lv ? in(rank - 1) : base + L + k;
+        const int h = (tgt >= 1 && tgt <= kk) ? holes[tgt - 1] : 0;
+        if (h >= 1 && h <= L)
+          a.R[nb + h - 1] = src;
+        else
+          atomicOr(a.err, ERR_ROW);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int t0 = 0, t1 = 0;
+        for (int v = 0; v < SB / 64; v++) {
+          t0 += wt[v][0];
+          t1 += wt[v][1];
+        }
+        run[0] += t0;
+        run[1] += t1;
+      }
+      __syncthreads();
+    }
+    totalq = run[1];
+    __syncthreads();
+    if (tid == 0) {
+      run[0] = 0;
+      run[1] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// stable counting sort of one row by cell (sort.f90:52-78), gathering B -> A
+__global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, int nbits) {
+  extern __shared__ int hist[];  // [SB/64][nx]
+  __shared__ int wsum[SB / 64];
+  __shared__ int carry;
+  const Grid2D& g = a.g;
+  const int nx = g.nx;
+  const int rw = blockIdx.x;
+  const int n = a.np2N[rw], nb = a.rowbaseN[rw];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  constexpr int NW = SB / 64;
+  for (int k = tid; k < NW * nx; k += SB) hist[k] = 0;
+  __syncthreads();
+  const int lo = (int)(((long long)n * w) / NW), hi = (int)(((long long)n * (w + 1)) / NW);
+  for (int k = lo + lane; k < hi; k += 64) {
+    int src = a.R[nb + k];
+    if ((unsigned)src >= (unsigned)(g.ncap * g.nsp)) src = 0;  // reported in the scatter pass
+    const int key = a.code[src] >> 2;
+    atomicAdd(&hist[w * nx + key], 1);
+  }
+  __syncthreads();
+  // exclusive scan over keys of the row totals -> cumcnt; per-wave starts
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  int* cc = a.cumcnt + (size_t)rw * (nx + 1);
+  for (int k0 = 0; k0 < nx; k0 += SB) {
+    const int key = k0 + tid;
+    int tot = 0;
+    if (key < nx)
+      for (int v = 0; v < NW; v++) tot += hist[v * nx + key];
+    int v = tot;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(v, off, 64);
+      if (lane >= off) v += t;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    int pre = carry;
+    for (int q = 0; q < w; q++) pre += wsum[q];
+    const int excl = pre + v - tot;
+    if (key < nx) {
+      cc[key] = excl;
+      int st = excl;
+      for (int q = 0; q < NW; q++) {
+        const int h = hist[q * nx + key];
+        hist[q * nx + key] = st;
+        st += h;
+      }
+    }
+    __syncthreads();
+    if (tid == SB - 1) carry = excl + tot;
+    __syncthreads();
+  }
+  if (tid == 0) cc[nx] = n;
+  // stable scatter: each wave walks its contiguous part in order
+  for (int k0 = lo; k0 < hi; k0 += 64) {
+    const int k = k0 + lane;
+    const bool act = k < hi;
+    int src = 0, key = 0;
+    if (act) {
+      src = a.R[nb + k];
+      if ((unsigned)src >= (unsigned)(g.ncap * g.nsp)) {
+        atomicOr(a.err, ERR_ROW);
+        src = 0;
+      }
+      key = a.code[src] >> 2;
+    }
+    uint64_t peers = __ballot(act);
+    for (int b = 0; b < nbits; b++) {
+      const bool bit = (key >> b) & 1;
+      const uint64_t m = __ballot(act && bit);
+      peers &= bit ? m : ~m;
+    }
+    const int rank = __popcll(peers & lanemask_lt(lane));
+    const int cnt = __popcll(peers);
+    int pos = 0;
+    if (act) pos = hist[w * nx + key] + rank;
+    if (act && rank == 0) hist[w * nx + key] = pos + cnt;
+    if (act) {
+      const int d = nb + pos;
+      a.A.x[d] = a.B.x[src];
+      a.A.y[d] = a.B.y[src];
+      a.A.ux[d] = a.B.ux[src];
+      a.A.uy[d] = a.B.uy[src];
+      a.A.uz[d] = a.B.uz[src];
+      a.iddst[d] = a.idsrc[src];
+    }
+  }
+}
+
+// sort__bucket on a gp that did not go through bc__particle_y (the restart
+// path, app.f90:351-352): identity permutation, key int(x)
+__global__ __launch_bounds__(SB) void k_codes_identity(SortArgs a) {
+  const Grid2D& g = a.g;
+  const int rw = blockIdx.x;
+  const int base = a.rowbaseB[rw], n = a.np2B[rw], nb = a.rowbaseN[rw];
+  for (int k = threadIdx.x; k < n; k += SB) {
+    int ni = (int)a.B.x[base + k];
+    if (ni < g.nxgs || ni > g.nxge) {
+      atomicOr(a.err, ERR_DOMAIN);
+      ni = ni < g.nxgs ? g.nxgs : g.nxge;
+    }
+    a.code[base + k] = ((ni - g.nxgs) << 2) | 1;
+    a.R[nb + k] = base + k;
+  }
+}
+
+static SortArgs sort_args(wpic_ctx* c) {
+  SortArgs a;
+  a.g = c->g;
+  a.A = c->A;
+  a.B = c->B;
+  a.idsrc = c->id[c->idb];
+  a.iddst = c->idb == c->ida ? c->id[1 - c->ida] : c->id[c->ida];
+  a.code = c->code;
+  a.R = c->R;
+  a.holes = c->holes;
+  a.ulist = c->ulist;
+  a.dlist = c->dlist;
+  a.rowbaseB = c->rowbaseB;
+  a.np2B = c->np2B;
+  a.rowbaseN = c->rowbaseN;
+  a.np2N = c->np2N;
+  a.rowbaseA = c->rowbaseA;
+  a.np2A = c->np2A;
+  a.cumcnt = c->cumcnt;
+  a.rowcnt = c->rowcnt;
+  a.err = c->err;
+  return a;
+}
+
+hipError_t launch_wrapx(wpic_ctx* c) {
+  SortArgs a = sort_args(c);
+  hipLaunchKernelGGL(k_wrapx, dim3(c->g.nsp * c->g.ny), dim3(SB), 0, c->st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowscan(wpic_ctx* c, bool ywrap) {
+  SortArgs a = sort_args(c);
+  if (ywrap)
+    hipLaunchKernelGGL(k_rowscan<true>, dim3(c->g.nsp * c->g.ny), dim3(SB), 0, c->st, a);
+  else
+    hipLaunchKernelGGL(k_rowscan<false>, dim3(c->g.nsp * c->g.ny), dim3(SB), 0, c->st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_codes_identity(wpic_ctx* c) {
+  SortArgs a = sort_args(c);
+  hipLaunchKernelGGL(k_codes_identity, dim3(c->g.nsp * c->g.ny), dim3(SB), 0, c->st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowcount(wpic_ctx* c) {
+  SortArgs a = sort_args(c);
+  hipLaunchKernelGGL(k_rowcount, dim3(1), dim3(1024), 0, c->st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowbuild(wpic_ctx* c) {
+  SortArgs a = sort_args(c);
+  hipLaunchKernelGGL(k_rowbuild, dim3(c->g.nsp * c->g.ny), dim3(SB), 0, c->st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowsort(wpic_ctx* c) {
+  SortArgs a = sort_args(c);
+  int nbits = 0;
+  while ((1 << nbits) < c->g.nx) nbits++;
+  const size_t shm = sizeof(int) * (SB / 64) * (size_t)c->g.nx;
+  hipLaunchKernelGGL(k_rowsort, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, nbits);
+  return hipGetLastError();
+}
+
+}  // namespace wpic

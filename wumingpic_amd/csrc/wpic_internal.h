@@ -1,0 +1,211 @@
+// wpic_internal.h -- device helpers, the context and the kernel launchers of
+// the 2D MI355X step loop.  Only the library's own translation units include
+// this; the public boundary is include/wpic.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/wpic.h"
+
+namespace wpic {
+
+// ---------------------------------------------------------------------------
+// Grid and physics constants, passed by value to every kernel.
+// Field planes carry the reference's two ghost layers:
+// i in nxgs-2..nxge+2, j in nys-2..nye+2, x fastest (coalesced along x).
+// ---------------------------------------------------------------------------
+struct Grid2D {
+  int nx, ny, nxgs, nxge, nygs, nyge, nys, nye;
+  int fw, fh;  // plane width nx+4 and height ny+4
+  int bc, nsp;
+  double delx, delt, c, gfac, d_delx, d_delt;
+  double f1, f2, f3, f4, f5, pi4dt;  // field.f90:53-57; 4*pi*delt
+  double spanx, spany;               // nx*delx, ny*delx rounded down
+  double wall_lo, wall_hi;           // 2*(nxs+1)*delx, 2*(nxe-1)*delx
+  double q[WPIC_MAXSP], r[WPIC_MAXSP];
+  long long ncap;                    // particle capacity per species
+};
+
+__host__ __device__ inline int fidx(const Grid2D& g, int i, int j) {
+  return (i - (g.nxgs - 2)) + g.fw * (j - (g.nys - 2));
+}
+// interior cell index (CG vectors, parity deposit blocks)
+__host__ __device__ inline int cidx(const Grid2D& g, int i, int j) {
+  return (i - g.nxgs) + g.nx * (j - g.nys);
+}
+
+struct Parts {
+  double *x, *y, *ux, *uy, *uz;
+};
+
+// device error bits, OR-ed by kernels and checked by the host at sync points
+enum : int { ERR_DOMAIN = 1, ERR_CG = 2, ERR_ROW = 4 };
+
+// ---------------------------------------------------------------------------
+// Round-toward-minus-infinity double arithmetic (the periodic particle wraps
+// run under ieee_down: boundary_periodic.f90:74,124).  Correctly rounded-down
+// results from round-to-nearest operations plus an exact error term.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double next_down(double s) {
+  long long b;
+  __builtin_memcpy(&b, &s, 8);
+  if (s > 0.0)
+    b -= 1;
+  else
+    b += 1;
+  double r;
+  __builtin_memcpy(&r, &b, 8);
+  return r;
+}
+
+__host__ __device__ inline double rd_add(double a, double b) {
+  double s = a + b;
+  if (s == 0.0) {
+    if (a == 0.0 && b == 0.0 && !__builtin_signbit(a) && !__builtin_signbit(b)) return 0.0;
+    return -0.0;
+  }
+  double bb = s - a;
+  double err = (a - (s - bb)) + (b - bb);
+  if (err < 0.0) s = next_down(s);
+  return s;
+}
+
+__host__ __device__ inline double rd_mul(double a, double b) {
+  double p = a * b;
+  double err = __builtin_fma(a, b, -p);
+  if (err < 0.0) p = next_down(p);
+  return p;
+}
+
+__host__ __device__ inline double rd_div(double a, double b) {
+  double q = a / b;
+  double r = __builtin_fma(-q, b, a);
+  if ((r < 0.0 && b > 0.0) || (r > 0.0 && b < 0.0)) q = next_down(q);
+  return q;
+}
+
+// quadratic spline with nodes at cell centres (particle.f90:99-107)
+__host__ __device__ inline void shape3(double dh, double& s0, double& s1, double& s2) {
+  s0 = 0.5 * (0.5 - dh) * (0.5 - dh);
+  s1 = 0.75 - dh * dh;
+  s2 = 0.5 * (0.5 + dh) * (0.5 + dh);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous band of logical tiles so neighbouring tiles (which share halo
+// atomics) meet in the same L2.  Launch 8*ceil(n/8) groups; extra ones exit.
+__device__ inline int xcd_tile(int bid, int ntiles) {
+  const int per = (ntiles + 7) >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+inline int xcd_grid(int ntiles) { return ((ntiles + 7) >> 3) << 3; }
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct CgScal {
+  double sum_g, eps, sumr_g, sum2_g, av, bv;
+  int ite, active;
+};
+
+struct TimerRec {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace wpic
+
+struct wpic_ctx {
+  wpic_params2d P;
+  wpic::Grid2D g;
+  int mode;
+  hipStream_t st;
+  int device;
+
+  // particles: species s occupies [s*ncap, (s+1)*ncap); rows compact within
+  wpic::Parts A, B;
+  uint64_t* id[2];
+  int ida, idb;
+  int *code, *R, *holes, *ulist, *dlist;
+  int* rows[3];                   // row structures: rowbase [nsp*ny] then np2 [nsp*ny]
+  int ra, rb, rn;                 // which row structure belongs to A, B, the exchange
+  int *rowbaseA, *np2A, *cumcnt;  // [nsp*ny], [nsp*ny], [nsp*ny*(nx+1)]
+  int *rowbaseB, *np2B;           // rows of B (gp)
+  int *rowbaseN, *np2N;           // rows after the y exchange
+  int* rowcnt;                    // [nsp*ny][4] leavers, up, down, incoming
+  bool b_valid;       // B holds a gp array
+  bool b_aligned;     // B's slots are A's slots (after particle__solv)
+  bool exch_pending;  // R / N rows describe B after bc__particle_y
+  std::vector<int> h_rows[3];  // host mirrors of the row structures
+  bool h_rows_ok[3];
+
+  // fields: planes of (nx+4)*(ny+4)
+  double *uf, *df, *uj;
+  double* pjbuf;  // parity-mode per-cell current blocks [nx*ny][65]
+
+  // CG: interior vectors per component
+  double *phi, *pa, *pb, *rr, *bb, *ap;  // [3][nx*ny]
+  double* part;                          // block partial sums [3][NPB][2]
+  wpic::CgScal* scal;                    // [3]
+  int* err;                              // device error bits
+  int* hflags;                           // pinned host mirror: err, active[3], ite[3]
+  int cg_iters[3];
+
+  double* staging;  // transfer staging (lazy)
+  size_t staging_n;
+  long long* aosbase;  // [nsp*ny] row starts in a compact AoS transfer buffer
+  double* gbuf;        // Gauss residual / rho scratch (lazy)
+  double* red;      // diagnostics reduction scratch
+  size_t red_n;
+
+  bool timing;
+  std::vector<wpic::TimerRec> timers;
+  std::vector<std::pair<std::string, std::pair<double, long long>>> tsum;
+};
+
+namespace wpic {
+
+constexpr int NPB = 256;  // fixed number of partial-sum blocks in CG reductions
+
+int set_error(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// launchers (each enqueues on ctx->st) -------------------------------------
+// push / deposit
+hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused);
+hipError_t launch_gather_pj(wpic_ctx* c);
+// fields
+hipError_t launch_zero(wpic_ctx* c, double* p, size_t n);
+hipError_t launch_curre_bc(wpic_ctx* c);
+hipError_t launch_dfield_bc(wpic_ctx* c);
+hipError_t launch_cg_init(wpic_ctx* c);
+hipError_t launch_cg_iter(wpic_ctx* c, int it_in_batch);
+hipError_t launch_cg_store(wpic_ctx* c);
+hipError_t launch_e_update(wpic_ctx* c);
+hipError_t launch_uf_add(wpic_ctx* c);
+hipError_t launch_energy(wpic_ctx* c, double* dout);
+hipError_t launch_gauss(wpic_ctx* c, double* G, double* rho);
+hipError_t launch_codes_identity(wpic_ctx* c);
+// particles: boundaries, exchange, sort
+hipError_t launch_wrapx(wpic_ctx* c);
+hipError_t launch_rowscan(wpic_ctx* c, bool ywrap);
+hipError_t launch_rowcount(wpic_ctx* c);
+hipError_t launch_rowbuild(wpic_ctx* c);
+hipError_t launch_rowsort(wpic_ctx* c);
+// transfers and initial state
+hipError_t launch_up_to_soa(wpic_ctx* c, const double* aos, long long n, int which);
+hipError_t launch_soa_to_aos(wpic_ctx* c, double* aos, int which);
+hipError_t launch_planes_from_aos(wpic_ctx* c, const double* aos, double* planes, int ncomp);
+hipError_t launch_planes_to_aos(wpic_ctx* c, const double* planes, double* aos, int ncomp);
+hipError_t launch_ic_beams(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
+                           double uth_i);
+hipError_t launch_fill(wpic_ctx* c, double* p, size_t n, double v);
+
+void timer_begin(wpic_ctx* c, const char* name, hipEvent_t* a);
+void timer_end(wpic_ctx* c, hipEvent_t a);
+
+}  // namespace wpic

@@ -1,0 +1,195 @@
+"""Host-side mirror of the reference's 2D module entry points over the C-ABI.
+
+`Pic2D` owns one device context.  Its methods carry the reference's names and
+argument meaning (2d/common/particle.f90, field.f90, sort.f90,
+boundary_periodic.f90, 2d/proj/reconnection/boundary_reconnection.f90); the
+particle and field arrays stay resident on the GPU between calls and cross
+the boundary only through the upload_*/download_* methods, in the
+reference's own Fortran layouts:
+
+    up(ndim, np, nys:nye, nsp), uf(6, nxgs-2:nxge+2, nys-2:nye+2),
+    cumcnt(nxgs:nxge+1, nys:nye, nsp), np2(nys:nye, nsp)
+
+Errors raise WpicError with the reference's message where the reference
+stops with one (e.g. "********** stop at cgm after ite_max **********").
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import BC_PERIODIC, BC_WALLX, MODE_FAST, MODE_PARITY, check  # noqa: F401
+
+
+def weibel_charges(n_ppc, omega_pe, mass_ratio, delx=1.0):
+    """q, r of 2d/proj/weibel/app.f90:293-302 with sigma_e = 0: ions first."""
+    pi = np.pi
+    wpe = omega_pe
+    wpi = wpe / np.sqrt(mass_ratio)
+    r = [mass_ratio, 1.0]
+    q = [+np.sqrt(r[0] / (4 * pi * n_ppc / delx ** 2)) * wpi,
+         -np.sqrt(r[1] / (4 * pi * n_ppc / delx ** 2)) * wpe]
+    return q, r
+
+
+class Pic2D:
+    def __init__(self, nx, ny, q, r, *, nsp=2, delx=1.0, delt=0.5, c=1.0, gfac=0.501,
+                 bc=BC_PERIODIC, mode=MODE_FAST, capacity=None, nxgs=2, nygs=2, n_ppc_hint=64):
+        L = _lib.lib()
+        p = _lib.Params2D()
+        p.ndim, p.nsp = 6, nsp
+        p.nxgs, p.nxge = nxgs, nxgs + nx - 1
+        p.nygs, p.nyge = nygs, nygs + ny - 1
+        p.delx, p.delt, p.c, p.gfac = delx, delt, c, gfac
+        for k in range(nsp):
+            p.q[k] = q[k]
+            p.r[k] = r[k]
+        p.bc, p.mode = bc, mode
+        p.capacity = int(capacity if capacity is not None else n_ppc_hint * nx * ny)
+        self.p = p
+        self.nx, self.ny, self.nsp = nx, ny, nsp
+        self._L = L
+        self._h = C.c_void_p()
+        check(L.wpic_create2d(C.byref(self._h), C.byref(p)))
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self):
+        if self._h:
+            self._L.wpic_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mode(self, mode):
+        check(self._L.wpic_set_mode(self._h, mode))
+
+    def sync(self):
+        check(self._L.wpic_sync(self._h))
+
+    # -- reference-layout transfers -------------------------------------------
+    def _shapes(self, np_cap):
+        return ((6, np_cap, self.ny, self.nsp), (self.nx + 1, self.ny, self.nsp), (self.ny, self.nsp))
+
+    def upload_up(self, up, np2, cumcnt):
+        up = np.asfortranarray(up, dtype=np.float64)
+        check(self._L.wpic_upload_up2d(self._h, up, np.asfortranarray(np2, dtype=np.int32),
+                                       np.asfortranarray(cumcnt, dtype=np.int32), up.shape[1]))
+
+    def download_up(self, np_cap, up=None):
+        su, sc, sn = self._shapes(np_cap)
+        up = np.zeros(su, order="F") if up is None else up
+        cumcnt = np.zeros(sc, dtype=np.int32, order="F")
+        np2 = np.zeros(sn, dtype=np.int32, order="F")
+        check(self._L.wpic_download_up2d(self._h, up, np2, cumcnt, np_cap))
+        return up, np2, cumcnt
+
+    def upload_gp(self, gp, np2):
+        gp = np.asfortranarray(gp, dtype=np.float64)
+        check(self._L.wpic_upload_gp2d(self._h, gp, np.asfortranarray(np2, dtype=np.int32), gp.shape[1]))
+
+    def download_gp(self, np_cap, gp=None):
+        su, _, sn = self._shapes(np_cap)
+        gp = np.zeros(su, order="F") if gp is None else gp
+        np2 = np.zeros(sn, dtype=np.int32, order="F")
+        check(self._L.wpic_download_gp2d(self._h, gp, np2, np_cap))
+        return gp, np2
+
+    def upload_fields(self, uf):
+        check(self._L.wpic_upload_fields2d(self._h, np.asfortranarray(uf, dtype=np.float64)))
+
+    def download_fields(self):
+        uf = np.zeros((6, self.nx + 4, self.ny + 4), order="F")
+        check(self._L.wpic_download_fields2d(self._h, uf))
+        return uf
+
+    def download_current(self):
+        uj = np.zeros((3, self.nx + 4, self.ny + 4), order="F")
+        check(self._L.wpic_download_current2d(self._h, uj))
+        return uj
+
+    def download_dfield(self):
+        df = np.zeros((6, self.nx + 4, self.ny + 4), order="F")
+        check(self._L.wpic_download_dfield2d(self._h, df))
+        return df
+
+    def upload_dfield(self, df):
+        check(self._L.wpic_upload_dfield2d(self._h, np.asfortranarray(df, dtype=np.float64)))
+
+    # -- reference entry points -----------------------------------------------
+    def particle_solv(self):
+        """particle__solv(gp, up, uf, cumcnt, nxs, nxe): 2d/common/particle.f90:48"""
+        check(self._L.wpic_particle_solv2d(self._h))
+
+    def field_fdtd_i(self):
+        """field__fdtd_i(uf, up, gp, cumcnt, ...): 2d/common/field.f90:66.
+        Returns the CG iteration count of each component."""
+        check(self._L.wpic_field_fdtd_i2d(self._h))
+        return self.cg_iters()
+
+    def bc_particle_x(self):
+        check(self._L.wpic_bc_particle_x2d(self._h))
+
+    def bc_particle_y(self):
+        check(self._L.wpic_bc_particle_y2d(self._h))
+
+    def sort_bucket(self):
+        """sort__bucket(up, gp, cumcnt, np2, nxs, nxe): 2d/common/sort.f90:36"""
+        check(self._L.wpic_sort_bucket2d(self._h))
+
+    def cg_iters(self):
+        it = np.zeros(3, dtype=np.int32)
+        check(self._L.wpic_cg_iters2d(self._h, it))
+        return tuple(int(v) for v in it)
+
+    def step(self, nsteps=1):
+        check(self._L.wpic_step2d(self._h, int(nsteps)))
+
+    # -- diagnostics and initial state ------------------------------------------
+    def energy(self):
+        out = np.zeros(3)
+        check(self._L.wpic_energy2d(self._h, out))
+        return out
+
+    def gauss(self):
+        G = np.zeros((self.nx, self.ny), order="F")
+        check(self._L.wpic_gauss2d(self._h, G))
+        return G
+
+    def counts(self):
+        n = np.zeros(_lib.MAXSP, dtype=np.int64)
+        check(self._L.wpic_counts2d(self._h, n))
+        return n[: self.nsp]
+
+    def ic_beams(self, seed, n_ppc, udrift, uth_e, uth_i, b0=0.0):
+        check(self._L.wpic_ic_beams2d(self._h, int(seed), int(n_ppc), float(udrift), float(uth_e),
+                                      float(uth_i), float(b0)))
+
+    def set_timing(self, on=True):
+        check(self._L.wpic_set_timing(self._h, 1 if on else 0))
+
+    def timing(self, name):
+        ms, cnt = C.c_double(), C.c_longlong()
+        check(self._L.wpic_timing(self._h, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+
+def weibel_beams_2d(nx, ny, n_ppc, *, seed=20261202, omega_pe=0.1, mass_ratio=25.0, udrift=0.5,
+                    uth_e=0.1, uth_i=0.01, delt=0.5, mode=MODE_FAST, capacity=None):
+    """The 2D Weibel beam workload: two species (ions first), periodic box,
+    dx = c = 1, seeded co-located random in-cell positions, electron beams at
+    u_x = +/-udrift with thermal spread uth_e, cold ions at uth_i."""
+    q, r = weibel_charges(n_ppc, omega_pe, mass_ratio)
+    cap = capacity if capacity is not None else n_ppc * nx * ny
+    sim = Pic2D(nx, ny, q, r, delt=delt, mode=mode, capacity=cap)
+    sim.ic_beams(seed, n_ppc, udrift, uth_e, uth_i, 0.0)
+    return sim
