@@ -48,7 +48,24 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=64, help="rows of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe", choices=["copy"], default=None,
+                    help="run a plain device copy of --probe-mib MiB (read) instead, to calibrate "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--probe-mib", type=int, default=1024)
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """one elementwise kernel that reads mib MiB and writes mib MiB"""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "copy", "read_mib": mib, "write_mib": mib, "launches": 3}))
 
 
 def cpu_baseline(w, rows, steps):
@@ -85,6 +102,9 @@ def log(msg):
 
 def main():
     a = parse()
+    if a.probe == "copy":
+        copy_probe(a.probe_mib)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {a.gpus}); "
               "multi-GPU sharding is designed but not built", file=sys.stderr)
