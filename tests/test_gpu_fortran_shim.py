@@ -1,0 +1,75 @@
+"""The reference app's call sequence, written in Fortran against the shim
+modules (particle, field, sort, boundary_periodic, boundary_reconnection),
+run on the GPU: the result equals the oracle's bit for bit in parity mode,
+with host mirroring after every call and in resident mode."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+import states2d as S
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "wumingpic_amd", "fortran", "shim_step2d_driver")
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def _state(bc, seed=2):
+    rng = np.random.default_rng(seed)
+    n_ppc, nx, ny = 8, 20, 16
+    q, r = S.weibel_consts(n_ppc)
+    p = po.make_params(nx, ny, np_cap=3 * n_ppc * nx, delt=0.5, q=q, r=r, bc=bc)
+    up, cc, np2 = S.sorted_particles(p, n_ppc, rng, u_scale=(0.02, 0.2), drift=(0.0, 0.5), colocate=True)
+    if bc == po.BC_WALLX:
+        for s in range(2):
+            for j in range(ny):
+                n = np2[j, s]
+                up[0, :n, j, s] = np.clip(up[0, :n, j, s], p.nxgs + 1.0, p.nxge - 1.0 - 1e-9)
+        o = po.Oracle2D(p)
+        out = np.zeros_like(up, order="F")
+        o.sort_bucket(out, up, cc, np2)
+        up = out
+    uf = S.periodic_fill_fields(p, S.random_fields(p, rng, scale=0.01))
+    return p, up, uf, cc, np2
+
+
+@pytest.mark.parametrize("bc", [po.BC_PERIODIC, po.BC_WALLX])
+@pytest.mark.parametrize("resident", [0, 1])
+def test_fortran_app_loop_on_shim_matches_oracle(bc, resident):
+    assert os.path.exists(DRIVER), "build with __graft_entry__.build()"
+    p, up, uf, cc, np2 = _state(bc)
+    nsteps = 5
+    with tempfile.TemporaryDirectory() as tmp:
+        with open(os.path.join(tmp, "params.txt"), "w") as f:
+            f.write(f"{nsteps} {bc} 1 {resident} {p.ndim} {p.np} {p.nsp} {p.nxgs} {p.nxge} {p.nygs} {p.nyge}\n")
+            f.write(f"{p.delx!r} {p.delt!r} {p.c!r} {p.gfac!r}\n")
+            f.write(" ".join(repr(float(p.q[k])) for k in range(p.nsp)) + "\n")
+            f.write(" ".join(repr(float(p.r[k])) for k in range(p.nsp)) + "\n")
+        up.ravel(order="F").tofile(os.path.join(tmp, "up.bin"))
+        uf.ravel(order="F").tofile(os.path.join(tmp, "uf.bin"))
+        cc.ravel(order="F").tofile(os.path.join(tmp, "cumcnt.bin"))
+        np2.ravel(order="F").tofile(os.path.join(tmp, "np2.bin"))
+        r = subprocess.run([DRIVER, tmp], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        u_dev = np.fromfile(os.path.join(tmp, "up.out")).reshape(up.shape, order="F")
+        f_dev = np.fromfile(os.path.join(tmp, "uf.out")).reshape(uf.shape, order="F")
+        c_dev = np.fromfile(os.path.join(tmp, "cumcnt.out"), dtype=np.int32).reshape(cc.shape, order="F")
+        n_dev = np.fromfile(os.path.join(tmp, "np2.out"), dtype=np.int32).reshape(np2.shape, order="F")
+    o = po.Oracle2D(p)
+    gp = np.zeros_like(up, order="F")
+    o.step(up, gp, uf, cc, np2, nsteps)
+    assert np.array_equal(n_dev, np2)
+    assert np.array_equal(c_dev, cc)
+    for s in range(2):
+        for j in range(p.ny):
+            n = np2[j, s]
+            assert np.array_equal(_bits(u_dev[:, :n, j, s]), _bits(up[:, :n, j, s])), (s, j)
+    assert np.array_equal(_bits(f_dev), _bits(uf))

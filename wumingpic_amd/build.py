@@ -40,6 +40,37 @@ def _compile(src):
     return obj, r.stderr
 
 
+FORT = os.path.join(HERE, "fortran")
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/amdflang")
+FSRC = ["wpic_capi.f90", "wpic_shim_state.f90", "shim_modules.f90"]
+FDRIVER = "shim_step2d_driver.f90"
+
+
+def build_fortran():
+    """The ISO_C_BINDING shim modules (the reference's module names) as
+    fortran/libwpic_fortran.a, and the app-loop driver linked to libwpic.so."""
+    mod = os.path.join(OBJ, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for f in FSRC + [FDRIVER]:
+        obj = os.path.join(mod, f + ".o")
+        cmd = [FLANG, "-O2", "-fPIC", "-module-dir", mod, "-I", mod, "-c", os.path.join(FORT, f), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {f}:\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    archive = os.path.join(FORT, "libwpic_fortran.a")
+    if os.path.exists(archive):
+        os.remove(archive)
+    subprocess.run(["ar", "rcs", archive, *objs[:-1]], check=True)
+    exe = os.path.join(FORT, "shim_step2d_driver")
+    cmd = [FLANG, "-o", exe, objs[-1], archive, "-L" + HERE, "-lwpic", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link of the Fortran driver failed:\n{r.stdout}\n{r.stderr}")
+    return exe
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
@@ -56,6 +87,7 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
 
 

@@ -1,0 +1,129 @@
+! wpic_capi.f90 -- ISO_C_BINDING interfaces to the C-ABI of include/wpic.h.
+module wpic_capi
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: WPIC_BC_PERIODIC = 0, WPIC_BC_WALLX = 1
+  integer(c_int), parameter :: WPIC_MODE_FAST = 0, WPIC_MODE_PARITY = 1
+
+  type, bind(C) :: wpic_params2d
+     integer(c_int) :: ndim, nsp
+     integer(c_int) :: nxgs, nxge, nygs, nyge
+     real(c_double) :: delx, delt, c, gfac
+     real(c_double) :: q(4), r(4)
+     integer(c_int) :: bc, mode
+     integer(c_long_long) :: capacity
+  end type wpic_params2d
+
+  interface
+     integer(c_int) function wpic_create2d(ctx, p) bind(C, name='wpic_create2d')
+       import :: c_int, c_ptr, wpic_params2d
+       type(c_ptr), intent(out) :: ctx
+       type(wpic_params2d), intent(in) :: p
+     end function wpic_create2d
+
+     integer(c_int) function wpic_destroy(ctx) bind(C, name='wpic_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_destroy
+
+     type(c_ptr) function wpic_last_error() bind(C, name='wpic_last_error')
+       import :: c_ptr
+     end function wpic_last_error
+
+     integer(c_int) function wpic_upload_up2d(ctx, up, np2, cumcnt, np) bind(C, name='wpic_upload_up2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: up(*)
+       integer(c_int), intent(in) :: np2(*), cumcnt(*)
+       integer(c_int), value :: np
+     end function wpic_upload_up2d
+
+     integer(c_int) function wpic_download_up2d(ctx, up, np2, cumcnt, np) bind(C, name='wpic_download_up2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: up(*)
+       integer(c_int), intent(out) :: np2(*), cumcnt(*)
+       integer(c_int), value :: np
+     end function wpic_download_up2d
+
+     integer(c_int) function wpic_upload_gp2d(ctx, gp, np2, np) bind(C, name='wpic_upload_gp2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: gp(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: np
+     end function wpic_upload_gp2d
+
+     integer(c_int) function wpic_download_gp2d(ctx, gp, np2, np) bind(C, name='wpic_download_gp2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: gp(*)
+       integer(c_int), intent(out) :: np2(*)
+       integer(c_int), value :: np
+     end function wpic_download_gp2d
+
+     integer(c_int) function wpic_upload_fields2d(ctx, uf) bind(C, name='wpic_upload_fields2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: uf(*)
+     end function wpic_upload_fields2d
+
+     integer(c_int) function wpic_download_fields2d(ctx, uf) bind(C, name='wpic_download_fields2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: uf(*)
+     end function wpic_download_fields2d
+
+     integer(c_int) function wpic_particle_solv2d(ctx) bind(C, name='wpic_particle_solv2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_particle_solv2d
+
+     integer(c_int) function wpic_field_fdtd_i2d(ctx) bind(C, name='wpic_field_fdtd_i2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_field_fdtd_i2d
+
+     integer(c_int) function wpic_bc_particle_x2d(ctx) bind(C, name='wpic_bc_particle_x2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_bc_particle_x2d
+
+     integer(c_int) function wpic_bc_particle_y2d(ctx) bind(C, name='wpic_bc_particle_y2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_bc_particle_y2d
+
+     integer(c_int) function wpic_sort_bucket2d(ctx) bind(C, name='wpic_sort_bucket2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_sort_bucket2d
+
+     integer(c_int) function wpic_step2d(ctx, nsteps) bind(C, name='wpic_step2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nsteps
+     end function wpic_step2d
+  end interface
+
+contains
+
+  ! the library's message for the last failing call, as a Fortran string
+  function wpic_error_text() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    p = wpic_last_error()
+    call c_f_pointer(p, s, [4096])
+    n = 0
+    do while (n < 4096)
+       if (s(n+1) == c_null_char) exit
+       n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function wpic_error_text
+
+end module wpic_capi
