@@ -45,8 +45,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="weibel1024", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="fast", choices=["fast", "parity"])
-    ap.add_argument("--cpu-rows", type=int, default=64, help="rows of the CPU baseline sample")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-rows", type=int, default=128, help="rows of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", choices=["copy"], default=None,
                     help="run a plain device copy of --probe-mib MiB (read) instead, to calibrate "
@@ -134,6 +134,7 @@ def main():
     log(f"{a.steps} timed steps in {elapsed:.3f} s")
     pd_ms, pd_n = sim.timing("push_deposit")
     fld_ms, fld_n = sim.timing("field")
+    phases = {name: sim.timing(name)[0] / a.steps for name in ("push_deposit", "field", "cg", "exchange", "sort")}
     cg_iters = sim.cg_iters()
     sim.set_timing(False)
     energy = sim.energy()
@@ -163,6 +164,7 @@ def main():
         "step_bandwidth_GBs": STEP_BYTES * npart * a.steps / elapsed / 1e9,
         "cell_updates_per_s": ncell * fld_n / (fld_ms / 1e3) if fld_ms > 0 else None,
         "field_ms_per_step": fld_ms / max(fld_n, 1),
+        "phase_ms_per_step": phases,
         "cg_iters_last_step": list(cg_iters),
         "energy_final": {"kinetic": energy[0], "E": energy[1], "B": energy[2]},
         "cpu_baseline": None,

@@ -60,10 +60,184 @@ __device__ inline double cell_field(const Grid2D& g, const double* uf, int c, in
 // to a never-negative-zero sum is the identity, so they are not carried.
 constexpr int NPJ = 65;
 
+// Esirkepov shape arrays of one particle (field.f90:223-268): s0 at the old
+// position relative to cell (i, j), ds = (shifted new shape) - s0
+__device__ inline void esirkepov_shapes(const Grid2D& g, double x, double y, double nxp, double nyp, int i, int j,
+                                        double s0x[5], double s0y[5], double dsx[5], double dsy[5]) {
+  const double d_delx = g.d_delx;
+  double t0, t1, t2;
+  shape3(x * d_delx - 0.5 - i, t0, t1, t2);
+  s0x[0] = 0.0; s0x[1] = t0; s0x[2] = t1; s0x[3] = t2; s0x[4] = 0.0;
+  shape3(y * d_delx - 0.5 - j, t0, t1, t2);
+  s0y[0] = 0.0; s0y[1] = t0; s0y[2] = t1; s0y[3] = t2; s0y[4] = 0.0;
+#pragma unroll
+  for (int d = 0; d < 2; d++) {
+    const double gp = d == 0 ? nxp : nyp;
+    const int cell = d == 0 ? i : j;
+    const int i2 = (int)(gp * d_delx);
+    const double dh = gp * d_delx - 0.5 - i2;
+    const int inc = i2 - cell;
+    const double s1_1 = 0.5 * (0.5 - dh) * (0.5 - dh);
+    const double s1_2 = 0.75 - dh * dh;
+    const double s1_3 = 0.5 * (0.5 + dh) * (0.5 + dh);
+    // smo_1 = -(inc-abs(inc))*0.5+0 etc. (field.f90:244-246): exactly 0 or 1
+    const double smo_1 = inc < 0 ? 1.0 : 0.0;
+    const double smo_2 = (double)(-abs(inc) + 1);
+    const double smo_3 = inc > 0 ? 1.0 : 0.0;
+    double* ds = d == 0 ? dsx : dsy;
+    const double* s0 = d == 0 ? s0x : s0y;
+    ds[0] = s1_1 * smo_1;
+    ds[1] = s1_1 * smo_2 + s1_2 * smo_1;
+    ds[2] = s1_2 * smo_2 + s1_3 * smo_1 + s1_1 * smo_3;
+    ds[3] = s1_3 * smo_2 + s1_2 * smo_3;
+    ds[4] = s1_3 * smo_3;
+#pragma unroll
+    for (int k = 0; k < 5; k++) ds[k] = ds[k] - s0[k];
+  }
+}
+
+// One cell's N accumulators: parity mode stores them to the cell's block at
+// offset OFF; fast mode reduces them over the G lanes of the group and adds
+// them to the tile accumulator, spreading the atomics over the lanes.
+template <int N, int OFF, int G, bool PARITY>
+__device__ inline void flush_block(double (&v)[N], double* pjbuf, double (*acc)[ACW * ACH], int cellidx, int li,
+                                   int lj, int lg) {
+  if (PARITY) {
+    double* dst = pjbuf + (size_t)cellidx * NPJ + OFF;
+#pragma unroll
+    for (int k = 0; k < N; k++) dst[k] = v[k];
+    return;
+  }
+  if (G > 1) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+#pragma unroll
+      for (int off = G / 2; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, G);
+    }
+  }
+#pragma unroll
+  for (int kk = 0; kk < N; kk++) {
+    if ((kk % G) != lg) continue;
+    const int k = kk + OFF;
+    int comp, ip, jp;
+    if (k < 20) {
+      comp = 0; jp = k / 4 - 2; ip = k % 4 - 1;
+    } else if (k < 40) {
+      comp = 1; jp = (k - 20) / 5 - 1; ip = (k - 20) % 5 - 2;
+    } else {
+      comp = 2; jp = (k - 40) / 5 - 2; ip = (k - 40) % 5 - 2;
+    }
+    atomicAdd(&acc[comp][(lj + 2 + jp) * ACW + (li + 2 + ip)], v[kk]);
+  }
+}
+
+// One pass over a cell's particles accumulating the current components
+// selected by DZ/DX/DY from the old (A) and new (B) positions, then flushing
+// them.  With WRAP it also applies boundary_periodic__particle_x/_y's
+// round-down wraps (:74-88, :124-154) and writes the exchange code.
+template <bool DZ, bool DX, bool DY, bool WRAP, int G, bool PARITY, int BC>
+__device__ inline void deposit_pass(const PushArgs& a, double (*acc)[ACW * ACH], int i, int j, int li, int lj, int lg,
+                                    int cellidx) {
+  const Grid2D& g = a.g;
+  const double cc2 = g.c * g.c;
+  const bool unit_c = cc2 == 1.0;
+  const double fac = 1.0 / 3.0;
+  double az[DZ ? 25 : 1], ax[DX ? 20 : 1], ay[DY ? 20 : 1];
+#pragma unroll
+  for (int k = 0; k < (DZ ? 25 : 1); k++) az[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < (DX ? 20 : 1); k++) ax[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < (DY ? 20 : 1); k++) ay[k] = 0.0;
+  for (int s = 0; s < g.nsp; s++) {
+    const int rw = s * g.ny + (j - g.nys);
+    const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
+    const int base = a.rowbase[rw];
+    const int pbeg = base + cr[i - g.nxgs], pend = base + cr[i - g.nxgs + 1];
+    const double q = g.q[s];
+    const double qk = q * g.delx * g.d_delt;
+    for (int pi = pbeg + lg; pi < pend; pi += G) {
+      double nxp = a.B.x[pi], nyp = a.B.y[pi];
+      double s0x[5], s0y[5], dsx[5], dsy[5];
+      esirkepov_shapes(g, a.A.x[pi], a.A.y[pi], nxp, nyp, i, j, s0x, s0y, dsx, dsy);
+      if (DZ) {
+        const double g3 = a.B.ux[pi], g4 = a.B.uy[pi], g5 = a.B.uz[pi];
+        const double u2 = +g3 * g3 + g4 * g4 + g5 * g5;
+        const double gvz = g5 / sqrt(1. + (unit_c ? u2 : u2 / cc2));
+        const double qgvz = q * gvz;
+#pragma unroll
+        for (int jp = 0; jp < 5; jp++)
+#pragma unroll
+          for (int ip = 0; ip < 5; ip++)
+            az[jp * 5 + ip] = az[jp * 5 + ip] + qgvz * (+s0x[ip] * s0y[jp] + 0.5 * dsx[ip] * s0y[jp] +
+                                                         0.5 * s0x[ip] * dsy[jp] + fac * dsx[ip] * dsy[jp]);
+      }
+      if (DX) {
+#pragma unroll
+        for (int jp = 0; jp < 5; jp++) {
+          const double wy = s0y[jp] + 0.5 * dsy[jp];
+          double run = 0.0;
+#pragma unroll
+          for (int ip = 0; ip < 4; ip++) {
+            run = run - qk * dsx[ip] * wy;
+            ax[jp * 4 + ip] = ax[jp * 4 + ip] + run;
+          }
+        }
+      }
+      if (DY) {
+#pragma unroll
+        for (int ip = 0; ip < 5; ip++) {
+          const double wx = s0x[ip] + 0.5 * dsx[ip];
+          double run = 0.0;
+#pragma unroll
+          for (int jp = 0; jp < 4; jp++) {
+            run = run - qk * dsy[jp] * wx;
+            ay[jp * 5 + ip] = ay[jp * 5 + ip] + run;
+          }
+        }
+      }
+      if (WRAP) {
+        if (BC == WPIC_BC_PERIODIC) {
+          const int ipos = (int)rd_div(nxp, g.delx);
+          if (ipos < g.nxgs)
+            nxp = rd_add(nxp, g.spanx);
+          else if (ipos >= g.nxge + 1)
+            nxp = rd_add(nxp, -g.spanx);
+        }
+        const int jpos = (int)rd_div(nyp, g.delx);
+        if (jpos != j) {
+          if (jpos <= g.nygs - 1)
+            nyp = rd_add(nyp, g.spany);
+          else if (jpos >= g.nyge + 1)
+            nyp = rd_add(nyp, -g.spany);
+        }
+        int ni = (int)nxp, dy = jpos - j;
+        if (ni < g.nxgs || ni > g.nxge || dy < -1 || dy > 1) {
+          atomicOr(a.err, ERR_DOMAIN);
+          ni = ni < g.nxgs ? g.nxgs : (ni > g.nxge ? g.nxge : ni);
+          dy = dy < -1 ? -1 : (dy > 1 ? 1 : dy);
+        }
+        a.B.x[pi] = nxp;
+        a.B.y[pi] = nyp;
+        a.code[pi] = ((ni - g.nxgs) << 2) | (dy + 1);
+      }
+    }
+  }
+  if constexpr (DZ) flush_block<25, 40, G, PARITY>(az, a.pjbuf, acc, cellidx, li, lj, lg);
+  if constexpr (DX) flush_block<20, 0, G, PARITY>(ax, a.pjbuf, acc, cellidx, li, lj, lg);
+  if constexpr (DY) flush_block<20, 20, G, PARITY>(ay, a.pjbuf, acc, cellidx, li, lj, lg);
+}
+
+// Per cell, three passes over its particles keep few values live at once:
+//   A: push and reflecting walls (the cell's 54 staged field values stay in
+//      registers), gp stored;  B: Jz and Jx;  C: Jy, then the periodic wraps
+//   and the exchange code.  B and C re-read the cell's particles while they
+//   are still in L2.  Each accumulator still receives the cell's particles in
+//   slot order, so G = 1 reproduces the reference's sums.
 template <int G, bool PUSH, bool DEPOSIT, bool FUSED, bool PARITY, int BC>
 __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
   __shared__ double sf[6][SFW * SFH];
-  __shared__ double acc[(DEPOSIT && !PARITY) ? 3 : 1][(DEPOSIT && !PARITY) ? ACW * ACH : 1];
+  __shared__ double acc[(DEPOSIT && !PARITY) ? 3 : 1][ACW * ACH];
 
   const Grid2D& g = a.g;
   const int ntiles = a.ntx * a.nty;
@@ -88,18 +262,17 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
   const int grp = tid / G, lg = tid % G;
   constexpr int NGRP = NT / G;
   const double d_delx = g.d_delx, delt = g.delt, c = g.c;
+  const double cc2 = c * c;
+  const bool unit_c = cc2 == 1.0;  // x/(c*c) == x exactly: skip the division
+  const double fac = 1.0 / 3.0;
 
   for (int cc = grp; cc < TX * TY; cc += NGRP) {
     const int li = cc % TX, lj = cc / TX;
     const int i = i0 + li, j = j0 + lj;
     if (i > g.nxge || j > g.nye) continue;  // uniform within a group
+    const int cellidx = cidx(g, i, j);
 
-    double pj[DEPOSIT ? NPJ : 1];
-    if (DEPOSIT) {
-#pragma unroll
-      for (int k = 0; k < NPJ; k++) pj[k] = 0.0;
-    }
-
+    // ---- pass A: push and reflecting walls; gp stored unwrapped ----
     for (int s = 0; s < g.nsp; s++) {
       const int rw = s * g.ny + (j - g.nys);
       const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
@@ -109,8 +282,6 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
       const double fac1 = q / rm * 0.5 * delt;
       const double txxx = fac1 * fac1;
       const double fac2 = q * delt / rm;
-      const double qk = q * g.delx * g.d_delt;
-
       for (int pi = pbeg + lg; pi < pend; pi += G) {
         const double x = a.A.x[pi], y = a.A.y[pi];
         double nxp, nyp, g3, g4, g5;
@@ -118,10 +289,15 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
           double sx0, sx1, sx2, sy0, sy1, sy2;
           shape3(x * d_delx - 0.5 - i, sx0, sx1, sx2);
           shape3(y * d_delx - 0.5 - j, sy0, sy1, sy2);
+          // re-read the cell's staged fields from LDS for every particle (the
+          // group's lanes read the same words: a broadcast) instead of holding
+          // all 54 in registers across the loop
+          int soff = (lj + 1) * SFW + (li + 1);
+          asm volatile("" : "+v"(soff));
           double fl[6];
 #pragma unroll
           for (int cmp = 0; cmp < 6; cmp++) {
-            const double* f = &sf[cmp][(lj + 1) * SFW + (li + 1)];
+            const double* f = &sf[cmp][soff];
             fl[cmp] = +(+f[-SFW - 1] * sx0 + f[-SFW] * sx1 + f[-SFW + 1] * sx2) * sy0 +
                       (+f[-1] * sx0 + f[0] * sx1 + f[1] * sx2) * sy1 +
                       (+f[SFW - 1] * sx0 + f[SFW] * sx1 + f[SFW + 1] * sx2) * sy2;
@@ -143,7 +319,8 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
           g3 = uvm1 + fac1 * epx;
           g4 = uvm2 + fac1 * epy;
           g5 = uvm3 + fac1 * epz;
-          gam = 1.0 / sqrt(1.0 + (+g3 * g3 + g4 * g4 + g5 * g5) / (c * c));
+          const double u2 = +g3 * g3 + g4 * g4 + g5 * g5;
+          gam = 1.0 / sqrt(1.0 + (unit_c ? u2 : u2 / cc2));
           nxp = x + g3 * delt * gam;
           nyp = y + g4 * delt * gam;
         } else {
@@ -153,7 +330,6 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
           g4 = a.B.uy[pi];
           g5 = a.B.uz[pi];
         }
-
         // reflecting walls act before the deposit in the reconnection order
         // (2d/proj/reconnection/app.f90:101-103; boundary_reconnection.f90:79-91)
         if (FUSED && BC == WPIC_BC_WALLX) {
@@ -166,97 +342,6 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
             g3 = -g3; g4 = -g4; g5 = -g5;
           }
         }
-
-        if (DEPOSIT) {
-          double s0x[5], s0y[5], dsx[5], dsy[5];
-          {
-            double t0, t1, t2;
-            shape3(x * d_delx - 0.5 - i, t0, t1, t2);
-            s0x[0] = 0.0; s0x[1] = t0; s0x[2] = t1; s0x[3] = t2; s0x[4] = 0.0;
-            shape3(y * d_delx - 0.5 - j, t0, t1, t2);
-            s0y[0] = 0.0; s0y[1] = t0; s0y[2] = t1; s0y[3] = t2; s0y[4] = 0.0;
-          }
-#pragma unroll
-          for (int d = 0; d < 2; d++) {
-            const double gp = d == 0 ? nxp : nyp;
-            const int cell = d == 0 ? i : j;
-            const int i2 = (int)(gp * d_delx);
-            const double dh = gp * d_delx - 0.5 - i2;
-            const int inc = i2 - cell;
-            const double s1_1 = 0.5 * (0.5 - dh) * (0.5 - dh);
-            const double s1_2 = 0.75 - dh * dh;
-            const double s1_3 = 0.5 * (0.5 + dh) * (0.5 + dh);
-            // smo_1 = -(inc-abs(inc))*0.5+0 etc. (field.f90:244-246): 0 or 1
-            const double smo_1 = inc < 0 ? 1.0 : 0.0;
-            const double smo_2 = (double)(-abs(inc) + 1);
-            const double smo_3 = inc > 0 ? 1.0 : 0.0;
-            double* ds = d == 0 ? dsx : dsy;
-            const double* s0 = d == 0 ? s0x : s0y;
-            ds[0] = s1_1 * smo_1;
-            ds[1] = s1_1 * smo_2 + s1_2 * smo_1;
-            ds[2] = s1_2 * smo_2 + s1_3 * smo_1 + s1_1 * smo_3;
-            ds[3] = s1_3 * smo_2 + s1_2 * smo_3;
-            ds[4] = s1_3 * smo_3;
-#pragma unroll
-            for (int k = 0; k < 5; k++) ds[k] = ds[k] - s0[k];
-          }
-          const double gvz = g5 / sqrt(1. + (+g3 * g3 + g4 * g4 + g5 * g5) / (c * c));
-          const double qgvz = q * gvz;
-          const double fac = 1.0 / 3.0;
-          // pjx(ip+1,jp) = pjx(ip,jp) - q dx/dt ds_x(ip) (s0_y(jp) + ds_y(jp)/2)
-#pragma unroll
-          for (int jp = 0; jp < 5; jp++) {
-            const double wy = s0y[jp] + 0.5 * dsy[jp];
-            double run = 0.0;
-#pragma unroll
-            for (int ip = 0; ip < 4; ip++) {
-              run = run - qk * dsx[ip] * wy;
-              pj[jp * 4 + ip] = pj[jp * 4 + ip] + run;
-            }
-          }
-#pragma unroll
-          for (int ip = 0; ip < 5; ip++) {
-            const double wx = s0x[ip] + 0.5 * dsx[ip];
-            double run = 0.0;
-#pragma unroll
-            for (int jp = 0; jp < 4; jp++) {
-              run = run - qk * dsy[jp] * wx;
-              pj[20 + jp * 5 + ip] = pj[20 + jp * 5 + ip] + run;
-            }
-          }
-#pragma unroll
-          for (int jp = 0; jp < 5; jp++)
-#pragma unroll
-            for (int ip = 0; ip < 5; ip++)
-              pj[40 + jp * 5 + ip] =
-                  pj[40 + jp * 5 + ip] + qgvz * (+s0x[ip] * s0y[jp] + 0.5 * dsx[ip] * s0y[jp] +
-                                                 0.5 * s0x[ip] * dsy[jp] + fac * dsx[ip] * dsy[jp]);
-        }
-
-        if (FUSED) {
-          // boundary_periodic__particle_x / _y wraps, round-down (:74-88, :124-154)
-          if (BC == WPIC_BC_PERIODIC) {
-            const int ipos = (int)rd_div(nxp, g.delx);
-            if (ipos < g.nxgs)
-              nxp = rd_add(nxp, g.spanx);
-            else if (ipos >= g.nxge + 1)
-              nxp = rd_add(nxp, -g.spanx);
-          }
-          const int jpos = (int)rd_div(nyp, g.delx);
-          if (jpos != j) {
-            if (jpos <= g.nygs - 1)
-              nyp = rd_add(nyp, g.spany);
-            else if (jpos >= g.nyge + 1)
-              nyp = rd_add(nyp, -g.spany);
-          }
-          int ni = (int)nxp, dy = jpos - j;
-          if (ni < g.nxgs || ni > g.nxge || dy < -1 || dy > 1) {
-            atomicOr(a.err, ERR_DOMAIN);
-            ni = ni < g.nxgs ? g.nxgs : (ni > g.nxge ? g.nxge : ni);
-            dy = dy < -1 ? -1 : (dy > 1 ? 1 : dy);
-          }
-          a.code[pi] = ((ni - g.nxgs) << 2) | (dy + 1);
-        }
         if (PUSH) {
           a.B.x[pi] = nxp;
           a.B.y[pi] = nyp;
@@ -266,35 +351,13 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
         }
       }
     }
+    if (!DEPOSIT) continue;
 
-    if (DEPOSIT) {
-      if (PARITY) {
-        double* dst = a.pjbuf + (size_t)cidx(g, i, j) * NPJ;
-#pragma unroll
-        for (int k = 0; k < NPJ; k++) dst[k] = pj[k];
-      } else {
-        if (G > 1) {
-#pragma unroll
-          for (int k = 0; k < NPJ; k++) {
-#pragma unroll
-            for (int off = G / 2; off >= 1; off >>= 1) pj[k] += __shfl_xor(pj[k], off, G);
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < NPJ; k++) {
-          if ((k % G) != lg) continue;
-          int comp, ip, jp;
-          if (k < 20) {
-            comp = 0; jp = k / 4 - 2; ip = k % 4 - 1;
-          } else if (k < 40) {
-            comp = 1; jp = (k - 20) / 5 - 1; ip = (k - 20) % 5 - 2;
-          } else {
-            comp = 2; jp = (k - 40) / 5 - 2; ip = (k - 40) % 5 - 2;
-          }
-          atomicAdd(&acc[comp][(lj + 2 + jp) * ACW + (li + 2 + ip)], pj[k]);
-        }
-      }
-    }
+    // ---- deposit passes: Jz (field.f90:292-298), Jx (:274-281), Jy (:283-290);
+    //      the last pass also applies the wraps and writes the exchange code ----
+    deposit_pass<true, false, false, false, G, PARITY, BC>(a, acc, i, j, li, lj, lg, cellidx);
+    deposit_pass<false, true, false, false, G, PARITY, BC>(a, acc, i, j, li, lj, lg, cellidx);
+    deposit_pass<false, false, true, FUSED, G, PARITY, BC>(a, acc, i, j, li, lj, lg, cellidx);
   }
 
   if (DEPOSIT && !PARITY) {

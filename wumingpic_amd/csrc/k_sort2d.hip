@@ -299,7 +299,23 @@ __global__ __launch_bounds__(SB) void k_rowbuild(SortArgs a) {
   }
 }
 
-// stable counting sort of one row by cell (sort.f90:52-78), gathering B -> A
+// peers of this lane among the active lanes: lanes holding the same key
+__device__ inline uint64_t key_peers(int key, bool act, int nbits) {
+  uint64_t peers = __ballot(act);
+  for (int b = 0; b < nbits; b++) {
+    const bool bit = (key >> b) & 1;
+    const uint64_t m = __ballot(act && bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+// stable counting sort of one row by cell (sort.f90:52-78), gathering B -> A.
+// Each wave owns a contiguous quarter of the row: its per-key histogram, then
+// (after a scan over keys and waves) its per-key write cursors.  Within a
+// wave, lanes with equal keys are ranked by one ballot per key bit.  Chunks
+// of 64 are processed U at a time so their loads overlap.
+constexpr int SU = 4;
 __global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, int nbits) {
   extern __shared__ int hist[];  // [SB/64][nx]
   __shared__ int wsum[SB / 64];
@@ -309,18 +325,33 @@ __global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, int nbits) {
   const int rw = blockIdx.x;
   const int n = a.np2N[rw], nb = a.rowbaseN[rw];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const unsigned ntot = (unsigned)(g.ncap * g.nsp);
+  const uint64_t lt = lanemask_lt(lane);
   constexpr int NW = SB / 64;
   for (int k = tid; k < NW * nx; k += SB) hist[k] = 0;
   __syncthreads();
   const int lo = (int)(((long long)n * w) / NW), hi = (int)(((long long)n * (w + 1)) / NW);
-  for (int k = lo + lane; k < hi; k += 64) {
-    int src = a.R[nb + k];
-    if ((unsigned)src >= (unsigned)(g.ncap * g.nsp)) src = 0;  // reported in the scatter pass
-    const int key = a.code[src] >> 2;
-    atomicAdd(&hist[w * nx + key], 1);
+  int* hw = hist + w * nx;
+  // per-wave histogram; one lane per distinct key adds its group's count
+  for (int k0 = lo; k0 < hi; k0 += 64 * SU) {
+    int key[SU];
+    bool act[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int k = k0 + u * 64 + lane;
+      act[u] = k < hi;
+      int src = act[u] ? a.R[nb + k] : 0;
+      if ((unsigned)src >= ntot) src = 0;  // reported in the scatter pass
+      key[u] = act[u] ? (a.code[src] >> 2) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const uint64_t peers = key_peers(key[u], act[u], nbits);
+      if (act[u] && __popcll(peers & lt) == 0) hw[key[u]] += __popcll(peers);
+    }
   }
   __syncthreads();
-  // exclusive scan over keys of the row totals -> cumcnt; per-wave starts
+  // exclusive scan over keys of the row totals -> cumcnt; per-wave cursors
   if (tid == 0) carry = 0;
   __syncthreads();
   int* cc = a.cumcnt + (size_t)rw * (nx + 1);
@@ -354,38 +385,53 @@ __global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, int nbits) {
     __syncthreads();
   }
   if (tid == 0) cc[nx] = n;
-  // stable scatter: each wave walks its contiguous part in order
-  for (int k0 = lo; k0 < hi; k0 += 64) {
-    const int k = k0 + lane;
-    const bool act = k < hi;
-    int src = 0, key = 0;
-    if (act) {
-      src = a.R[nb + k];
-      if ((unsigned)src >= (unsigned)(g.ncap * g.nsp)) {
+  // stable scatter: each wave walks its part in order
+  for (int k0 = lo; k0 < hi; k0 += 64 * SU) {
+    int src[SU], key[SU];
+    bool act[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int k = k0 + u * 64 + lane;
+      act[u] = k < hi;
+      src[u] = act[u] ? a.R[nb + k] : 0;
+      if ((unsigned)src[u] >= ntot) {
         atomicOr(a.err, ERR_ROW);
-        src = 0;
+        act[u] = false;
+        src[u] = 0;
       }
-      key = a.code[src] >> 2;
     }
-    uint64_t peers = __ballot(act);
-    for (int b = 0; b < nbits; b++) {
-      const bool bit = (key >> b) & 1;
-      const uint64_t m = __ballot(act && bit);
-      peers &= bit ? m : ~m;
+#pragma unroll
+    for (int u = 0; u < SU; u++) key[u] = act[u] ? (a.code[src[u]] >> 2) : 0;
+    double vx[SU], vy[SU], vux[SU], vuy[SU], vuz[SU];
+    uint64_t vid[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      if (act[u]) {
+        vx[u] = a.B.x[src[u]];
+        vy[u] = a.B.y[src[u]];
+        vux[u] = a.B.ux[src[u]];
+        vuy[u] = a.B.uy[src[u]];
+        vuz[u] = a.B.uz[src[u]];
+        vid[u] = a.idsrc[src[u]];
+      }
     }
-    const int rank = __popcll(peers & lanemask_lt(lane));
-    const int cnt = __popcll(peers);
-    int pos = 0;
-    if (act) pos = hist[w * nx + key] + rank;
-    if (act && rank == 0) hist[w * nx + key] = pos + cnt;
-    if (act) {
-      const int d = nb + pos;
-      a.A.x[d] = a.B.x[src];
-      a.A.y[d] = a.B.y[src];
-      a.A.ux[d] = a.B.ux[src];
-      a.A.uy[d] = a.B.uy[src];
-      a.A.uz[d] = a.B.uz[src];
-      a.iddst[d] = a.idsrc[src];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const uint64_t peers = key_peers(key[u], act[u], nbits);
+      const int rank = __popcll(peers & lt);
+      const int cnt = __popcll(peers);
+      int pos = 0;
+      if (act[u]) pos = hw[key[u]] + rank;
+      if (act[u] && rank == 0) hw[key[u]] = pos + cnt;
+      if (act[u]) {
+        const int d = nb + pos;
+        a.A.x[d] = vx[u];
+        a.A.y[d] = vy[u];
+        a.A.ux[d] = vux[u];
+        a.A.uy[d] = vuy[u];
+        a.A.uz[d] = vuz[u];
+        a.iddst[d] = vid[u];
+      }
     }
   }
 }
