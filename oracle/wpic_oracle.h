@@ -108,6 +108,59 @@ void or2d_energy(const or2d_params *p, const double *up, const double *uf,
 void or2d_gauss(const or2d_params *p, double *G, const double *uf,
                 const double *up, const int *np2);
 
+/* ---------------------------------------------------------------------------
+ * 3D (the 3d/common modules), periodic in x, y and z at one rank.
+ *   up(ndim=7, np, nys:nye, nzs:nze, nsp), uf(6, nxgs-2:nxge+2, nys-2:nye+2,
+ *   nzs-2:nze+2), cumcnt(nxgs:nxge+1, nys:nye, nzs:nze, nsp), np2(nys:nye,
+ *   nzs:nze, nsp).
+ * Pinning: or3d_particle_solv and or3d_sort_bucket are bit-exact against the
+ * reference's own 3d/common/particle.f90 and sort.f90 (oracle/_ref); the
+ * deposit by discrete continuity; the CG by an FFT solve; the step by Gauss's
+ * law; the y-z exchange order (boundary_periodic__particle_yz) is parity
+ * unpinned.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int ndim, np, nsp;
+  int nxgs, nxge, nygs, nyge, nzgs, nzge, nys, nye, nzs, nze;
+  double delx, delt, c, gfac;
+  double q[OR_MAXSP], r[OR_MAXSP];
+} or3d_params;
+
+typedef struct {
+  double *df, *gkl, *uj;
+  int cg_iters[3];
+} or3d_fstate;
+
+int or3d_fstate_init(const or3d_params *p, or3d_fstate *st);
+void or3d_fstate_free(or3d_fstate *st);
+void or3d_field_consts(const or3d_params *p, double f[5]);
+/* 3d/common/particle.f90:52-233 */
+void or3d_particle_solv(const or3d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt);
+/* 3d/common/field.f90:211-406 */
+void or3d_ele_cur(const or3d_params *p, double *uj, const double *up, const double *gp, const int *cumcnt);
+/* 3d/common/boundary_periodic.f90:676-978, 458-673, 981-1099 */
+void or3d_bc_curre(const or3d_params *p, double *uj);
+void or3d_bc_dfield(const or3d_params *p, double *df);
+void or3d_bc_phi(const or3d_params *p, double *phi, int l);
+/* 3d/common/field.f90:409-560 */
+int or3d_cgm(const or3d_params *p, or3d_fstate *st);
+/* 3d/common/field.f90:70-208 */
+int or3d_field_fdtd_i(const or3d_params *p, or3d_fstate *st, double *uf, const double *up, const double *gp,
+                      const int *cumcnt);
+/* 3d/common/boundary_periodic.f90:68-101 (round to nearest) */
+void or3d_bc_particle_x(const or3d_params *p, double *up, const int *np2);
+/* 3d/common/boundary_periodic.f90:104-455 */
+int or3d_bc_particle_yz(const or3d_params *p, double *up, int *np2);
+/* 3d/common/sort.f90:40-88 */
+void or3d_sort_bucket(const or3d_params *p, double *gp, const double *up, int *cumcnt, const int *np2);
+/* 3d/proj/weibel/app.f90:103-108 */
+int or3d_step(const or3d_params *p, or3d_fstate *st, double *up, double *gp, double *uf, int *cumcnt, int *np2,
+              int nsteps);
+/* out = {sum r(gamma-1), sum E^2/8pi, sum B^2/8pi} over interior cells */
+void or3d_energy(const or3d_params *p, const double *up, const double *uf, const int *np2, double out[3]);
+/* G(nx, ny, nz) = div E - 4 pi rho at cell centres */
+void or3d_gauss(const or3d_params *p, double *G, const double *uf, const double *up, const int *np2);
+
 /* round-toward-minus-infinity arithmetic used by the periodic wraps
  * (ieee_set_rounding_mode(ieee_down), boundary_periodic.f90:74,124) */
 double or_add_rd(double a, double b);
