@@ -254,3 +254,29 @@ def test_walls_steps_parity_mode_bit_exact():
     ok, info = _active_equal(u_dev, up, np2)
     assert ok, info
     assert np.array_equal(_bits(uf_dev), _bits(uf))
+
+
+@pytest.mark.parametrize("bc", [po.BC_PERIODIC, po.BC_WALLX])
+def test_fast_mode_one_step_particles_exact_fields_close(bc):
+    """Fast mode's fused step: the push, walls, wraps, exchange and sort are the
+    parity kernels' arithmetic, so after one step the particles equal the
+    oracle's bit for bit; the factored deposit changes only the rounding of uj,
+    so the fields agree to a relative 1e-12."""
+    from wumingpic_amd import MODE_FAST
+    p, up, uf, cc, np2 = _physical_state(24, 16, 8, 23, bc=bc, fscale=0.0 if bc == po.BC_WALLX else 0.02)
+    with _pic(p, MODE_FAST, capacity=3 * 8 * 24 * 16) as sim:
+        sim.upload_up(up, np2, cc)
+        sim.upload_fields(uf)
+        sim.step(1)
+        u_dev, n2_dev, cc_dev = sim.download_up(p.np)
+        uf_dev = sim.download_fields()
+        uj_dev = sim.download_current()
+    o = po.Oracle2D(p)
+    gp = np.zeros_like(up, order="F")
+    o.step(up, gp, uf, cc, np2, 1)
+    assert np.array_equal(n2_dev, np2)
+    assert np.array_equal(cc_dev, cc)
+    ok, info = _active_equal(u_dev, up, np2)
+    assert ok, info
+    assert np.abs(uj_dev - o.uj).max() <= 1e-13 * np.abs(o.uj).max()
+    assert np.abs(uf_dev - uf).max() <= 1e-12 * np.abs(uf).max()

@@ -52,6 +52,53 @@ __device__ inline double cell_field(const Grid2D& g, const double* uf, int c, in
   }
 }
 
+// Buneman-Boris push of one particle in cell (i, j) (particle.f90:98-163),
+// fields gathered from the tile's staged cell-centred values.  Shared by the
+// parity and fast kernels, so both produce bit-identical gp.  gmv returns the
+// 1/gamma of the move.
+__device__ inline void boris_push(const double (*sf)[SFW * SFH], int li, int lj, int i, int j, double d_delx,
+                                  double delt, double c, bool unit_c, double fac1, double txxx, double fac2,
+                                  double x, double y, double ux, double uy, double uz, double& nxp, double& nyp,
+                                  double& g3, double& g4, double& g5, double& gmv) {
+  double sx0, sx1, sx2, sy0, sy1, sy2;
+  shape3(x * d_delx - 0.5 - i, sx0, sx1, sx2);
+  shape3(y * d_delx - 0.5 - j, sy0, sy1, sy2);
+  // re-read the cell's staged fields from LDS for every particle (the group's
+  // lanes read the same words: a broadcast) instead of holding all 54 in
+  // registers across the particle loop
+  int soff = (lj + 1) * SFW + (li + 1);
+  asm volatile("" : "+v"(soff));
+  double fl[6];
+#pragma unroll
+  for (int cmp = 0; cmp < 6; cmp++) {
+    const double* f = &sf[cmp][soff];
+    fl[cmp] = +(+f[-SFW - 1] * sx0 + f[-SFW] * sx1 + f[-SFW + 1] * sx2) * sy0 +
+              (+f[-1] * sx0 + f[0] * sx1 + f[1] * sx2) * sy1 +
+              (+f[SFW - 1] * sx0 + f[SFW] * sx1 + f[SFW + 1] * sx2) * sy2;
+  }
+  const double bpx = fl[0], bpy = fl[1], bpz = fl[2], epx = fl[3], epy = fl[4], epz = fl[5];
+  double uvm1 = ux + fac1 * epx;
+  double uvm2 = uy + fac1 * epy;
+  double uvm3 = uz + fac1 * epz;
+  double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+  const double igam = 1. / gam;
+  const double fac1r = fac1 * igam;
+  const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+  const double uvm4 = uvm1 + fac1r * (+uvm2 * bpz - uvm3 * bpy);
+  const double uvm5 = uvm2 + fac1r * (+uvm3 * bpx - uvm1 * bpz);
+  const double uvm6 = uvm3 + fac1r * (+uvm1 * bpy - uvm2 * bpx);
+  uvm1 = uvm1 + fac2r * (+uvm5 * bpz - uvm6 * bpy);
+  uvm2 = uvm2 + fac2r * (+uvm6 * bpx - uvm4 * bpz);
+  uvm3 = uvm3 + fac2r * (+uvm4 * bpy - uvm5 * bpx);
+  g3 = uvm1 + fac1 * epx;
+  g4 = uvm2 + fac1 * epy;
+  g5 = uvm3 + fac1 * epz;
+  const double u2 = +g3 * g3 + g4 * g4 + g5 * g5;
+  gmv = 1.0 / sqrt(1.0 + (unit_c ? u2 : u2 / (c * c)));
+  nxp = x + g3 * delt * gmv;
+  nyp = y + g4 * delt * gmv;
+}
+
 // register layout of one cell's current block:
 //   [0,20)  pjx(ip,jp), ip = -1..2, jp = -2..2  -> (jp+2)*4 + ip+1
 //   [20,40) pjy(ip,jp), ip = -2..2, jp = -1..2  -> 20 + (jp+1)*5 + ip+2
@@ -131,6 +178,34 @@ __device__ inline void flush_block(double (&v)[N], double* pjbuf, double (*acc)[
   }
 }
 
+// boundary_periodic__particle_x (:74-88, x wrap under round-down, periodic
+// only) and particle_y's row test and y wrap (:120-154) for a particle of row
+// j; returns the exchange code (new x cell << 2 | row step + 1).
+template <int BC>
+__device__ inline int wrap_code(const Grid2D& g, double& nxp, double& nyp, int j, int* err) {
+  if (BC == WPIC_BC_PERIODIC) {
+    const int ipos = (int)rd_div(nxp, g.delx);
+    if (ipos < g.nxgs)
+      nxp = rd_add(nxp, g.spanx);
+    else if (ipos >= g.nxge + 1)
+      nxp = rd_add(nxp, -g.spanx);
+  }
+  const int jpos = (int)rd_div(nyp, g.delx);
+  if (jpos != j) {
+    if (jpos <= g.nygs - 1)
+      nyp = rd_add(nyp, g.spany);
+    else if (jpos >= g.nyge + 1)
+      nyp = rd_add(nyp, -g.spany);
+  }
+  int ni = (int)nxp, dy = jpos - j;
+  if (ni < g.nxgs || ni > g.nxge || dy < -1 || dy > 1) {
+    atomicOr(err, ERR_DOMAIN);
+    ni = ni < g.nxgs ? g.nxgs : (ni > g.nxge ? g.nxge : ni);
+    dy = dy < -1 ? -1 : (dy > 1 ? 1 : dy);
+  }
+  return ((ni - g.nxgs) << 2) | (dy + 1);
+}
+
 // One pass over a cell's particles accumulating the current components
 // selected by DZ/DX/DY from the old (A) and new (B) positions, then flushing
 // them.  With WRAP it also applies boundary_periodic__particle_x/_y's
@@ -197,29 +272,10 @@ __device__ inline void deposit_pass(const PushArgs& a, double (*acc)[ACW * ACH],
         }
       }
       if (WRAP) {
-        if (BC == WPIC_BC_PERIODIC) {
-          const int ipos = (int)rd_div(nxp, g.delx);
-          if (ipos < g.nxgs)
-            nxp = rd_add(nxp, g.spanx);
-          else if (ipos >= g.nxge + 1)
-            nxp = rd_add(nxp, -g.spanx);
-        }
-        const int jpos = (int)rd_div(nyp, g.delx);
-        if (jpos != j) {
-          if (jpos <= g.nygs - 1)
-            nyp = rd_add(nyp, g.spany);
-          else if (jpos >= g.nyge + 1)
-            nyp = rd_add(nyp, -g.spany);
-        }
-        int ni = (int)nxp, dy = jpos - j;
-        if (ni < g.nxgs || ni > g.nxge || dy < -1 || dy > 1) {
-          atomicOr(a.err, ERR_DOMAIN);
-          ni = ni < g.nxgs ? g.nxgs : (ni > g.nxge ? g.nxge : ni);
-          dy = dy < -1 ? -1 : (dy > 1 ? 1 : dy);
-        }
+        const int code = wrap_code<BC>(g, nxp, nyp, j, a.err);
         a.B.x[pi] = nxp;
         a.B.y[pi] = nyp;
-        a.code[pi] = ((ni - g.nxgs) << 2) | (dy + 1);
+        a.code[pi] = code;
       }
     }
   }
@@ -286,43 +342,9 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
         const double x = a.A.x[pi], y = a.A.y[pi];
         double nxp, nyp, g3, g4, g5;
         if (PUSH) {
-          double sx0, sx1, sx2, sy0, sy1, sy2;
-          shape3(x * d_delx - 0.5 - i, sx0, sx1, sx2);
-          shape3(y * d_delx - 0.5 - j, sy0, sy1, sy2);
-          // re-read the cell's staged fields from LDS for every particle (the
-          // group's lanes read the same words: a broadcast) instead of holding
-          // all 54 in registers across the loop
-          int soff = (lj + 1) * SFW + (li + 1);
-          asm volatile("" : "+v"(soff));
-          double fl[6];
-#pragma unroll
-          for (int cmp = 0; cmp < 6; cmp++) {
-            const double* f = &sf[cmp][soff];
-            fl[cmp] = +(+f[-SFW - 1] * sx0 + f[-SFW] * sx1 + f[-SFW + 1] * sx2) * sy0 +
-                      (+f[-1] * sx0 + f[0] * sx1 + f[1] * sx2) * sy1 +
-                      (+f[SFW - 1] * sx0 + f[SFW] * sx1 + f[SFW + 1] * sx2) * sy2;
-          }
-          const double bpx = fl[0], bpy = fl[1], bpz = fl[2], epx = fl[3], epy = fl[4], epz = fl[5];
-          double uvm1 = a.A.ux[pi] + fac1 * epx;
-          double uvm2 = a.A.uy[pi] + fac1 * epy;
-          double uvm3 = a.A.uz[pi] + fac1 * epz;
-          double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
-          const double igam = 1. / gam;
-          const double fac1r = fac1 * igam;
-          const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
-          const double uvm4 = uvm1 + fac1r * (+uvm2 * bpz - uvm3 * bpy);
-          const double uvm5 = uvm2 + fac1r * (+uvm3 * bpx - uvm1 * bpz);
-          const double uvm6 = uvm3 + fac1r * (+uvm1 * bpy - uvm2 * bpx);
-          uvm1 = uvm1 + fac2r * (+uvm5 * bpz - uvm6 * bpy);
-          uvm2 = uvm2 + fac2r * (+uvm6 * bpx - uvm4 * bpz);
-          uvm3 = uvm3 + fac2r * (+uvm4 * bpy - uvm5 * bpx);
-          g3 = uvm1 + fac1 * epx;
-          g4 = uvm2 + fac1 * epy;
-          g5 = uvm3 + fac1 * epz;
-          const double u2 = +g3 * g3 + g4 * g4 + g5 * g5;
-          gam = 1.0 / sqrt(1.0 + (unit_c ? u2 : u2 / cc2));
-          nxp = x + g3 * delt * gam;
-          nyp = y + g4 * delt * gam;
+          double gmv;
+          boris_push(sf, li, lj, i, j, d_delx, delt, c, unit_c, fac1, txxx, fac2, x, y, a.A.ux[pi], a.A.uy[pi],
+                     a.A.uz[pi], nxp, nyp, g3, g4, g5, gmv);
         } else {
           nxp = a.B.x[pi];
           nyp = a.B.y[pi];
@@ -371,6 +393,139 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
       if (I > g.nxge + 2 || J > g.nye + 2) continue;
       unsafeAtomicAdd(a.uj + pl * comp + fidx(g, I, J), v);
     }
+  }
+}
+
+// Fast mode, push and deposit in one pass over each particle.  The push is the
+// shared bit-exact boris_push; the current is the same Esirkepov weights in a
+// factored form that reorders the reference's sums (so uj agrees to rounding,
+// not bit for bit):
+//   Jx(ip, jp) += cx(ip) * wy(jp),  cx(m) = -sum_{k<=m} dsx(k),  wy = qk (s0y + ds_y/2)
+//   Jy(ip, jp) += cy(jp) * wx(ip),  likewise with x and y exchanged
+//   Jz(ip, jp) += s0x(ip) * A(jp) + dsx(ip) * B(jp),
+//                 A = q vz (s0y + dsy/2),  B = q vz (s0y/2 + dsy/3)
+// which expands to field.f90:274-298 term by term.  With FUSED the wraps and
+// the exchange code follow in the same pass; otherwise gp keeps the unwrapped
+// positions as particle__solv leaves them.
+template <int G, bool FUSED, int BC>
+__global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
+  __shared__ double sf[6][SFW * SFH];
+  __shared__ double acc[3][ACW * ACH];
+
+  const Grid2D& g = a.g;
+  const int ntiles = a.ntx * a.nty;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  const int i0 = g.nxgs + (tile % a.ntx) * TX;
+  const int j0 = g.nys + (tile / a.ntx) * TY;
+  const int tid = threadIdx.x;
+
+  for (int k = tid; k < 6 * SFW * SFH; k += NT) {
+    const int c = k / (SFW * SFH), rem = k - c * (SFW * SFH);
+    const int i = i0 - 1 + rem % SFW, j = j0 - 1 + rem / SFW;
+    if (i <= g.nxge + 1 && j <= g.nye + 1) sf[c][rem] = cell_field(g, a.uf, c, i, j);
+  }
+  for (int k = tid; k < 3 * ACW * ACH; k += NT) (&acc[0][0])[k] = 0.0;
+  __syncthreads();
+
+  const int grp = tid / G, lg = tid % G;
+  constexpr int NGRP = NT / G;
+  const double d_delx = g.d_delx, delt = g.delt, c = g.c;
+  const bool unit_c = c * c == 1.0;
+
+  for (int cc = grp; cc < TX * TY; cc += NGRP) {
+    const int li = cc % TX, lj = cc / TX;
+    const int i = i0 + li, j = j0 + lj;
+    if (i > g.nxge || j > g.nye) continue;  // uniform within a group
+    double ax[20], ay[20], az[25];
+#pragma unroll
+    for (int k = 0; k < 20; k++) ax[k] = ay[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 25; k++) az[k] = 0.0;
+
+    for (int s = 0; s < g.nsp; s++) {
+      const int rw = s * g.ny + (j - g.nys);
+      const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
+      const int base = a.rowbase[rw];
+      const int pbeg = base + cr[i - g.nxgs], pend = base + cr[i - g.nxgs + 1];
+      const double q = g.q[s], rm = g.r[s];
+      const double fac1 = q / rm * 0.5 * delt;
+      const double txxx = fac1 * fac1;
+      const double fac2 = q * delt / rm;
+      const double qk = q * g.delx * g.d_delt;
+      for (int pi = pbeg + lg; pi < pend; pi += G) {
+        const double x = a.A.x[pi], y = a.A.y[pi];
+        double nxp, nyp, g3, g4, g5, gmv;
+        boris_push(sf, li, lj, i, j, d_delx, delt, c, unit_c, fac1, txxx, fac2, x, y, a.A.ux[pi], a.A.uy[pi],
+                   a.A.uz[pi], nxp, nyp, g3, g4, g5, gmv);
+        if (FUSED && BC == WPIC_BC_WALLX) {
+          const int ipos = (int)(nxp / g.delx);
+          if (ipos < g.nxgs + 1) {
+            nxp = g.wall_lo - nxp;
+            g3 = -g3; g4 = -g4; g5 = -g5;
+          } else if (ipos >= g.nxge - 1) {
+            nxp = g.wall_hi - nxp;
+            g3 = -g3; g4 = -g4; g5 = -g5;
+          }
+        }
+        a.B.ux[pi] = g3;
+        a.B.uy[pi] = g4;
+        a.B.uz[pi] = g5;
+
+        double s0x[5], s0y[5], dsx[5], dsy[5];
+        esirkepov_shapes(g, x, y, nxp, nyp, i, j, s0x, s0y, dsx, dsy);
+        const double qvz = q * (g5 * gmv);
+        double cx = 0.0, cy = 0.0;
+        double wx[5], wy[5], A[5], B[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+          const double ty = fma(0.5, dsy[k], s0y[k]);
+          wy[k] = qk * ty;
+          A[k] = qvz * ty;
+          B[k] = qvz * fma(1.0 / 3.0, dsy[k], 0.5 * s0y[k]);
+          wx[k] = qk * fma(0.5, dsx[k], s0x[k]);
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          cx -= dsx[m];
+          cy -= dsy[m];
+#pragma unroll
+          for (int jp = 0; jp < 5; jp++) ax[jp * 4 + m] = fma(cx, wy[jp], ax[jp * 4 + m]);
+#pragma unroll
+          for (int ip = 0; ip < 5; ip++) ay[m * 5 + ip] = fma(cy, wx[ip], ay[m * 5 + ip]);
+        }
+#pragma unroll
+        for (int jp = 0; jp < 5; jp++)
+#pragma unroll
+          for (int ip = 0; ip < 5; ip++) {
+            double t = fma(dsx[ip], B[jp], az[jp * 5 + ip]);
+            if (ip >= 1 && ip <= 3) t = fma(s0x[ip], A[jp], t);
+            az[jp * 5 + ip] = t;
+          }
+
+        if (FUSED) {
+          const int code = wrap_code<BC>(g, nxp, nyp, j, a.err);
+          a.code[pi] = code;
+        }
+        a.B.x[pi] = nxp;
+        a.B.y[pi] = nyp;
+      }
+    }
+    const int cellidx = cidx(g, i, j);
+    flush_block<25, 40, G, false>(az, a.pjbuf, acc, cellidx, li, lj, lg);
+    flush_block<20, 0, G, false>(ax, a.pjbuf, acc, cellidx, li, lj, lg);
+    flush_block<20, 20, G, false>(ay, a.pjbuf, acc, cellidx, li, lj, lg);
+  }
+
+  __syncthreads();
+  const size_t pl = (size_t)g.fw * g.fh;
+  for (int k = tid; k < 3 * ACW * ACH; k += NT) {
+    const int comp = k / (ACW * ACH), rem = k - comp * (ACW * ACH);
+    const double v = acc[comp][rem];
+    if (v == 0.0) continue;
+    const int I = i0 - 2 + rem % ACW, J = j0 - 2 + rem / ACW;
+    if (I > g.nxge + 2 || J > g.nye + 2) continue;
+    unsafeAtomicAdd(a.uj + pl * comp + fidx(g, I, J), v);
   }
 }
 
@@ -437,6 +592,14 @@ hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused)
   if (parity) {
     return c->g.bc == WPIC_BC_WALLX ? launch_t<1, true, WPIC_BC_WALLX>(c, push, deposit, fused, a, nblk)
                                     : launch_t<1, true, WPIC_BC_PERIODIC>(c, push, deposit, fused, a, nblk);
+  }
+  if (push && deposit && fused) {
+    constexpr int GF = 4;
+    if (c->g.bc == WPIC_BC_WALLX)
+      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_WALLX>), dim3(nblk), dim3(NT), 0, c->st, a);
+    else
+      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_PERIODIC>), dim3(nblk), dim3(NT), 0, c->st, a);
+    return hipGetLastError();
   }
   return c->g.bc == WPIC_BC_WALLX ? launch_t<8, false, WPIC_BC_WALLX>(c, push, deposit, fused, a, nblk)
                                   : launch_t<8, false, WPIC_BC_PERIODIC>(c, push, deposit, fused, a, nblk);
