@@ -12,7 +12,12 @@ Inputs are resident in HBM before the timed region.
 Prints one JSON line: particle-pushes/s for the whole job, the roofline of the
 dominant kernel (push+deposit, HIP-event timed on the library's own stream)
 and a CPU baseline: the oracle (an OpenMP C restatement of the reference step)
-timed on this host's cores on a bounded sample of the same workload.
+timed on this host's cores on a bounded sample of the same workload.  The
+"mixed" object is the mixed-precision form (int32 cell + FP32 offsets, FP32
+push and deposit, FP64 field solve) run from the same seeded state for the
+same steps, with its speed-up and its field and energy error against the FP64
+run.  The "traffic" of the roofline comes from the rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary committed under profiles/ for this kernel, when present.
 """
 import argparse
 import json
@@ -30,12 +35,24 @@ PUSH_DEPOSIT_BYTES = 80
 # one particle-step of the whole loop: the 48-byte record read once and
 # written once to its sorted slot
 STEP_BYTES = 96
+# the kernel that implements the fused push + deposit of the FP64 fast mode
+PUSH_DEPOSIT_KERNEL = "k_push_deposit_fast<16, true, 0>"
+# mixed form: read and write the 24-byte record (cell, 2 offsets, 3 momenta)
+PUSH_DEPOSIT_BYTES_MIXED = 48
 
 WORKLOADS = {
     "weibel1024": dict(nx=1024, ny=1024, n_ppc=64,
                        name="2D Weibel beams 1024x1024, 64 ppc/species, FP64 path"),
     "weibel64": dict(nx=64, ny=64, n_ppc=16, name="2D Weibel beams 64x64, 16 ppc/species, FP64 path"),
+    # a 3D run for reference figures (not the headline line): 3d/proj/weibel at dt = 0.5
+    "weibel3d256": dict(nx=256, ny=256, nz=256, n_ppc=16,
+                        name="3D Weibel beams 256x256x256, 16 ppc/species, FP64 path"),
+    "weibel3d64": dict(nx=64, ny=64, nz=64, n_ppc=16, name="3D Weibel beams 64x64x64, 16 ppc/species, FP64 path"),
 }
+# algorithmic bytes of one 3D particle through the push (read x,y,z,u and
+# write them: 12 doubles) and through the deposit (read old and new x,y,z)
+PUSH3D_BYTES = 96
+DEPOSIT3D_BYTES = 48
 
 
 def parse():
@@ -48,6 +65,7 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=128, help="rows of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
     ap.add_argument("--probe", choices=["copy"], default=None,
                     help="run a plain device copy of --probe-mib MiB (read) instead, to calibrate "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -96,8 +114,104 @@ def cpu_baseline(w, rows, steps):
                       f"{dt:.2f} s"}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary"""
+    path = os.path.join(ROOT, "profiles", "push_deposit_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel:
+        return None
+    return t.get("traffic_bytes_per_launch")
+
+
+def timed_run(sim, warmup, steps):
+    import torch
+    sim.step(warmup)
+    torch.cuda.synchronize()
+    sim.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def mixed_run(w, a, npart, elapsed64, uf64, energy64):
+    """the mixed-precision form over the same steps from the same seed"""
+    import numpy as np
+
+    from wumingpic_amd import MODE_MIXED
+    from wumingpic_amd.pic2d import weibel_beams_2d
+
+    sim = weibel_beams_2d(w["nx"], w["ny"], w["n_ppc"], mode=MODE_MIXED)
+    elapsed = timed_run(sim, a.warmup, a.steps)
+    log(f"mixed form: {a.warmup} warmup + {a.steps} timed steps: {elapsed:.3f} s")
+    pd_ms, pd_n = sim.timing("push_deposit")
+    phases = {name: sim.timing(name)[0] / a.steps for name in ("push_deposit", "field", "cg", "exchange", "sort")}
+    sim.set_timing(False)
+    energy = sim.energy()
+    uf = sim.download_fields()
+    sim.close()
+    names = ("Bx", "By", "Bz", "Ex", "Ey", "Ez")
+    err = {}
+    for c, nm in enumerate(names):
+        ref = uf64[c]
+        scale = float(np.abs(ref).max())
+        err[nm] = float(np.abs(uf[c] - ref).max() / scale) if scale > 0 else None
+    pd_avg_s = pd_ms / max(pd_n, 1) / 1e3
+    return {
+        "value": npart * a.steps / elapsed,
+        "unit": "particle-pushes/s",
+        "ms_per_step": elapsed / a.steps * 1e3,
+        "speedup_vs_f64": elapsed64 / elapsed,
+        "dtype": "int32 cell + f32 offsets/momenta; f32 push/deposit; f64 field solve and current sum",
+        "push_deposit": {"avg_launch_ms": pd_avg_s * 1e3, "bytes_per_particle": PUSH_DEPOSIT_BYTES_MIXED,
+                         "achieved_GBs": PUSH_DEPOSIT_BYTES_MIXED * npart / pd_avg_s / 1e9},
+        "phase_ms_per_step": phases,
+        "steps_compared": a.warmup + a.steps,
+        "field_rel_err_vs_f64": err,
+        "energy_rel_err_vs_f64": {k: float(abs(energy[i] - energy64[i]) / abs(energy64[i]))
+                                  for i, k in enumerate(("kinetic", "E", "B"))},
+    }
+
+
+def main3d(a, w):
+    """3D figures: pushes/s and the per-phase times of the fused 3D step"""
+    from wumingpic_amd import MODE_FAST, MODE_PARITY
+    from wumingpic_amd.pic3d import weibel_beams_3d
+
+    mode = MODE_PARITY if a.mode == "parity" else MODE_FAST
+    sim = weibel_beams_3d(w["nx"], w["ny"], w["nz"], w["n_ppc"], mode=mode,
+                          capacity=int(w["n_ppc"] * w["nx"] * w["ny"] * w["nz"] * 1.05))
+    npart = int(sim.counts().sum())
+    log(f"initial state ready: {w['name']}, {npart} particles")
+    elapsed = timed_run(sim, a.warmup, a.steps)
+    log(f"{a.warmup} warmup + {a.steps} timed steps: {elapsed:.3f} s")
+    ph = {name: sim.timing(name) for name in ("push", "deposit", "field", "cg", "exchange", "sort")}
+    cg = sim.cg_iters()
+    sim.close()
+    push_s = ph["push"][0] / max(ph["push"][1], 1) / 1e3
+    dep_s = ph["deposit"][0] / max(ph["deposit"][1], 1) / 1e3
+    out = {
+        "metric": "particle-pushes/s (push+deposit+sort) per GPU, 3D",
+        "value": npart * a.steps / elapsed, "unit": "particle-pushes/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded counter-based initial state, generated on the device)",
+        "config": {"workload": w["name"], "particles": npart, "mode": a.mode},
+        "phase_ms_per_step": {k: v[0] / a.steps for k, v in ph.items()},
+        "push_GBs": PUSH3D_BYTES * npart / push_s / 1e9,
+        "deposit_GBs": DEPOSIT3D_BYTES * npart / dep_s / 1e9,
+        "cg_iters_last_step": list(cg),
+    }
+    print(json.dumps(out))
 
 
 def main():
@@ -115,30 +229,28 @@ def main():
     from wumingpic_amd.pic2d import weibel_beams_2d
 
     w = WORKLOADS[a.workload]
+    if "nz" in w:
+        main3d(a, w)
+        return
     nx, ny, n_ppc = w["nx"], w["ny"], w["n_ppc"]
     mode = MODE_PARITY if a.mode == "parity" else MODE_FAST
     sim = weibel_beams_2d(nx, ny, n_ppc, mode=mode)
     npart = int(sim.counts().sum())
     ncell = nx * ny
     log(f"initial state ready: {w['name']}, {npart} particles")
-    sim.step(a.warmup)
-    torch.cuda.synchronize()
-    log(f"{a.warmup} warmup steps done")
-    sim.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sim.step(a.steps)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    log(f"{a.steps} timed steps in {elapsed:.3f} s")
+    elapsed = timed_run(sim, a.warmup, a.steps)
+    log(f"{a.warmup} warmup + {a.steps} timed steps: {elapsed:.3f} s")
     pd_ms, pd_n = sim.timing("push_deposit")
     fld_ms, fld_n = sim.timing("field")
     phases = {name: sim.timing(name)[0] / a.steps for name in ("push_deposit", "field", "cg", "exchange", "sort")}
     cg_iters = sim.cg_iters()
     sim.set_timing(False)
     energy = sim.energy()
+    uf64 = sim.download_fields()
     sim.close()
+    mixed = None
+    if not a.no_mixed:
+        mixed = mixed_run(w, a, npart, elapsed, uf64, energy)
 
     pd_avg_s = pd_ms / max(pd_n, 1) / 1e3
     achieved = PUSH_DEPOSIT_BYTES * npart / pd_avg_s / 1e9
@@ -158,7 +270,8 @@ def main():
         "config": {"workload": w["name"], "nx": nx, "ny": ny, "particles_per_cell_per_species": n_ppc,
                    "particles": npart, "mode": a.mode, "parallelism": "single GPU"},
         "roofline": {"kernel": "push_deposit", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(PUSH_DEPOSIT_KERNEL),
                      "bytes_per_particle": PUSH_DEPOSIT_BYTES, "avg_launch_ms": pd_avg_s * 1e3,
                      "launches": pd_n},
         "step_bandwidth_GBs": STEP_BYTES * npart * a.steps / elapsed / 1e9,
@@ -168,6 +281,7 @@ def main():
         "cg_iters_last_step": list(cg_iters),
         "energy_final": {"kinetic": energy[0], "E": energy[1], "B": energy[2]},
         "cpu_baseline": None,
+        "mixed": mixed,
     }
     if not a.no_cpu_baseline:
         log("timing the CPU baseline")

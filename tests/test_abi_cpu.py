@@ -9,11 +9,12 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "wpic.h")
+HEADER3D = os.path.join(ROOT, "include", "wpic3d.h")
 LIB = os.path.join(ROOT, "wumingpic_amd", "libwpic.so")
 
 
-def declared():
-    text = open(HEADER).read()
+def declared(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(wpic_[a-z0-9_]+)\s*\(", text)))
 
@@ -22,8 +23,9 @@ def test_library_built():
     assert os.path.exists(LIB), "build with `python -m wumingpic_amd.build` (or __graft_entry__.build())"
 
 
-def test_every_declared_symbol_is_exported():
-    names = declared()
+@pytest.mark.parametrize("header", [HEADER, HEADER3D])
+def test_every_declared_symbol_is_exported(header):
+    names = declared(header)
     assert len(names) >= 25
     lib = C.CDLL(LIB)
     missing = [n for n in names if not hasattr(lib, n)]
@@ -36,6 +38,7 @@ def test_every_declared_symbol_is_exported():
 def test_python_binding_covers_the_header():
     from wumingpic_amd import _lib
     assert sorted(_lib.EXPORTS) == declared()
+    assert sorted(_lib.EXPORTS3D) == declared(HEADER3D)
 
 
 def test_missing_library_fails_loudly(monkeypatch):
@@ -61,3 +64,23 @@ def test_invalid_parameters_rejected_without_gpu_work():
     p.capacity = 1000
     assert L.wpic_create2d(C.byref(h), C.byref(p)) == _lib.E_INVAL
     assert b"c*delt" in L.wpic_last_error()
+
+
+def test_invalid_3d_parameters_rejected_without_gpu_work():
+    from wumingpic_amd import _lib
+    L = _lib.lib()
+    p = _lib.Params3D()
+    p.ndim, p.nsp = 6, 2
+    h = C.c_void_p()
+    assert L.wpic_create3d(C.byref(h), C.byref(p)) == _lib.E_INVAL
+    assert b"ndim must be 7" in L.wpic_last_error()
+    p.ndim = 7
+    p.nxgs, p.nxge, p.nygs, p.nyge, p.nzgs, p.nzge = 2, 9, 2, 9, 2, 3
+    p.delx, p.delt, p.c, p.gfac = 1.0, 0.5, 1.0, 0.501
+    p.capacity = 1000
+    assert L.wpic_create3d(C.byref(h), C.byref(p)) == _lib.E_INVAL
+    assert b"4x3x3" in L.wpic_last_error()
+    p.nzge = 9
+    p.delx = 0.5
+    assert L.wpic_create3d(C.byref(h), C.byref(p)) == _lib.E_INVAL
+    assert b"delx" in L.wpic_last_error()

@@ -1,4 +1,5 @@
-"""ctypes binding of libwpic.so (the C-ABI declared in include/wpic.h).
+"""ctypes binding of libwpic.so (the C-ABI declared in include/wpic.h and
+include/wpic3d.h).
 
 The library is loaded from this package directory only; there is no CPU
 fallback: importing the bindings without the built library raises.
@@ -13,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libwpic.so")
 
 MAXSP = 4
 BC_PERIODIC, BC_WALLX = 0, 1
-MODE_FAST, MODE_PARITY = 0, 1
+MODE_FAST, MODE_PARITY, MODE_MIXED = 0, 1, 2
 E_OK, E_INVAL, E_HIP, E_CG, E_MEM, E_DOMAIN = 0, -1, -2, -3, -4, -5
 
 # every symbol include/wpic.h declares
@@ -28,6 +29,18 @@ EXPORTS = (
     "wpic_set_timing", "wpic_timing",
 )
 
+# every symbol include/wpic3d.h declares
+EXPORTS3D = (
+    "wpic_create3d", "wpic_destroy3d", "wpic_set_mode3d", "wpic_sync3d",
+    "wpic_upload_up3d", "wpic_download_up3d", "wpic_upload_gp3d", "wpic_download_gp3d",
+    "wpic_upload_fields3d", "wpic_download_fields3d", "wpic_download_current3d",
+    "wpic_download_dfield3d", "wpic_upload_dfield3d",
+    "wpic_particle_solv3d", "wpic_field_fdtd_i3d", "wpic_bc_particle_x3d", "wpic_bc_particle_yz3d",
+    "wpic_sort_bucket3d", "wpic_cg_iters3d", "wpic_step3d",
+    "wpic_energy3d", "wpic_gauss3d", "wpic_counts3d", "wpic_ic_beams3d",
+    "wpic_set_timing3d", "wpic_timing3d",
+)
+
 
 class Params2D(C.Structure):
     _fields_ = [
@@ -36,6 +49,17 @@ class Params2D(C.Structure):
         ("delx", C.c_double), ("delt", C.c_double), ("c", C.c_double), ("gfac", C.c_double),
         ("q", C.c_double * MAXSP), ("r", C.c_double * MAXSP),
         ("bc", C.c_int), ("mode", C.c_int), ("capacity", C.c_longlong),
+    ]
+
+
+class Params3D(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int), ("nsp", C.c_int),
+        ("nxgs", C.c_int), ("nxge", C.c_int), ("nygs", C.c_int), ("nyge", C.c_int),
+        ("nzgs", C.c_int), ("nzge", C.c_int),
+        ("delx", C.c_double), ("delt", C.c_double), ("c", C.c_double), ("gfac", C.c_double),
+        ("q", C.c_double * MAXSP), ("r", C.c_double * MAXSP),
+        ("mode", C.c_int), ("capacity", C.c_longlong),
     ]
 
 
@@ -87,6 +111,32 @@ def lib():
         "wpic_ic_beams2d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
         "wpic_set_timing": [vp, C.c_int],
         "wpic_timing": [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
+        "wpic_create3d": [C.POINTER(vp), C.POINTER(Params3D)],
+        "wpic_destroy3d": [vp],
+        "wpic_set_mode3d": [vp, C.c_int],
+        "wpic_sync3d": [vp],
+        "wpic_upload_up3d": [vp, d, i, i, C.c_int],
+        "wpic_download_up3d": [vp, d, i, i, C.c_int],
+        "wpic_upload_gp3d": [vp, d, i, C.c_int],
+        "wpic_download_gp3d": [vp, d, i, C.c_int],
+        "wpic_upload_fields3d": [vp, d],
+        "wpic_download_fields3d": [vp, d],
+        "wpic_download_current3d": [vp, d],
+        "wpic_download_dfield3d": [vp, d],
+        "wpic_upload_dfield3d": [vp, d],
+        "wpic_particle_solv3d": [vp],
+        "wpic_field_fdtd_i3d": [vp],
+        "wpic_bc_particle_x3d": [vp],
+        "wpic_bc_particle_yz3d": [vp],
+        "wpic_sort_bucket3d": [vp],
+        "wpic_cg_iters3d": [vp, np.ctypeslib.ndpointer(np.int32)],
+        "wpic_step3d": [vp, C.c_int],
+        "wpic_energy3d": [vp, np.ctypeslib.ndpointer(np.float64)],
+        "wpic_gauss3d": [vp, d],
+        "wpic_counts3d": [vp, np.ctypeslib.ndpointer(np.int64)],
+        "wpic_ic_beams3d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
+        "wpic_set_timing3d": [vp, C.c_int],
+        "wpic_timing3d": [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -118,6 +118,71 @@ hipError_t launch_soa_to_aos(wpic_ctx* c, double* aos, int which) {
   return hipGetLastError();
 }
 
+// mixed-precision transfers: the reference's FP64 AoS rows <-> x cell + FP32
+// offsets and momenta (y offset relative to the row)
+__global__ void k_aos_to_mixed(XferArgs a, PartsM M, const double* __restrict__ aos) {
+  const int rw = blockIdx.x;
+  const int base = a.rowbase[rw], n = a.np2[rw];
+  const int j = a.g.nys + rw % a.g.ny;
+  const double* src = aos + a.aosbase[rw] * 6;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const double* s = src + (size_t)k * 6;
+    const int d = base + k;
+    const int ix = (int)s[0];
+    M.ix[d] = ix;
+    M.ox[d] = to_offset(s[0] - ix);
+    M.oy[d] = to_offset(s[1] - j);
+    M.ux[d] = (float)s[2];
+    M.uy[d] = (float)s[3];
+    M.uz[d] = (float)s[4];
+    uint64_t b;
+    __builtin_memcpy(&b, &s[5], 8);
+    a.id[d] = b;
+  }
+}
+
+__global__ void k_mixed_to_aos(XferArgs a, PartsM M, double* __restrict__ aos) {
+  const int rw = blockIdx.x;
+  const int base = a.rowbase[rw], n = a.np2[rw];
+  const int j = a.g.nys + rw % a.g.ny;
+  double* dst = aos + a.aosbase[rw] * 6;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int s = base + k;
+    double* d = dst + (size_t)k * 6;
+    d[0] = (double)M.ix[s] + (double)M.ox[s];
+    d[1] = (double)j + (double)M.oy[s];
+    d[2] = M.ux[s];
+    d[3] = M.uy[s];
+    d[4] = M.uz[s];
+    const uint64_t b = a.id[s];
+    __builtin_memcpy(&d[5], &b, 8);
+  }
+}
+
+hipError_t launch_up_to_mixed(wpic_ctx* c, const double* aos) {
+  XferArgs a;
+  a.g = c->g;
+  a.id = c->id[c->ida];
+  a.rowbase = c->rowbaseA;
+  a.np2 = c->np2A;
+  a.R = nullptr;
+  a.aosbase = c->aosbase;
+  hipLaunchKernelGGL(k_aos_to_mixed, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, c->MA, aos);
+  return hipGetLastError();
+}
+
+hipError_t launch_mixed_to_aos(wpic_ctx* c, double* aos) {
+  XferArgs a;
+  a.g = c->g;
+  a.id = c->id[c->ida];
+  a.rowbase = c->rowbaseA;
+  a.np2 = c->np2A;
+  a.R = nullptr;
+  a.aosbase = c->aosbase;
+  hipLaunchKernelGGL(k_mixed_to_aos, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, c->MA, aos);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // seeded initial state: counter-based (splitmix64) draws keyed by
 // (seed, stream, row, cell, slot), so the state is independent of the launch
@@ -146,12 +211,16 @@ __device__ inline void normal2(uint64_t h1, uint64_t h2, double& n0, double& n1)
 struct IcArgs {
   Grid2D g;
   Parts A;
+  PartsM M;
   uint64_t* id;
   unsigned long long seed;
   int n_ppc;
   double udrift, uth_e, uth_i;
 };
 
+// MIXED: the same draws stored as x cell + FP32 offsets and momenta, so the
+// FP64 and mixed forms start from the same state up to FP32 rounding
+template <bool MIXED>
 __global__ void k_ic_beams(IcArgs a) {
   const Grid2D& g = a.g;
   const long long per = (long long)a.n_ppc * g.nx * g.ny;
@@ -170,8 +239,6 @@ __global__ void k_ic_beams(IcArgs a) {
   double x = ((double)i + u0) * g.delx, y = ((double)j + u1) * g.delx;
   if ((int)(x * g.d_delx) != i) x = next_down((double)(i + 1) * g.delx);
   if ((int)(y * g.d_delx) != j) y = next_down((double)(j + 1) * g.delx);
-  a.A.x[d] = x;
-  a.A.y[d] = y;
   double n0, n1, n2, n3;
   normal2(rkey(a.seed, 10 + 4 * s, r, ci, sl), rkey(a.seed, 11 + 4 * s, r, ci, sl), n0, n1);
   normal2(rkey(a.seed, 12 + 4 * s, r, ci, sl), rkey(a.seed, 13 + 4 * s, r, ci, sl), n2, n3);
@@ -179,27 +246,53 @@ __global__ void k_ic_beams(IcArgs a) {
   const double th = electron ? a.uth_e : a.uth_i;
   double ux = th * n0;
   if (electron) ux += (sl & 1) ? -a.udrift : a.udrift;
-  a.A.ux[d] = ux;
-  a.A.uy[d] = th * n1;
-  a.A.uz[d] = th * n2;
+  if (MIXED) {
+    a.M.ix[d] = i;
+    a.M.ox[d] = to_offset(x - i);
+    a.M.oy[d] = to_offset(y - j);
+    a.M.ux[d] = (float)ux;
+    a.M.uy[d] = (float)(th * n1);
+    a.M.uz[d] = (float)(th * n2);
+  } else {
+    a.A.x[d] = x;
+    a.A.y[d] = y;
+    a.A.ux[d] = ux;
+    a.A.uy[d] = th * n1;
+    a.A.uz[d] = th * n2;
+  }
   // id = -(pid), pid counted row by row from 1 (app.f90:455-470)
   const long long pid = k + 1;
   a.id[d] = (uint64_t)(-pid);
 }
 
-hipError_t launch_ic_beams(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
-                           double uth_i) {
+static IcArgs ic_args(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
+                      double uth_i) {
   IcArgs a;
   a.g = c->g;
   a.A = c->A;
+  a.M = c->MA;
   a.id = c->id[c->ida];
   a.seed = seed;
   a.n_ppc = n_ppc;
   a.udrift = udrift;
   a.uth_e = uth_e;
   a.uth_i = uth_i;
+  return a;
+}
+
+hipError_t launch_ic_beams(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
+                           double uth_i) {
+  IcArgs a = ic_args(c, seed, n_ppc, udrift, uth_e, uth_i);
   const long long n = (long long)n_ppc * c->g.nx * c->g.ny * c->g.nsp;
-  hipLaunchKernelGGL(k_ic_beams, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_ic_beams<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ic_beams_mixed(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
+                                 double uth_i) {
+  IcArgs a = ic_args(c, seed, n_ppc, udrift, uth_e, uth_i);
+  const long long n = (long long)n_ppc * c->g.nx * c->g.ny * c->g.nsp;
+  hipLaunchKernelGGL(k_ic_beams<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }
 
@@ -218,7 +311,8 @@ __device__ inline double bsum(double v, double* sh) {
 }
 
 // out[blk*3 + {0,1,2}] = partial {kinetic, E^2, B^2}
-__global__ __launch_bounds__(256) void k_energy(Grid2D g, Parts A, const int* rowbase, const int* np2,
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_energy(Grid2D g, Parts A, PartsM M, const int* rowbase, const int* np2,
                                                 const double* uf, double* out) {
   __shared__ double sh[4];
   double kin = 0.0, e2 = 0.0, b2 = 0.0;
@@ -228,7 +322,10 @@ __global__ __launch_bounds__(256) void k_energy(Grid2D g, Parts A, const int* ro
     const int base = rowbase[rw], n = np2[rw];
     for (int k = threadIdx.x; k < n; k += 256) {
       const int p = base + k;
-      const double u2 = A.ux[p] * A.ux[p] + A.uy[p] * A.uy[p] + A.uz[p] * A.uz[p];
+      const double ux = MIXED ? (double)M.ux[p] : A.ux[p];
+      const double uy = MIXED ? (double)M.uy[p] : A.uy[p];
+      const double uz = MIXED ? (double)M.uz[p] : A.uz[p];
+      const double u2 = ux * ux + uy * uy + uz * uz;
       const double gam = sqrt(1 + u2 / (g.c * g.c));
       kin = kin + g.r[s] * (gam - 1);
     }
@@ -251,19 +348,30 @@ __global__ __launch_bounds__(256) void k_energy(Grid2D g, Parts A, const int* ro
 }
 
 hipError_t launch_energy(wpic_ctx* c, double* dout) {
-  hipLaunchKernelGGL(k_energy, dim3(RB), dim3(256), 0, c->st, c->g, c->A, c->rowbaseA, c->np2A, c->uf, dout);
+  hipLaunchKernelGGL(k_energy<false>, dim3(RB), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA, c->np2A, c->uf,
+                     dout);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_mixed(wpic_ctx* c, double* dout) {
+  hipLaunchKernelGGL(k_energy<true>, dim3(RB), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA, c->np2A, c->uf,
+                     dout);
   return hipGetLastError();
 }
 
 // rho with the quadratic shape at cell centres, then G = div E - 4 pi rho
-__global__ __launch_bounds__(256) void k_rho(Grid2D g, Parts A, const int* rowbase, const int* np2, double* rho) {
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_rho(Grid2D g, Parts A, PartsM M, const int* rowbase, const int* np2,
+                                             double* rho) {
   const int rw = blockIdx.x;
   const int s = rw / g.ny;
   const int base = rowbase[rw], n = np2[rw];
   const int gw = g.fw;
+  const int jrow = g.nys + rw % g.ny;
   for (int k = threadIdx.x; k < n; k += 256) {
     const int p = base + k;
-    const double x = A.x[p], y = A.y[p];
+    const double x = MIXED ? (double)M.ix[p] + (double)M.ox[p] : A.x[p];
+    const double y = MIXED ? (double)jrow + (double)M.oy[p] : A.y[p];
     const int i0 = (int)(x * g.d_delx), j0 = (int)(y * g.d_delx);
     if (i0 < g.nxgs || i0 > g.nxge || j0 < g.nys || j0 > g.nye) continue;
     double sx[3], sy[3];
@@ -308,7 +416,12 @@ hipError_t launch_gauss(wpic_ctx* c, double* G, double* rho) {
   const size_t n = (size_t)c->g.fw * c->g.fh;
   hipError_t e = launch_zero(c, rho, n);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_rho, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, c->g, c->A, c->rowbaseA, c->np2A, rho);
+  if (c->mixed)
+    hipLaunchKernelGGL(k_rho<true>, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA,
+                       c->np2A, rho);
+  else
+    hipLaunchKernelGGL(k_rho<false>, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA,
+                       c->np2A, rho);
   if (c->g.bc == WPIC_BC_PERIODIC) {
     hipLaunchKernelGGL(k_rho_fold_y, dim3((c->g.nx + 2 + 255) / 256), dim3(256), 0, c->st, c->g, rho);
     hipLaunchKernelGGL(k_rho_fold_x, dim3((c->g.ny + 255) / 256), dim3(256), 0, c->st, c->g, rho);

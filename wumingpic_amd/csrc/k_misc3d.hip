@@ -1,0 +1,311 @@
+// k_misc3d.hip -- 3D layout conversion at the boundary (reference AoS of
+// ndim = 7 doubles <-> device SoA), the seeded initial state and diagnostics.
+#include "wpic_internal3d.h"
+
+namespace wpic {
+
+struct Xfer3Args {
+  Grid3D g;
+  Parts3 P;
+  uint64_t* id;
+  const int *colbase, *np2;
+  const int* R;              // non-null: B seen through the exchange permutation
+  const long long* aosbase;  // [nsp*ncol] start of each column in the AoS buffer
+};
+
+__global__ void k_aos_to_soa3(Xfer3Args a, const double* __restrict__ aos) {
+  const int rw = blockIdx.x;
+  const int base = a.colbase[rw], n = a.np2[rw];
+  const double* src = aos + a.aosbase[rw] * 7;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const double* s = src + (size_t)q * 7;
+    const int d = base + q;
+    a.P.x[d] = s[0];
+    a.P.y[d] = s[1];
+    a.P.z[d] = s[2];
+    a.P.ux[d] = s[3];
+    a.P.uy[d] = s[4];
+    a.P.uz[d] = s[5];
+    uint64_t b;
+    __builtin_memcpy(&b, &s[6], 8);
+    a.id[d] = b;
+  }
+}
+
+__global__ void k_soa_to_aos3(Xfer3Args a, double* __restrict__ aos) {
+  const int rw = blockIdx.x;
+  const int base = a.colbase[rw], n = a.np2[rw];
+  double* dst = aos + a.aosbase[rw] * 7;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    int s = a.R ? a.R[base + q] : base + q;
+    if ((unsigned)s >= (unsigned)(a.g.ncap * a.g.nsp)) s = 0;
+    double* d = dst + (size_t)q * 7;
+    d[0] = a.P.x[s];
+    d[1] = a.P.y[s];
+    d[2] = a.P.z[s];
+    d[3] = a.P.ux[s];
+    d[4] = a.P.uy[s];
+    d[5] = a.P.uz[s];
+    const uint64_t b = a.id[s];
+    __builtin_memcpy(&d[6], &b, 8);
+  }
+}
+
+__global__ void k_planes_from_aos3(size_t n, int ncomp, const double* __restrict__ aos, double* __restrict__ pl) {
+  const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  for (int c = 0; c < ncomp; c++) pl[(size_t)c * n + q] = aos[q * ncomp + c];
+}
+
+__global__ void k_planes_to_aos3(size_t n, int ncomp, const double* __restrict__ pl, double* __restrict__ aos) {
+  const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  for (int c = 0; c < ncomp; c++) aos[q * ncomp + c] = pl[(size_t)c * n + q];
+}
+
+hipError_t launch_planes_from_aos3d(wpic_ctx3d* c, const double* aos, double* planes, int ncomp) {
+  const size_t n = plane3(c->g);
+  hipLaunchKernelGGL(k_planes_from_aos3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, n, ncomp, aos,
+                     planes);
+  return hipGetLastError();
+}
+
+hipError_t launch_planes_to_aos3d(wpic_ctx3d* c, const double* planes, double* aos, int ncomp) {
+  const size_t n = plane3(c->g);
+  hipLaunchKernelGGL(k_planes_to_aos3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, n, ncomp, planes,
+                     aos);
+  return hipGetLastError();
+}
+
+// which: 0 = A columns, 1 = B seen through R (exchange pending), 2 = B columns
+hipError_t launch_up_to_soa3d(wpic_ctx3d* c, const double* aos, long long n, int which) {
+  (void)n;
+  Xfer3Args a;
+  a.g = c->g;
+  a.P = which == 0 ? c->A : c->B;
+  a.id = which == 0 ? c->id[c->ida] : c->id[c->idb];
+  a.colbase = which == 0 ? c->colbaseA : c->colbaseB;
+  a.np2 = which == 0 ? c->np2A : c->np2B;
+  a.R = nullptr;
+  a.aosbase = c->aosbase;
+  hipLaunchKernelGGL(k_aos_to_soa3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, a, aos);
+  return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos3d(wpic_ctx3d* c, double* aos, int which) {
+  Xfer3Args a;
+  a.g = c->g;
+  a.P = which == 0 ? c->A : c->B;
+  a.id = which == 0 ? c->id[c->ida] : c->id[c->idb];
+  if (which == 0) {
+    a.colbase = c->colbaseA;
+    a.np2 = c->np2A;
+    a.R = nullptr;
+  } else if (which == 1) {
+    a.colbase = c->colbaseN;
+    a.np2 = c->np2N;
+    a.R = c->R;
+  } else {
+    a.colbase = c->colbaseB;
+    a.np2 = c->np2B;
+    a.R = nullptr;
+  }
+  a.aosbase = c->aosbase;
+  hipLaunchKernelGGL(k_soa_to_aos3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, a, aos);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// seeded initial state: counter-based splitmix64 draws keyed by (seed, stream,
+// column, cell, slot); Box-Muller normals
+// ---------------------------------------------------------------------------
+namespace {
+__device__ inline uint64_t smix3(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__device__ inline uint64_t rkey3(uint64_t seed, uint64_t stream, uint64_t a, uint64_t b, uint64_t c) {
+  return smix3(smix3(smix3(smix3(seed ^ 0x3c6ef372fe94f82bull) ^ stream) ^ a) ^ ((b << 20) ^ c));
+}
+__device__ inline double u01_3(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
+__device__ inline void normal2_3(uint64_t h1, uint64_t h2, double& n0, double& n1) {
+  const double u1 = 1.0 - u01_3(h1);
+  const double u2 = u01_3(h2);
+  const double rr = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  n0 = rr * c;
+  n1 = rr * s;
+}
+// a coordinate uniform in cell `cell`, kept inside it after rounding
+__device__ inline double in_cell(int cell, double u, double delx, double d_delx) {
+  double x = ((double)cell + u) * delx;
+  if ((int)(x * d_delx) != cell) x = next_down((double)(cell + 1) * delx);
+  return x;
+}
+}  // namespace
+
+struct Ic3Args {
+  Grid3D g;
+  Parts3 A;
+  uint64_t* id;
+  unsigned long long seed;
+  int n_ppc;
+  double udrift, uth_e, uth_i;
+};
+
+__global__ void k_ic_beams3(Ic3Args a) {
+  const Grid3D& g = a.g;
+  const long long colen = (long long)a.n_ppc * g.nx;
+  const long long per = colen * g.ncol;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= per * g.nsp) return;
+  const int s = (int)(t / per);
+  const long long q = t - s * per;
+  const int col = (int)(q / colen);
+  const long long m = q - col * colen;
+  const int ci = (int)(m / a.n_ppc), sl = (int)(m - (long long)ci * a.n_ppc);
+  const long long d = s * g.ncap + q;
+  const int i = g.nxgs + ci, j = g.nys + col % g.ny, k = g.nzs + col / g.ny;
+  a.A.x[d] = in_cell(i, u01_3(rkey3(a.seed, 1, col, ci, sl)), g.delx, g.d_delx);
+  a.A.y[d] = in_cell(j, u01_3(rkey3(a.seed, 2, col, ci, sl)), g.delx, g.d_delx);
+  a.A.z[d] = in_cell(k, u01_3(rkey3(a.seed, 3, col, ci, sl)), g.delx, g.d_delx);
+  double n0, n1, n2, n3;
+  normal2_3(rkey3(a.seed, 10 + 4 * s, col, ci, sl), rkey3(a.seed, 11 + 4 * s, col, ci, sl), n0, n1);
+  normal2_3(rkey3(a.seed, 12 + 4 * s, col, ci, sl), rkey3(a.seed, 13 + 4 * s, col, ci, sl), n2, n3);
+  const bool electron = s == g.nsp - 1 && g.nsp > 1;
+  const double th = electron ? a.uth_e : a.uth_i;
+  double ux = th * n0;
+  if (electron) ux += (sl & 1) ? -a.udrift : a.udrift;
+  a.A.ux[d] = ux;
+  a.A.uy[d] = th * n1;
+  a.A.uz[d] = th * n2;
+  a.id[d] = (uint64_t)(-(q + 1));
+}
+
+hipError_t launch_ic_beams3d(wpic_ctx3d* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
+                             double uth_i) {
+  Ic3Args a;
+  a.g = c->g;
+  a.A = c->A;
+  a.id = c->id[c->ida];
+  a.seed = seed;
+  a.n_ppc = n_ppc;
+  a.udrift = udrift;
+  a.uth_e = uth_e;
+  a.uth_i = uth_i;
+  const long long n = (long long)n_ppc * c->g.nx * c->g.ncol * c->g.nsp;
+  hipLaunchKernelGGL(k_ic_beams3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int RB3 = 256;
+__device__ inline double bsum3(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return t;
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, const int* colbase, const int* np2,
+                                                 const double* uf, double* out) {
+  __shared__ double sh[4];
+  double kin = 0.0, e2 = 0.0, b2 = 0.0;
+  const int ncols = g.nsp * g.ncol;
+  for (int rw = blockIdx.x; rw < ncols; rw += gridDim.x) {
+    const int s = rw / g.ncol;
+    const int base = colbase[rw], n = np2[rw];
+    for (int q = threadIdx.x; q < n; q += 256) {
+      const int p = base + q;
+      const double u2 = A.ux[p] * A.ux[p] + A.uy[p] * A.uy[p] + A.uz[p] * A.uz[p];
+      kin = kin + g.r[s] * (sqrt(1 + u2 / (g.c * g.c)) - 1);
+    }
+  }
+  const size_t pl = plane3(g);
+  const size_t ncell = (size_t)g.nx * g.ny * g.nz;
+  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < ncell; q += (size_t)gridDim.x * 256) {
+    const int i = g.nxgs + (int)(q % g.nx), j = g.nys + (int)((q / g.nx) % g.ny),
+              k = g.nzs + (int)(q / ((size_t)g.nx * g.ny));
+    const size_t f = fidx3(g, i, j, k);
+    b2 = b2 + uf[f] * uf[f] + uf[pl + f] * uf[pl + f] + uf[2 * pl + f] * uf[2 * pl + f];
+    e2 = e2 + uf[3 * pl + f] * uf[3 * pl + f] + uf[4 * pl + f] * uf[4 * pl + f] + uf[5 * pl + f] * uf[5 * pl + f];
+  }
+  kin = bsum3(kin, sh);
+  e2 = bsum3(e2, sh);
+  b2 = bsum3(b2, sh);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x * 3 + 0] = kin;
+    out[blockIdx.x * 3 + 1] = e2;
+    out[blockIdx.x * 3 + 2] = b2;
+  }
+}
+
+hipError_t launch_energy3d(wpic_ctx3d* c, double* dout) {
+  hipLaunchKernelGGL(k_energy3, dim3(RB3), dim3(256), 0, c->st, c->g, c->A, c->colbaseA, c->np2A, c->uf, dout);
+  return hipGetLastError();
+}
+
+// rho at cell centres with the quadratic shape, wrapped periodically, on the
+// interior array [nx*ny*nz]
+__global__ __launch_bounds__(256) void k_rho3(Grid3D g, Parts3 A, const int* colbase, const int* np2, double* rho) {
+  const int rw = blockIdx.x;
+  const int s = rw / g.ncol;
+  const int base = colbase[rw], n = np2[rw];
+  for (int q = threadIdx.x; q < n; q += 256) {
+    const int p = base + q;
+    const double x = A.x[p], y = A.y[p], z = A.z[p];
+    const int i0 = (int)(x * g.d_delx), j0 = (int)(y * g.d_delx), k0 = (int)(z * g.d_delx);
+    if (i0 < g.nxgs || i0 > g.nxge || j0 < g.nys || j0 > g.nye || k0 < g.nzs || k0 > g.nze) continue;
+    double sx[3], sy[3], sz[3];
+    shape3(x * g.d_delx - 0.5 - i0, sx[0], sx[1], sx[2]);
+    shape3(y * g.d_delx - 0.5 - j0, sy[0], sy[1], sy[2]);
+    shape3(z * g.d_delx - 0.5 - k0, sz[0], sz[1], sz[2]);
+    for (int c = 0; c < 3; c++)
+      for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) {
+          int ii = i0 - 1 + a, jj = j0 - 1 + b, kk = k0 - 1 + c;
+          if (ii < g.nxgs) ii += g.nx;
+          if (ii > g.nxge) ii -= g.nx;
+          if (jj < g.nys) jj += g.ny;
+          if (jj > g.nye) jj -= g.ny;
+          if (kk < g.nzs) kk += g.nz;
+          if (kk > g.nze) kk -= g.nz;
+          unsafeAtomicAdd(&rho[cidx3(g, ii, jj, kk)], g.q[s] * sx[a] * sy[b] * sz[c]);
+        }
+  }
+}
+
+__global__ void k_gauss3(Grid3D g, const double* uf, const double* rho, double* G) {
+  const size_t n = (size_t)g.nx * g.ny * g.nz;
+  const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int i = g.nxgs + (int)(q % g.nx), j = g.nys + (int)((q / g.nx) % g.ny), k = g.nzs + (int)(q / ((size_t)g.nx * g.ny));
+  const size_t pl = plane3(g);
+  const double *ex = uf + 3 * pl, *ey = uf + 4 * pl, *ez = uf + 5 * pl;
+  const double dive = (ex[fidx3(g, i + 1, j, k)] - ex[fidx3(g, i, j, k)]) * g.d_delx +
+                      (ey[fidx3(g, i, j + 1, k)] - ey[fidx3(g, i, j, k)]) * g.d_delx +
+                      (ez[fidx3(g, i, j, k + 1)] - ez[fidx3(g, i, j, k)]) * g.d_delx;
+  G[q] = dive - 4.0 * 3.141592653589793 * rho[q];
+}
+
+hipError_t launch_gauss3d(wpic_ctx3d* c, double* G, double* rho) {
+  const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
+  hipError_t e = launch_fill3d(c, rho, n, 0.0);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_rho3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, c->g, c->A, c->colbaseA,
+                     c->np2A, rho);
+  hipLaunchKernelGGL(k_gauss3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, c->g, c->uf, rho, G);
+  return hipGetLastError();
+}
+
+}  // namespace wpic

@@ -1,0 +1,275 @@
+// k_mixed2d.hip -- the mixed-precision form of the fused push + deposit +
+// wrap (the same physics as 2d/common/particle.f90:48-179 and
+// 2d/common/field.f90:189-316 followed by the particle boundaries).
+//
+// A particle is an int32 x cell plus FP32 offsets (x - ix, y - row) in [0, 1)
+// and FP32 momenta: positions keep ~1e-7 of a cell everywhere in the box, the
+// shape functions see offsets directly, and a particle record is 32 B instead
+// of 48 B.  Push and deposit run in FP32 (FMA contraction on) with the tile's
+// current accumulated in FP32 in LDS; the tile is added to the FP64 uj with
+// FP64 atomics (the global current reduction stays FP64), and the field solve
+// is the FP64 one.  Work mapping as the FP64 fast kernel: one workgroup per
+// 32x4-cell tile, 16-lane groups per cell.
+#include "wpic_internal.h"
+
+namespace wpic {
+
+namespace mx {
+constexpr int TX = 32, TY = 4, NT = 256, G = 16;
+constexpr int SFW = TX + 2, SFH = TY + 2;
+constexpr int ACW = TX + 4, ACH = TY + 4;
+}  // namespace mx
+
+struct MixArgs {
+  Grid2D g;
+  PartsM A, B;
+  const int* rowbase;
+  const int* cumcnt;
+  const double* uf;
+  double* uj;
+  int* code;
+  int* err;
+  int ntx, nty;
+};
+
+// cell-centred field component c at (i, j) (particle.f90:74-80), FP64
+__device__ inline double cell_field_m(const Grid2D& g, const double* uf, int c, int i, int j) {
+  const size_t pl = (size_t)g.fw * g.fh;
+  const double* u = uf + pl * c;
+  switch (c) {
+    case 0: return 0.5 * (u[fidx(g, i, j)] + u[fidx(g, i, j + 1)]);
+    case 1: return 0.5 * (u[fidx(g, i, j)] + u[fidx(g, i + 1, j)]);
+    case 2: return 0.25 * (u[fidx(g, i, j)] + u[fidx(g, i + 1, j)] + u[fidx(g, i, j + 1)] + u[fidx(g, i + 1, j + 1)]);
+    case 3: return 0.5 * (u[fidx(g, i, j)] + u[fidx(g, i + 1, j)]);
+    case 4: return 0.5 * (u[fidx(g, i, j)] + u[fidx(g, i, j + 1)]);
+    default: return u[fidx(g, i, j)];
+  }
+}
+
+__device__ inline void shape3f(float dh, float& s0, float& s1, float& s2) {
+  s0 = 0.5f * (0.5f - dh) * (0.5f - dh);
+  s1 = 0.75f - dh * dh;
+  s2 = 0.5f * (0.5f + dh) * (0.5f + dh);
+}
+
+// ds = (new shape shifted by inc) - s0, index 0..4 <-> -2..2, with the
+// reference's smo_* weights (field.f90:244-266); |inc| <= 1 except for a
+// particle that a reflecting wall sends across two cells, where the formula is
+// applied as the reference applies it
+__device__ inline void ds_of(float s0m, float s00, float s0p, float fnew, int inc, float ds[5]) {
+  float s1m, s10, s1p;
+  shape3f(fnew - 0.5f, s1m, s10, s1p);
+  const float a = (float)abs(inc);
+  const float smo1 = inc < 0 ? a : 0.0f, smo2 = 1.0f - a, smo3 = inc > 0 ? a : 0.0f;
+  ds[0] = s1m * smo1;
+  ds[1] = s1m * smo2 + s10 * smo1 - s0m;
+  ds[2] = s10 * smo2 + s1p * smo1 + s1m * smo3 - s00;
+  ds[3] = s1p * smo2 + s10 * smo3 - s0p;
+  ds[4] = s1p * smo3;
+}
+
+template <int BC>
+__global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
+#pragma clang fp contract(fast)
+  using namespace mx;
+  __shared__ float sf[6][SFW * SFH];
+  __shared__ float acc[3][ACW * ACH];
+
+  const Grid2D& g = a.g;
+  const int ntiles = a.ntx * a.nty;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  const int i0 = g.nxgs + (tile % a.ntx) * TX;
+  const int j0 = g.nys + (tile / a.ntx) * TY;
+  const int tid = threadIdx.x;
+
+  for (int k = tid; k < 6 * SFW * SFH; k += NT) {
+    const int c = k / (SFW * SFH), rem = k - c * (SFW * SFH);
+    const int i = i0 - 1 + rem % SFW, j = j0 - 1 + rem / SFW;
+    if (i <= g.nxge + 1 && j <= g.nye + 1) sf[c][rem] = (float)cell_field_m(g, a.uf, c, i, j);
+  }
+  for (int k = tid; k < 3 * ACW * ACH; k += NT) (&acc[0][0])[k] = 0.0f;
+  __syncthreads();
+
+  const int grp = tid / G, lg = tid % G;
+  constexpr int NGRP = NT / G;
+  const float delt = (float)g.delt;
+  const float ic2 = (float)(1.0 / (g.c * g.c));
+  const float cc2 = (float)(g.c * g.c);
+
+  for (int cc = grp; cc < TX * TY; cc += NGRP) {
+    const int li = cc % TX, lj = cc / TX;
+    const int i = i0 + li, j = j0 + lj;
+    if (i > g.nxge || j > g.nye) continue;  // uniform within a group
+    float ax[20], ay[20], az[25];
+#pragma unroll
+    for (int k = 0; k < 20; k++) ax[k] = ay[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 25; k++) az[k] = 0.0f;
+
+    for (int s = 0; s < g.nsp; s++) {
+      const int rw = s * g.ny + (j - g.nys);
+      const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
+      const int base = a.rowbase[rw];
+      const int pbeg = base + cr[i - g.nxgs], pend = base + cr[i - g.nxgs + 1];
+      const float fac1 = (float)(g.q[s] / g.r[s] * 0.5 * g.delt);
+      const float txxx = fac1 * fac1;
+      const float fac2 = (float)(g.q[s] * g.delt / g.r[s]);
+      const float qk = (float)(g.q[s] * g.delx * g.d_delt);
+      const float qf = (float)g.q[s];
+      for (int pi = pbeg + lg; pi < pend; pi += G) {
+        const float ox = a.A.ox[pi], oy = a.A.oy[pi];
+        float sx0, sx1, sx2, sy0, sy1, sy2;
+        shape3f(ox - 0.5f, sx0, sx1, sx2);
+        shape3f(oy - 0.5f, sy0, sy1, sy2);
+        int so = (lj + 1) * SFW + (li + 1);
+        asm volatile("" : "+v"(so));
+        float fl[6];
+#pragma unroll
+        for (int cm = 0; cm < 6; cm++) {
+          const float* f = &sf[cm][so];
+          fl[cm] = (f[-SFW - 1] * sx0 + f[-SFW] * sx1 + f[-SFW + 1] * sx2) * sy0 +
+                   (f[-1] * sx0 + f[0] * sx1 + f[1] * sx2) * sy1 +
+                   (f[SFW - 1] * sx0 + f[SFW] * sx1 + f[SFW + 1] * sx2) * sy2;
+        }
+        const float bpx = fl[0], bpy = fl[1], bpz = fl[2], epx = fl[3], epy = fl[4], epz = fl[5];
+        float uvm1 = a.A.ux[pi] + fac1 * epx;
+        float uvm2 = a.A.uy[pi] + fac1 * epy;
+        float uvm3 = a.A.uz[pi] + fac1 * epz;
+        const float gam = sqrtf(cc2 + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+        const float igam = 1.0f / gam;
+        const float fac1r = fac1 * igam;
+        const float fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+        const float uvm4 = uvm1 + fac1r * (uvm2 * bpz - uvm3 * bpy);
+        const float uvm5 = uvm2 + fac1r * (uvm3 * bpx - uvm1 * bpz);
+        const float uvm6 = uvm3 + fac1r * (uvm1 * bpy - uvm2 * bpx);
+        uvm1 = uvm1 + fac2r * (uvm5 * bpz - uvm6 * bpy);
+        uvm2 = uvm2 + fac2r * (uvm6 * bpx - uvm4 * bpz);
+        uvm3 = uvm3 + fac2r * (uvm4 * bpy - uvm5 * bpx);
+        float g3 = uvm1 + fac1 * epx, g4 = uvm2 + fac1 * epy, g5 = uvm3 + fac1 * epz;
+        const float gmv = 1.0f / sqrtf(1.0f + (g3 * g3 + g4 * g4 + g5 * g5) * ic2);
+        // new position as cell + offset
+        const float nx_ = ox + g3 * delt * gmv, ny_ = oy + g4 * delt * gmv;
+        float fxi = floorf(nx_), fyi = floorf(ny_);
+        int incx = (int)fxi, incy = (int)fyi;
+        float fx = nx_ - fxi, fy = ny_ - fyi;
+        if (fx >= 1.0f) { fx -= 1.0f; incx += 1; }
+        if (fy >= 1.0f) { fy -= 1.0f; incy += 1; }
+        int cx = i + incx;
+        // reflecting walls act before the deposit (reconnection order,
+        // boundary_reconnection.f90:79-91): x' = 2(nxs+1) - x or 2(nxe-1) - x
+        if (BC == WPIC_BC_WALLX && (cx < g.nxgs + 1 || cx >= g.nxge - 1)) {
+          const int W = cx < g.nxgs + 1 ? 2 * (g.nxgs + 1) : 2 * (g.nxge - 1);
+          cx = W - cx - 1;
+          fx = 1.0f - fx;
+          if (fx >= 1.0f) { fx -= 1.0f; cx += 1; }
+          g3 = -g3; g4 = -g4; g5 = -g5;
+          incx = cx - i;
+        }
+        a.B.ux[pi] = g3;
+        a.B.uy[pi] = g4;
+        a.B.uz[pi] = g5;
+        if (incy < -1 || incy > 1) {
+          atomicOr(a.err, ERR_DOMAIN);
+          incy = incy < -1 ? -1 : 1;
+        }
+
+        // Esirkepov weights from offsets, factored as the FP64 fast kernel
+        float dsx[5], dsy[5];
+        ds_of(sx0, sx1, sx2, fx, incx, dsx);
+        ds_of(sy0, sy1, sy2, fy, incy, dsy);
+        const float s0x[5] = {0.0f, sx0, sx1, sx2, 0.0f}, s0y[5] = {0.0f, sy0, sy1, sy2, 0.0f};
+        const float qvz = qf * (g5 * gmv);
+        float wx[5], wy[5], A[5], B[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+          const float ty = s0y[k] + 0.5f * dsy[k];
+          wy[k] = qk * ty;
+          A[k] = qvz * ty;
+          B[k] = qvz * (0.5f * s0y[k] + (1.0f / 3.0f) * dsy[k]);
+          wx[k] = qk * (s0x[k] + 0.5f * dsx[k]);
+        }
+        float cxs = 0.0f, cys = 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          cxs -= dsx[m];
+          cys -= dsy[m];
+#pragma unroll
+          for (int jp = 0; jp < 5; jp++) ax[jp * 4 + m] += cxs * wy[jp];
+#pragma unroll
+          for (int ip = 0; ip < 5; ip++) ay[m * 5 + ip] += cys * wx[ip];
+        }
+#pragma unroll
+        for (int jp = 0; jp < 5; jp++)
+#pragma unroll
+          for (int ip = 0; ip < 5; ip++) az[jp * 5 + ip] += s0x[ip] * A[jp] + dsx[ip] * B[jp];
+
+        // periodic x wrap of the cell; the row step goes to the exchange
+        if (BC == WPIC_BC_PERIODIC) {
+          if (cx < g.nxgs) cx += g.nx;
+          else if (cx > g.nxge) cx -= g.nx;
+        }
+        if (cx < g.nxgs || cx > g.nxge) {
+          atomicOr(a.err, ERR_DOMAIN);
+          cx = cx < g.nxgs ? g.nxgs : g.nxge;
+        }
+        a.B.ix[pi] = cx;
+        a.B.ox[pi] = fx;
+        a.B.oy[pi] = fy;
+        a.code[pi] = ((cx - g.nxgs) << 2) | (incy + 1);
+      }
+    }
+    // reduce the cell's 65 weights over the group, spread the LDS atomics
+#pragma unroll
+    for (int k = 0; k < 65; k++) {
+      float v = k < 20 ? ax[k] : (k < 40 ? ay[k - 20] : az[k - 40]);
+#pragma unroll
+      for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, G);
+      if ((k % G) == lg) {
+        int comp, ip, jp;
+        if (k < 20) {
+          comp = 0; jp = k / 4 - 2; ip = k % 4 - 1;
+        } else if (k < 40) {
+          comp = 1; jp = (k - 20) / 5 - 1; ip = (k - 20) % 5 - 2;
+        } else {
+          comp = 2; jp = (k - 40) / 5 - 2; ip = (k - 40) % 5 - 2;
+        }
+        atomicAdd(&acc[comp][(lj + 2 + jp) * ACW + (li + 2 + ip)], v);
+      }
+    }
+  }
+
+  __syncthreads();
+  const size_t pl = (size_t)g.fw * g.fh;
+  for (int k = tid; k < 3 * ACW * ACH; k += NT) {
+    const int comp = k / (ACW * ACH), rem = k - comp * (ACW * ACH);
+    const float v = acc[comp][rem];
+    if (v == 0.0f) continue;
+    const int I = i0 - 2 + rem % ACW, J = j0 - 2 + rem / ACW;
+    if (I > g.nxge + 2 || J > g.nye + 2) continue;
+    unsafeAtomicAdd(a.uj + pl * comp + fidx(g, I, J), (double)v);
+  }
+}
+
+hipError_t launch_push_deposit_mixed(wpic_ctx* c) {
+  MixArgs a;
+  a.g = c->g;
+  a.A = c->MA;
+  a.B = c->MB;
+  a.rowbase = c->rowbaseA;
+  a.cumcnt = c->cumcnt;
+  a.uf = c->uf;
+  a.uj = c->uj;
+  a.code = c->code;
+  a.err = c->err;
+  a.ntx = (c->g.nx + mx::TX - 1) / mx::TX;
+  a.nty = (c->g.ny + mx::TY - 1) / mx::TY;
+  const int nblk = xcd_grid(a.ntx * a.nty);
+  if (c->g.bc == WPIC_BC_WALLX)
+    hipLaunchKernelGGL(k_push_deposit_mixed<WPIC_BC_WALLX>, dim3(nblk), dim3(mx::NT), 0, c->st, a);
+  else
+    hipLaunchKernelGGL(k_push_deposit_mixed<WPIC_BC_PERIODIC>, dim3(nblk), dim3(mx::NT), 0, c->st, a);
+  return hipGetLastError();
+}
+
+}  // namespace wpic

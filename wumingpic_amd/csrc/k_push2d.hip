@@ -127,10 +127,12 @@ __device__ inline void esirkepov_shapes(const Grid2D& g, double x, double y, dou
     const double s1_1 = 0.5 * (0.5 - dh) * (0.5 - dh);
     const double s1_2 = 0.75 - dh * dh;
     const double s1_3 = 0.5 * (0.5 + dh) * (0.5 + dh);
-    // smo_1 = -(inc-abs(inc))*0.5+0 etc. (field.f90:244-246): exactly 0 or 1
-    const double smo_1 = inc < 0 ? 1.0 : 0.0;
+    // smo_1 = -(inc-abs(inc))*0.5+0 etc. (field.f90:244-246): whole numbers,
+    // never -0; |inc| = 1 at most except where a reflecting wall moves a
+    // particle by more, and then the same formula applies
+    const double smo_1 = inc < 0 ? (double)(-inc) : 0.0;
     const double smo_2 = (double)(-abs(inc) + 1);
-    const double smo_3 = inc > 0 ? 1.0 : 0.0;
+    const double smo_3 = inc > 0 ? (double)inc : 0.0;
     double* ds = d == 0 ? dsx : dsy;
     const double* s0 = d == 0 ? s0x : s0y;
     ds[0] = s1_1 * smo_1;
@@ -594,7 +596,7 @@ hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused)
                                     : launch_t<1, true, WPIC_BC_PERIODIC>(c, push, deposit, fused, a, nblk);
   }
   if (push && deposit && fused) {
-    constexpr int GF = 4;
+    constexpr int GF = 16;  // a group's 16 lanes read and write whole 128-byte lines
     if (c->g.bc == WPIC_BC_WALLX)
       hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_WALLX>), dim3(nblk), dim3(NT), 0, c->st, a);
     else

@@ -310,13 +310,60 @@ __device__ inline uint64_t key_peers(int key, bool act, int nbits) {
   return peers;
 }
 
-// stable counting sort of one row by cell (sort.f90:52-78), gathering B -> A.
+// stable counting sort of one row by cell (sort.f90:52-78), gathering B -> A
+// (the FP64 or the mixed record, by the mover M).
 // Each wave owns a contiguous quarter of the row: its per-key histogram, then
 // (after a scan over keys and waves) its per-key write cursors.  Within a
 // wave, lanes with equal keys are ranked by one ballot per key bit.  Chunks
 // of 64 are processed U at a time so their loads overlap.
 constexpr int SU = 4;
-__global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, int nbits) {
+
+// particle record movers of the sort: the FP64 record (x, y, ux, uy, uz, id)
+// and the mixed record (x cell, FP32 offsets and momenta, id)
+struct MoveF64 {
+  Parts src, dst;
+  const uint64_t* idsrc;
+  uint64_t* iddst;
+  struct Rec {
+    double x, y, ux, uy, uz;
+    uint64_t id;
+  };
+  __device__ Rec load(int s) const { return {src.x[s], src.y[s], src.ux[s], src.uy[s], src.uz[s], idsrc[s]}; }
+  __device__ void store(int d, const Rec& r) const {
+    dst.x[d] = r.x;
+    dst.y[d] = r.y;
+    dst.ux[d] = r.ux;
+    dst.uy[d] = r.uy;
+    dst.uz[d] = r.uz;
+    iddst[d] = r.id;
+  }
+};
+
+struct MoveMixed {
+  PartsM src, dst;
+  const uint64_t* idsrc;
+  uint64_t* iddst;
+  struct Rec {
+    int ix;
+    float ox, oy, ux, uy, uz;
+    uint64_t id;
+  };
+  __device__ Rec load(int s) const {
+    return {src.ix[s], src.ox[s], src.oy[s], src.ux[s], src.uy[s], src.uz[s], idsrc[s]};
+  }
+  __device__ void store(int d, const Rec& r) const {
+    dst.ix[d] = r.ix;
+    dst.ox[d] = r.ox;
+    dst.oy[d] = r.oy;
+    dst.ux[d] = r.ux;
+    dst.uy[d] = r.uy;
+    dst.uz[d] = r.uz;
+    iddst[d] = r.id;
+  }
+};
+
+template <class M>
+__global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, M mv, int nbits) {
   extern __shared__ int hist[];  // [SB/64][nx]
   __shared__ int wsum[SB / 64];
   __shared__ int carry;
@@ -402,19 +449,10 @@ __global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, int nbits) {
     }
 #pragma unroll
     for (int u = 0; u < SU; u++) key[u] = act[u] ? (a.code[src[u]] >> 2) : 0;
-    double vx[SU], vy[SU], vux[SU], vuy[SU], vuz[SU];
-    uint64_t vid[SU];
+    typename M::Rec v[SU];
 #pragma unroll
-    for (int u = 0; u < SU; u++) {
-      if (act[u]) {
-        vx[u] = a.B.x[src[u]];
-        vy[u] = a.B.y[src[u]];
-        vux[u] = a.B.ux[src[u]];
-        vuy[u] = a.B.uy[src[u]];
-        vuz[u] = a.B.uz[src[u]];
-        vid[u] = a.idsrc[src[u]];
-      }
-    }
+    for (int u = 0; u < SU; u++)
+      if (act[u]) v[u] = mv.load(src[u]);
 #pragma unroll
     for (int u = 0; u < SU; u++) {
       const uint64_t peers = key_peers(key[u], act[u], nbits);
@@ -423,15 +461,7 @@ __global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, int nbits) {
       int pos = 0;
       if (act[u]) pos = hw[key[u]] + rank;
       if (act[u] && rank == 0) hw[key[u]] = pos + cnt;
-      if (act[u]) {
-        const int d = nb + pos;
-        a.A.x[d] = vx[u];
-        a.A.y[d] = vy[u];
-        a.A.ux[d] = vux[u];
-        a.A.uy[d] = vuy[u];
-        a.A.uz[d] = vuz[u];
-        a.iddst[d] = vid[u];
-      }
+      if (act[u]) mv.store(nb + pos, v[u]);
     }
   }
 }
@@ -515,7 +545,13 @@ hipError_t launch_rowsort(wpic_ctx* c) {
   int nbits = 0;
   while ((1 << nbits) < c->g.nx) nbits++;
   const size_t shm = sizeof(int) * (SB / 64) * (size_t)c->g.nx;
-  hipLaunchKernelGGL(k_rowsort, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, nbits);
+  if (c->mixed) {
+    MoveMixed mv{c->MB, c->MA, a.idsrc, a.iddst};
+    hipLaunchKernelGGL(k_rowsort<MoveMixed>, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, mv, nbits);
+  } else {
+    MoveF64 mv{c->B, c->A, a.idsrc, a.iddst};
+    hipLaunchKernelGGL(k_rowsort<MoveF64>, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, mv, nbits);
+  }
   return hipGetLastError();
 }
 

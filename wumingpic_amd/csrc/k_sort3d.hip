@@ -1,0 +1,534 @@
+// k_sort3d.hip -- 3D particle boundaries, the y-z column exchange and the
+// per-column counting sort at one rank (3d/common/boundary_periodic.f90:68-455,
+// 3d/common/sort.f90:40-88).
+//
+// Particles of species s live in (j, k) columns, each a contiguous segment of
+// the species' block.  boundary_periodic__particle_yz appends every leaver to
+// the bucket of its destination column in the scan order (k, j, slot), then
+// moves the ghost buckets across the periodic faces (y first, then z) by
+// appending whole buckets.  The arrivals of a column are therefore a fixed
+// sequence of "segments" -- all movers of one source column in one direction,
+// in slot order -- and that sequence depends only on the grid, so the host
+// computes it once per context (seg / nseg).  Then, in parallel:
+//   k_colscan3d   per column: y/z wraps, direction, exchange code; leaver slots
+//                 (holes) and the movers of each direction in slot order;
+//   k_colcount3d  arrivals, new column lengths and bases;
+//   k_colbuild3d  R = for each slot of the post-exchange column the particle
+//                 that lands there (hole filling as :418-455 does it);
+//   k_colsort3d   stable counting sort of R by x cell, gathering B into A.
+#include "wpic_internal3d.h"
+
+namespace wpic {
+
+namespace {
+constexpr int SB = 256;
+__device__ inline uint64_t lt_mask(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+}  // namespace
+
+struct Sort3Args {
+  Grid3D g;
+  Parts3 A, B;
+  const uint64_t* idsrc;
+  uint64_t* iddst;
+  int *code, *R, *holes, *mlist;
+  int *colbaseB, *np2B, *colbaseN, *np2N, *cumcnt;
+  int *dcnt, *dstart, *colkm;
+  const int *seg, *nseg;
+  int* err;
+};
+
+// boundary_periodic__particle_x (:80-95): ipos = int(x*d_delx), round to nearest
+__global__ __launch_bounds__(SB) void k_wrapx3d(Sort3Args a) {
+  const Grid3D& g = a.g;
+  const int rw = blockIdx.x;
+  const int base = a.colbaseB[rw], n = a.np2B[rw];
+  for (int q = threadIdx.x; q < n; q += SB) {
+    const int p = base + q;
+    const double x = a.B.x[p];
+    const int ipos = (int)(x * g.d_delx);
+    if (ipos < g.nxgs)
+      a.B.x[p] = x + g.spanx;
+    else if (ipos >= g.nxge + 1)
+      a.B.x[p] = x - g.spanx;
+  }
+}
+
+// per column: the y/z wraps of particle_yz (:147-165) and the exchange code;
+// pass 2 lists the holes and the movers of each direction in slot order
+__global__ __launch_bounds__(SB) void k_colscan3d(Sort3Args a, int wrap) {
+  __shared__ int wt[SB / 64][10];
+  __shared__ int run[10];
+  __shared__ int dst[9];
+  const Grid3D& g = a.g;
+  const int rw = blockIdx.x;
+  const int col = rw % g.ncol;
+  const int j = g.nys + col % g.ny, k = g.nzs + col / g.ny;
+  const int base = a.colbaseB[rw], n = a.np2B[rw];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint64_t lt = lt_mask(lane);
+  if (tid < 10) run[tid] = 0;
+  __syncthreads();
+  // pass 1: codes and per-direction counts
+  for (int c0 = 0; c0 < n; c0 += SB) {
+    const int q = c0 + tid;
+    int dir = -1;
+    if (q < n) {
+      const int p = base + q;
+      if (wrap) {
+        const double y = a.B.y[p], z = a.B.z[p];
+        const int jpos = (int)(y * g.d_delx), kpos = (int)(z * g.d_delx);
+        int dj = jpos - j, dk = kpos - k;
+        if (!(jpos == j && kpos == k)) {
+          if (jpos <= g.nygs - 1)
+            a.B.y[p] = y + g.spany;
+          else if (jpos >= g.nyge + 1)
+            a.B.y[p] = y - g.spany;
+          if (kpos <= g.nzgs - 1)
+            a.B.z[p] = z + g.spanz;
+          else if (kpos >= g.nzge + 1)
+            a.B.z[p] = z - g.spanz;
+        }
+        int ni = (int)a.B.x[p];
+        if (ni < g.nxgs || ni > g.nxge || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
+          atomicOr(a.err, ERR_DOMAIN);
+          ni = ni < g.nxgs ? g.nxgs : (ni > g.nxge ? g.nxge : ni);
+          dj = dj < -1 ? -1 : (dj > 1 ? 1 : dj);
+          dk = dk < -1 ? -1 : (dk > 1 ? 1 : dk);
+        }
+        dir = (dk + 1) * 3 + (dj + 1);
+        a.code[p] = ((ni - g.nxgs) << CODE_SHIFT3) | dir;
+      } else {
+        dir = a.code[p] & 15;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 9; d++) {
+      const uint64_t b = __ballot(dir == d);
+      if (lane == 0) wt[w][d] = __popcll(b);
+    }
+    __syncthreads();
+    if (tid < 9) {
+      int t = 0;
+      for (int v = 0; v < SB / 64; v++) t += wt[v][tid];
+      run[tid] += t;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    int acc = 0;
+    for (int d = 0; d < 9; d++) {
+      dst[d] = acc;
+      a.dcnt[rw * 9 + d] = run[d];
+      a.dstart[rw * 9 + d] = acc;
+      if (d != DIR_STAY) acc += run[d];
+    }
+    a.colkm[rw * 2] = acc;  // leavers
+  }
+  __syncthreads();
+  if (tid < 10) run[tid] = 0;
+  __syncthreads();
+  // pass 2: holes (1-based slots of the leavers) and per-direction mover lists
+  for (int c0 = 0; c0 < n; c0 += SB) {
+    const int q = c0 + tid;
+    const int dir = q < n ? (a.code[base + q] & 15) : -1;
+    const bool lv = q < n && dir != DIR_STAY;
+    const uint64_t bl = __ballot(lv);
+    if (lane == 0) wt[w][9] = __popcll(bl);
+    uint64_t bd = 0;
+#pragma unroll
+    for (int d = 0; d < 9; d++) {
+      const uint64_t b = __ballot(dir == d);
+      if (lane == 0) wt[w][d] = __popcll(b);
+      if (dir == d) bd = b;
+    }
+    __syncthreads();
+    if (lv) {
+      int oh = run[9], od = run[dir];
+      for (int v = 0; v < w; v++) {
+        oh += wt[v][9];
+        od += wt[v][dir];
+      }
+      a.holes[base + oh + __popcll(bl & lt)] = q + 1;
+      a.mlist[base + dst[dir] + od + __popcll(bd & lt)] = base + q;
+    }
+    __syncthreads();
+    if (tid < 10) {
+      int t = 0;
+      for (int v = 0; v < SB / 64; v++) t += wt[v][tid];
+      run[tid] += t;
+    }
+    __syncthreads();
+  }
+}
+
+// arrivals m of every column, new length L = np2 - leavers + m, bases
+__global__ __launch_bounds__(1024) void k_colcount3d(Sort3Args a) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const Grid3D& g = a.g;
+  const int ncol = g.ncol, tid = threadIdx.x;
+  for (int s = 0; s < g.nsp; s++) {
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int r0 = 0; r0 < ncol; r0 += 1024) {
+      const int r = r0 + tid;
+      int L = 0;
+      if (r < ncol) {
+        const int rw = s * ncol + r;
+        int m = 0;
+        for (int e = 0; e < a.nseg[r]; e++) {
+          const int sg = a.seg[r * 8 + e];
+          m += a.dcnt[(s * ncol + (sg >> 4)) * 9 + (sg & 15)];
+        }
+        a.colkm[rw * 2 + 1] = m;
+        L = a.np2B[rw] - a.colkm[rw * 2] + m;
+        a.np2N[rw] = L;
+      }
+      const int lane = tid & 63, w = tid >> 6;
+      int v = L;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+      }
+      if (lane == 63) wsum[w] = v;
+      __syncthreads();
+      if (tid == 0) {
+        int acc = 0;
+        for (int q = 0; q < 16; q++) {
+          const int t = wsum[q];
+          wsum[q] = acc;
+          acc += t;
+        }
+      }
+      __syncthreads();
+      const int excl = carry + wsum[w] + v - L;
+      if (r < ncol) a.colbaseN[s * ncol + r] = (int)(s * g.ncap) + excl;
+      __syncthreads();
+      if (tid == 1023) carry = excl + L;
+      __syncthreads();
+    }
+  }
+}
+
+// R of one column: slot -> source particle (boundary_periodic.f90:418-455)
+__global__ __launch_bounds__(SB) void k_colbuild3d(Sort3Args a) {
+  __shared__ int wt[SB / 64][2];
+  __shared__ int run[2];
+  __shared__ int sh_a;
+  __shared__ int scnt[8], sptr[8];
+  __shared__ int nsg;
+  const Grid3D& g = a.g;
+  const int rw = blockIdx.x;
+  const int s = rw / g.ncol, col = rw % g.ncol;
+  const int base = a.colbaseB[rw], n = a.np2B[rw];
+  const int kk = a.colkm[rw * 2], m = a.colkm[rw * 2 + 1];
+  const int L = a.np2N[rw], nb = a.colbaseN[rw];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid == 0) {
+    nsg = a.nseg[col];
+    for (int e = 0; e < nsg; e++) {
+      const int sg = a.seg[col * 8 + e];
+      const int src = s * g.ncol + (sg >> 4), d = sg & 15;
+      scnt[e] = a.dcnt[src * 9 + d];
+      sptr[e] = a.colbaseB[src] + a.dstart[src * 9 + d];
+    }
+  }
+  __syncthreads();
+  // the t-th arrival (0-based) of this column
+  auto in = [&](int t) -> int {
+    for (int e = 0; e < nsg; e++) {
+      if (t < scnt[e]) return a.mlist[sptr[e] + t];
+      t -= scnt[e];
+    }
+    return -1;
+  };
+  const int* holes = a.holes + base;
+
+  const int keep = n < L ? n : L;
+  for (int q = tid; q < keep; q += SB)
+    if ((a.code[base + q] & 15) == DIR_STAY) a.R[nb + q] = base + q;
+  const int kf = kk < m ? kk : m;
+  for (int t = tid; t < kf; t += SB) {
+    const int h = holes[t];
+    if (h >= 1 && h <= L) a.R[nb + h - 1] = in(t);
+  }
+  for (int t = tid; t < m - kk; t += SB) a.R[nb + n + t] = in(kk + t);
+  if (m >= kk) return;
+
+  // fewer arrivals than holes: tail particles fill the remaining holes, the
+  // last one into the first free hole (loop1).  Tail = slots (L, n].
+  if (tid == 0) {
+    int lo = 0, hi = kk;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (holes[mid] <= L)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    sh_a = lo;
+    run[0] = 0;
+    run[1] = 0;
+  }
+  __syncthreads();
+  const int aa = sh_a;
+  const int T = n - L;
+  int totalq = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int c0 = 0; c0 < T; c0 += SB) {
+      const int q = c0 + tid;
+      bool lv = false;
+      if (q < T) lv = (a.code[base + L + q] & 15) != DIR_STAY;
+      const uint64_t bl = __ballot(lv);
+      if (lane == 0) wt[w][0] = __popcll(bl);
+      __syncthreads();
+      int ol = run[0];
+      for (int v = 0; v < w; v++) ol += wt[v][0];
+      const int rank = aa + ol + __popcll(bl & lt_mask(lane)) + 1;
+      const bool nonh = q < T && (!lv || rank <= m);
+      const uint64_t bn = __ballot(nonh);
+      if (lane == 0) wt[w][1] = __popcll(bn);
+      __syncthreads();
+      int on = run[1];
+      for (int v = 0; v < w; v++) on += wt[v][1];
+      if (pass == 1 && nonh) {
+        const int incl = on + __popcll(bn & lt_mask(lane)) + 1;
+        const int rr = totalq - incl + 1;
+        const int tgt = m + rr;
+        const int src = lv ? in(rank - 1) : base + L + q;
+        const int h = (tgt >= 1 && tgt <= kk) ? holes[tgt - 1] : 0;
+        if (h >= 1 && h <= L && src >= 0)
+          a.R[nb + h - 1] = src;
+        else
+          atomicOr(a.err, ERR_ROW);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int t0 = 0, t1 = 0;
+        for (int v = 0; v < SB / 64; v++) {
+          t0 += wt[v][0];
+          t1 += wt[v][1];
+        }
+        run[0] += t0;
+        run[1] += t1;
+      }
+      __syncthreads();
+    }
+    totalq = run[1];
+    __syncthreads();
+    if (tid == 0) {
+      run[0] = 0;
+      run[1] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ inline uint64_t peers_of(int key, bool act, int nbits) {
+  uint64_t peers = __ballot(act);
+  for (int b = 0; b < nbits; b++) {
+    const bool bit = (key >> b) & 1;
+    const uint64_t msk = __ballot(act && bit);
+    peers &= bit ? msk : ~msk;
+  }
+  return peers;
+}
+
+// stable counting sort of one column by x cell (sort.f90:56-81), B -> A
+__global__ __launch_bounds__(SB) void k_colsort3d(Sort3Args a, int nbits) {
+  extern __shared__ int hist[];  // [SB/64][nx]
+  __shared__ int wsum[SB / 64];
+  __shared__ int carry;
+  constexpr int SU = 4, NW = SB / 64;
+  const Grid3D& g = a.g;
+  const int nx = g.nx;
+  const int rw = blockIdx.x;
+  const int n = a.np2N[rw], nb = a.colbaseN[rw];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const unsigned ntot = (unsigned)(g.ncap * g.nsp);
+  const uint64_t lt = lt_mask(lane);
+  for (int q = tid; q < NW * nx; q += SB) hist[q] = 0;
+  __syncthreads();
+  const int lo = (int)(((long long)n * w) / NW), hi = (int)(((long long)n * (w + 1)) / NW);
+  int* hw = hist + w * nx;
+  for (int k0 = lo; k0 < hi; k0 += 64 * SU) {
+    int key[SU];
+    bool act[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int q = k0 + u * 64 + lane;
+      act[u] = q < hi;
+      int src = act[u] ? a.R[nb + q] : 0;
+      if ((unsigned)src >= ntot) src = 0;
+      key[u] = act[u] ? (a.code[src] >> CODE_SHIFT3) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const uint64_t peers = peers_of(key[u], act[u], nbits);
+      if (act[u] && __popcll(peers & lt) == 0) hw[key[u]] += __popcll(peers);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  int* cc = a.cumcnt + (size_t)rw * (nx + 1);
+  for (int k0 = 0; k0 < nx; k0 += SB) {
+    const int key = k0 + tid;
+    int tot = 0;
+    if (key < nx)
+      for (int v = 0; v < NW; v++) tot += hist[v * nx + key];
+    int v = tot;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(v, off, 64);
+      if (lane >= off) v += t;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    int pre = carry;
+    for (int q = 0; q < w; q++) pre += wsum[q];
+    const int excl = pre + v - tot;
+    if (key < nx) {
+      cc[key] = excl;
+      int st = excl;
+      for (int q = 0; q < NW; q++) {
+        const int h = hist[q * nx + key];
+        hist[q * nx + key] = st;
+        st += h;
+      }
+    }
+    __syncthreads();
+    if (tid == SB - 1) carry = excl + tot;
+    __syncthreads();
+  }
+  if (tid == 0) cc[nx] = n;
+  for (int k0 = lo; k0 < hi; k0 += 64 * SU) {
+    int src[SU], key[SU];
+    bool act[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int q = k0 + u * 64 + lane;
+      act[u] = q < hi;
+      src[u] = act[u] ? a.R[nb + q] : 0;
+      if ((unsigned)src[u] >= ntot) {
+        atomicOr(a.err, ERR_ROW);
+        act[u] = false;
+        src[u] = 0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) key[u] = act[u] ? (a.code[src[u]] >> CODE_SHIFT3) : 0;
+    double vx[SU], vy[SU], vz[SU], vux[SU], vuy[SU], vuz[SU];
+    uint64_t vid[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      if (act[u]) {
+        vx[u] = a.B.x[src[u]];
+        vy[u] = a.B.y[src[u]];
+        vz[u] = a.B.z[src[u]];
+        vux[u] = a.B.ux[src[u]];
+        vuy[u] = a.B.uy[src[u]];
+        vuz[u] = a.B.uz[src[u]];
+        vid[u] = a.idsrc[src[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const uint64_t peers = peers_of(key[u], act[u], nbits);
+      const int rank = __popcll(peers & lt);
+      const int cnt = __popcll(peers);
+      int pos = 0;
+      if (act[u]) pos = hw[key[u]] + rank;
+      if (act[u] && rank == 0) hw[key[u]] = pos + cnt;
+      if (act[u]) {
+        const int d = nb + pos;
+        a.A.x[d] = vx[u];
+        a.A.y[d] = vy[u];
+        a.A.z[d] = vz[u];
+        a.A.ux[d] = vux[u];
+        a.A.uy[d] = vuy[u];
+        a.A.uz[d] = vuz[u];
+        a.iddst[d] = vid[u];
+      }
+    }
+  }
+}
+
+// sort__bucket on a gp that did not go through particle_yz: identity R
+__global__ __launch_bounds__(SB) void k_codes_identity3d(Sort3Args a) {
+  const Grid3D& g = a.g;
+  const int rw = blockIdx.x;
+  const int base = a.colbaseB[rw], n = a.np2B[rw], nb = a.colbaseN[rw];
+  for (int q = threadIdx.x; q < n; q += SB) {
+    int ni = (int)a.B.x[base + q];
+    if (ni < g.nxgs || ni > g.nxge) {
+      atomicOr(a.err, ERR_DOMAIN);
+      ni = ni < g.nxgs ? g.nxgs : g.nxge;
+    }
+    a.code[base + q] = ((ni - g.nxgs) << CODE_SHIFT3) | DIR_STAY;
+    a.R[nb + q] = base + q;
+  }
+}
+
+static Sort3Args sort3_args(wpic_ctx3d* c) {
+  Sort3Args a;
+  a.g = c->g;
+  a.A = c->A;
+  a.B = c->B;
+  a.idsrc = c->id[c->idb];
+  a.iddst = c->idb == c->ida ? c->id[1 - c->ida] : c->id[c->ida];
+  a.code = c->code;
+  a.R = c->R;
+  a.holes = c->holes;
+  a.mlist = c->mlist;
+  a.colbaseB = c->colbaseB;
+  a.np2B = c->np2B;
+  a.colbaseN = c->colbaseN;
+  a.np2N = c->np2N;
+  a.cumcnt = c->cumcnt;
+  a.dcnt = c->dcnt;
+  a.dstart = c->dstart;
+  a.colkm = c->colkm;
+  a.seg = c->seg;
+  a.nseg = c->nseg;
+  a.err = c->err;
+  return a;
+}
+
+static dim3 ncols(wpic_ctx3d* c) { return dim3((unsigned)(c->g.nsp * c->g.ncol)); }
+
+hipError_t launch_wrapx3d(wpic_ctx3d* c) {
+  hipLaunchKernelGGL(k_wrapx3d, ncols(c), dim3(SB), 0, c->st, sort3_args(c));
+  return hipGetLastError();
+}
+
+hipError_t launch_colscan3d(wpic_ctx3d* c) {
+  hipLaunchKernelGGL(k_colscan3d, ncols(c), dim3(SB), 0, c->st, sort3_args(c), 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_colcount3d(wpic_ctx3d* c) {
+  hipLaunchKernelGGL(k_colcount3d, dim3(1), dim3(1024), 0, c->st, sort3_args(c));
+  return hipGetLastError();
+}
+
+hipError_t launch_colbuild3d(wpic_ctx3d* c) {
+  hipLaunchKernelGGL(k_colbuild3d, ncols(c), dim3(SB), 0, c->st, sort3_args(c));
+  return hipGetLastError();
+}
+
+hipError_t launch_colsort3d(wpic_ctx3d* c) {
+  int nbits = 0;
+  while ((1 << nbits) < c->g.nx) nbits++;
+  const size_t shm = sizeof(int) * (SB / 64) * (size_t)c->g.nx;
+  hipLaunchKernelGGL(k_colsort3d, ncols(c), dim3(SB), shm, c->st, sort3_args(c), nbits);
+  return hipGetLastError();
+}
+
+hipError_t launch_codes_identity3d(wpic_ctx3d* c) {
+  hipLaunchKernelGGL(k_codes_identity3d, ncols(c), dim3(SB), 0, c->st, sort3_args(c));
+  return hipGetLastError();
+}
+
+}  // namespace wpic

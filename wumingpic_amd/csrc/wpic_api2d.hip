@@ -186,6 +186,12 @@ int field_after_deposit(wpic_ctx* c) {
 }
 
 int deposit(wpic_ctx* c, bool push, bool fused) {
+  if (c->mixed) {
+    HIPCHK(launch_zero(c, c->uj, (size_t)3 * c->g.fw * c->g.fh));
+    Timed t(c, "push_deposit");
+    HIPCHK(launch_push_deposit_mixed(c));
+    return 0;
+  }
   const bool parity = c->mode == WPIC_MODE_PARITY;
   if (parity) {
     int rc = ensure_pjbuf(c);
@@ -210,6 +216,11 @@ int exchange(wpic_ctx* c, bool ywrap) {
   c->h_rows_ok[c->rn] = false;
   c->exch_pending = true;
   return 0;
+}
+
+int mixed_only_step() {
+  return set_error(WPIC_EINVAL,
+                   "the mixed-precision form runs the fused loop (wpic_step2d); single routines need an FP64 mode");
 }
 
 // sort__bucket: gather B through R into A, by cell within rows
@@ -252,6 +263,8 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
   if (!(p->c * p->delt <= p->delx))
     return set_error(WPIC_EINVAL, "c*delt must not exceed delx (particles move less than one cell per step)");
   if (p->bc != WPIC_BC_PERIODIC && p->bc != WPIC_BC_WALLX) return set_error(WPIC_EINVAL, "unknown boundary kind");
+  if (p->mode != WPIC_MODE_FAST && p->mode != WPIC_MODE_PARITY && p->mode != WPIC_MODE_MIXED)
+    return set_error(WPIC_EINVAL, "bad mode");
   if (p->capacity <= 0 || (long long)p->capacity * p->nsp >= (1ll << 31))
     return set_error(WPIC_EINVAL, "capacity must be positive and nsp*capacity below 2^31");
 
@@ -259,6 +272,7 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
   std::memset((void*)&c->g, 0, sizeof(c->g));
   c->P = *p;
   c->mode = p->mode;
+  c->mixed = p->mode == WPIC_MODE_MIXED;
   Grid2D& g = c->g;
   g.nx = nx;
   g.ny = ny;
@@ -302,9 +316,17 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
   const size_t N = (size_t)g.ncap * g.nsp;
   const int nr = g.nsp * ny;
   const size_t pl = (size_t)g.fw * g.fh, nc = (size_t)nx * ny;
-  double** dp[] = {&c->A.x, &c->A.y, &c->A.ux, &c->A.uy, &c->A.uz, &c->B.x, &c->B.y, &c->B.ux, &c->B.uy, &c->B.uz};
-  for (auto q : dp)
-    if (e == hipSuccess) e = dalloc(q, N);
+  if (c->mixed) {
+    for (PartsM* m : {&c->MA, &c->MB}) {
+      if (e == hipSuccess) e = dalloc(&m->ix, N);
+      for (float** q : {&m->ox, &m->oy, &m->ux, &m->uy, &m->uz})
+        if (e == hipSuccess) e = dalloc(q, N);
+    }
+  } else {
+    double** dp[] = {&c->A.x, &c->A.y, &c->A.ux, &c->A.uy, &c->A.uz, &c->B.x, &c->B.y, &c->B.ux, &c->B.uy, &c->B.uz};
+    for (auto q : dp)
+      if (e == hipSuccess) e = dalloc(q, N);
+  }
   for (int k = 0; k < 2 && e == hipSuccess; k++) e = dalloc(&c->id[k], N);
   int** ip[] = {&c->code, &c->R, &c->holes, &c->ulist, &c->dlist};
   for (auto q : ip)
@@ -358,7 +380,9 @@ int wpic_destroy(wpic_ctx* c) {
   void* ps[] = {c->A.x, c->A.y, c->A.ux, c->A.uy, c->A.uz, c->B.x, c->B.y, c->B.ux, c->B.uy, c->B.uz,
                 c->id[0], c->id[1], c->code, c->R, c->holes, c->ulist, c->dlist, c->rows[0], c->rows[1],
                 c->rows[2], c->cumcnt, c->rowcnt, c->aosbase, c->uf, c->df, c->uj, c->pjbuf, c->phi, c->pa,
-                c->pb, c->rr, c->bb, c->ap, c->part, c->scal, c->err, c->staging, c->gbuf, c->red};
+                c->pb, c->rr, c->bb, c->ap, c->part, c->scal, c->err, c->staging, c->gbuf, c->red,
+                c->MA.ix, c->MA.ox, c->MA.oy, c->MA.ux, c->MA.uy, c->MA.uz,
+                c->MB.ix, c->MB.ox, c->MB.oy, c->MB.ux, c->MB.uy, c->MB.uz};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -367,7 +391,10 @@ int wpic_destroy(wpic_ctx* c) {
 }
 
 int wpic_set_mode(wpic_ctx* c, int mode) {
-  if (!c || (mode != WPIC_MODE_FAST && mode != WPIC_MODE_PARITY)) return set_error(WPIC_EINVAL, "bad mode");
+  if (!c || (mode != WPIC_MODE_FAST && mode != WPIC_MODE_PARITY && mode != WPIC_MODE_MIXED))
+    return set_error(WPIC_EINVAL, "bad mode");
+  if (c->mixed != (mode == WPIC_MODE_MIXED))
+    return set_error(WPIC_EINVAL, "the mixed-precision form is fixed when the context is created");
   c->mode = mode;
   return 0;
 }
@@ -420,7 +447,10 @@ static int upload_rows(wpic_ctx* c, int which, const double* up, const int* np2,
   HIPCHK(hipMemcpyAsync(c->staging, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->rows[which], hrows.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->aosbase, aosb.data(), sizeof(long long) * nr, hipMemcpyHostToDevice, c->st));
-  HIPCHK(launch_up_to_soa(c, c->staging, total, kind));
+  if (c->mixed)
+    HIPCHK(launch_up_to_mixed(c, c->staging));
+  else
+    HIPCHK(launch_up_to_soa(c, c->staging, total, kind));
   HIPCHK(hipStreamSynchronize(c->st));
   c->h_rows[which] = hrows;
   c->h_rows_ok[which] = true;
@@ -442,7 +472,10 @@ static int download_rows(wpic_ctx* c, int which, int kind, double* up, int* np2,
   rc = ensure_staging(c, (size_t)total * 6);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(c->aosbase, aosb.data(), sizeof(long long) * nr, hipMemcpyHostToDevice, c->st));
-  HIPCHK(launch_soa_to_aos(c, c->staging, kind));
+  if (c->mixed)
+    HIPCHK(launch_mixed_to_aos(c, c->staging));
+  else
+    HIPCHK(launch_soa_to_aos(c, c->staging, kind));
   std::vector<double> h((size_t)total * 6);
   HIPCHK(hipMemcpyAsync(h.data(), c->staging, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -489,6 +522,7 @@ int wpic_download_up2d(wpic_ctx* c, double* up, int* np2, int* cumcnt, int np) {
 }
 
 int wpic_upload_gp2d(wpic_ctx* c, const double* gp, const int* np2, int np) {
+  if (c && c->mixed) return mixed_only_step();
   if (!c || !gp || !np2) return set_error(WPIC_EINVAL, "null argument");
   if (c->rb == c->ra) c->rb = pick_free(c->ra, c->rn);
   set_rows(c);
@@ -506,6 +540,7 @@ int wpic_upload_gp2d(wpic_ctx* c, const double* gp, const int* np2, int np) {
 }
 
 int wpic_download_gp2d(wpic_ctx* c, double* gp, int* np2, int np) {
+  if (c && c->mixed) return mixed_only_step();
   if (!c || !gp) return set_error(WPIC_EINVAL, "null argument");
   if (!c->b_valid) return set_error(WPIC_EINVAL, "no gp on the device (call particle__solv first)");
   if (c->exch_pending) return download_rows(c, c->rn, 1, gp, np2, np);
@@ -564,6 +599,7 @@ int wpic_upload_dfield2d(wpic_ctx* c, const double* df) {
 // ---------------------------------------------------------------------------
 int wpic_particle_solv2d(wpic_ctx* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
   c->rb = c->ra;
   if (c->rn == c->ra) c->rn = pick_free(c->ra, c->rb);
   set_rows(c);
@@ -580,6 +616,7 @@ int wpic_particle_solv2d(wpic_ctx* c) {
 
 int wpic_field_fdtd_i2d(wpic_ctx* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
   if (!c->b_valid || !c->b_aligned || c->exch_pending)
     return set_error(WPIC_EINVAL, "field__fdtd_i needs gp slot-aligned with up (call particle__solv first)");
   int rc = deposit(c, false, false);
@@ -591,6 +628,7 @@ int wpic_field_fdtd_i2d(wpic_ctx* c) {
 
 int wpic_bc_particle_x2d(wpic_ctx* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
   if (!c->b_valid || c->exch_pending) return set_error(WPIC_EINVAL, "bc__particle_x needs gp before the y exchange");
   HIPCHK(launch_wrapx(c));
   return check_err(c);
@@ -598,6 +636,7 @@ int wpic_bc_particle_x2d(wpic_ctx* c) {
 
 int wpic_bc_particle_y2d(wpic_ctx* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
   if (!c->b_valid || c->exch_pending) return set_error(WPIC_EINVAL, "bc__particle_y needs gp before the y exchange");
   if (c->rn == c->rb || c->rn == c->ra) c->rn = pick_free(c->ra, c->rb);
   set_rows(c);
@@ -608,6 +647,7 @@ int wpic_bc_particle_y2d(wpic_ctx* c) {
 
 int wpic_sort_bucket2d(wpic_ctx* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
   if (!c->b_valid) return set_error(WPIC_EINVAL, "sort__bucket needs gp on the device");
   if (!c->exch_pending) {
     // the restart path: sort gp as it stands
@@ -658,7 +698,10 @@ int wpic_step2d(wpic_ctx* c, int nsteps) {
 // ---------------------------------------------------------------------------
 int wpic_energy2d(wpic_ctx* c, double out[3]) {
   if (!c || !out) return set_error(WPIC_EINVAL, "null argument");
-  HIPCHK(launch_energy(c, c->red));
+  if (c->mixed)
+    HIPCHK(launch_energy_mixed(c, c->red));
+  else
+    HIPCHK(launch_energy(c, c->red));
   std::vector<double> h(256 * 3);
   HIPCHK(hipMemcpyAsync(h.data(), c->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -716,7 +759,10 @@ int wpic_ic_beams2d(wpic_ctx* c, unsigned long long seed, int n_ppc, double udri
     }
   HIPCHK(hipMemcpyAsync(c->rows[c->ra], hrows.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->cumcnt, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice, c->st));
-  HIPCHK(launch_ic_beams(c, seed, n_ppc, udrift, uth_e, uth_i));
+  if (c->mixed)
+    HIPCHK(launch_ic_beams_mixed(c, seed, n_ppc, udrift, uth_e, uth_i));
+  else
+    HIPCHK(launch_ic_beams(c, seed, n_ppc, udrift, uth_e, uth_i));
   const size_t pl = (size_t)g.fw * g.fh;
   HIPCHK(launch_zero(c, c->uf, 6 * pl));
   HIPCHK(launch_fill(c, c->uf + 2 * pl, pl, b0));

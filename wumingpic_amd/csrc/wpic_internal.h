@@ -42,6 +42,21 @@ struct Parts {
   double *x, *y, *ux, *uy, *uz;
 };
 
+// mixed-precision particles: x cell index, FP32 in-cell offsets (x - ix and
+// y - row, both in [0, 1)) and FP32 momenta; 32 B per particle with the id
+struct PartsM {
+  int* ix;
+  float *ox, *oy, *ux, *uy, *uz;
+};
+
+// an in-cell offset in [0, 1) as FP32: rounding up to 1 is pulled back below
+__host__ __device__ inline float to_offset(double f) {
+  float o = (float)f;
+  if (o >= 1.0f) o = 0x1.fffffep-1f;
+  if (o < 0.0f) o = 0.0f;
+  return o;
+}
+
 // device error bits, OR-ed by kernels and checked by the host at sync points
 enum : int { ERR_DOMAIN = 1, ERR_CG = 2, ERR_ROW = 4 };
 
@@ -128,6 +143,8 @@ struct wpic_ctx {
 
   // particles: species s occupies [s*ncap, (s+1)*ncap); rows compact within
   wpic::Parts A, B;
+  bool mixed;            // WPIC_MODE_MIXED context: MA/MB hold the particles
+  wpic::PartsM MA, MB;
   uint64_t* id[2];
   int ida, idb;
   int *code, *R, *holes, *ulist, *dlist;
@@ -204,6 +221,13 @@ hipError_t launch_planes_to_aos(wpic_ctx* c, const double* planes, double* aos, 
 hipError_t launch_ic_beams(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
                            double uth_i);
 hipError_t launch_fill(wpic_ctx* c, double* p, size_t n, double v);
+// mixed-precision form
+hipError_t launch_push_deposit_mixed(wpic_ctx* c);
+hipError_t launch_up_to_mixed(wpic_ctx* c, const double* aos);
+hipError_t launch_mixed_to_aos(wpic_ctx* c, double* aos);
+hipError_t launch_ic_beams_mixed(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
+                                 double uth_i);
+hipError_t launch_energy_mixed(wpic_ctx* c, double* dout);
 
 void timer_begin(wpic_ctx* c, const char* name, hipEvent_t* a);
 void timer_end(wpic_ctx* c, hipEvent_t a);

@@ -1,0 +1,125 @@
+// wpic_internal3d.h -- grid, context and launchers of the 3D step loop
+// (periodic in x, y and z at one rank).  Only the library's own translation
+// units include this; the public boundary is include/wpic3d.h.
+#pragma once
+
+#include "../../include/wpic3d.h"
+#include "wpic_internal.h"
+
+namespace wpic {
+
+// Field arrays carry the reference's two ghost layers in every direction:
+// i in nxgs-2..nxge+2, j in nys-2..nye+2, k in nzs-2..nze+2, x fastest.
+struct Grid3D {
+  int nx, ny, nz, nxgs, nxge, nygs, nyge, nzgs, nzge, nys, nye, nzs, nze;
+  int fw, fh, fd;  // nx+4, ny+4, nz+4
+  int nsp, ncol;   // ncol = ny*nz (j, k) columns per species
+  double delx, delt, c, gfac, d_delx, d_delt;
+  double f1, f2, f3, f4, f5, pi4dt;  // 3d/common/field.f90:57-61; 4*pi*delt
+  double spanx, spany, spanz;        // (nxge-nxgs+1)*delx etc., round to nearest
+  double q[WPIC_MAXSP], r[WPIC_MAXSP];
+  long long ncap;
+};
+
+__host__ __device__ inline size_t fidx3(const Grid3D& g, int i, int j, int k) {
+  return (size_t)(i - (g.nxgs - 2)) + (size_t)g.fw * ((size_t)(j - (g.nys - 2)) + (size_t)g.fh * (k - (g.nzs - 2)));
+}
+__host__ __device__ inline size_t cidx3(const Grid3D& g, int i, int j, int k) {
+  return (size_t)(i - g.nxgs) + (size_t)g.nx * ((size_t)(j - g.nys) + (size_t)g.ny * (k - g.nzs));
+}
+__host__ __device__ inline size_t plane3(const Grid3D& g) { return (size_t)g.fw * g.fh * g.fd; }
+
+struct Parts3 {
+  double *x, *y, *z, *ux, *uy, *uz;
+};
+
+// Exchange direction of a particle leaving column (j, k) for (j+dj, k+dk):
+// dir = (dk+1)*3 + (dj+1); DIR_STAY = 4.  The exchange code of a particle is
+// (new x cell - nxgs) << 4 | dir.
+constexpr int DIR_STAY = 4;
+constexpr int CODE_SHIFT3 = 4;
+
+}  // namespace wpic
+
+struct wpic_ctx3d {
+  wpic_params3d P;
+  wpic::Grid3D g;
+  int mode;
+  hipStream_t st;
+  int device;
+
+  // particles: species s occupies [s*ncap, (s+1)*ncap); columns compact within
+  wpic::Parts3 A, B;
+  uint64_t* id[2];
+  int ida, idb;
+  int *code, *R, *holes, *mlist;  // [N]
+  int* cols[3];                   // column structures: base [nsp*ncol] then np2 [nsp*ncol]
+  int ca, cb, cn;                 // which column structure belongs to A, B, the exchange
+  int *colbaseA, *np2A, *cumcnt;  // cumcnt [nsp*ncol*(nx+1)]
+  int *colbaseB, *np2B;
+  int *colbaseN, *np2N;
+  int* dcnt;    // [nsp*ncol][9] movers per direction (DIR_STAY: stayers)
+  int* dstart;  // [nsp*ncol][9] exclusive start of each direction in mlist
+  int* colkm;   // [nsp*ncol][2] leavers, arrivals
+  int* seg;     // [ncol][8] arrival segments (source column << 4 | dir), in order
+  int* nseg;    // [ncol]
+  bool b_valid, b_aligned, exch_pending;
+  std::vector<int> h_cols[3];
+  bool h_cols_ok[3];
+
+  double *uf, *df, *uj;  // 6, 6, 3 components of (nx+4)(ny+4)(nz+4)
+  double* pjbuf;         // parity deposit blocks [nx*ny*nz][300] (lazy)
+
+  double *phi, *pa, *pb, *rr, *bb, *ap;  // CG vectors [3][nx*ny*nz]
+  double* part;
+  wpic::CgScal* scal;
+  int* err;
+  int cg_iters[3];
+
+  double* staging;
+  size_t staging_n;
+  long long* aosbase;  // [nsp*ncol]
+  double* gbuf;
+  double* red;
+
+  bool timing;
+  std::vector<wpic::TimerRec> timers;
+  std::vector<std::pair<std::string, std::pair<double, long long>>> tsum;
+};
+
+namespace wpic {
+
+int set_error(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// push / deposit
+hipError_t launch_push3d(wpic_ctx3d* c);
+hipError_t launch_deposit3d(wpic_ctx3d* c);
+hipError_t launch_gather_pj3d(wpic_ctx3d* c);
+// fields
+hipError_t launch_fill3d(wpic_ctx3d* c, double* p, size_t n, double v);
+hipError_t launch_curre_bc3d(wpic_ctx3d* c);
+hipError_t launch_dfield_bc3d(wpic_ctx3d* c);
+hipError_t launch_cg_init3d(wpic_ctx3d* c);
+hipError_t launch_cg_iter3d(wpic_ctx3d* c, int it);
+hipError_t launch_cg_store3d(wpic_ctx3d* c);
+hipError_t launch_e_update3d(wpic_ctx3d* c);
+hipError_t launch_uf_add3d(wpic_ctx3d* c);
+hipError_t launch_energy3d(wpic_ctx3d* c, double* dout);
+hipError_t launch_gauss3d(wpic_ctx3d* c, double* G, double* rho);
+// particles
+hipError_t launch_wrapx3d(wpic_ctx3d* c);
+hipError_t launch_colscan3d(wpic_ctx3d* c);
+hipError_t launch_colcount3d(wpic_ctx3d* c);
+hipError_t launch_colbuild3d(wpic_ctx3d* c);
+hipError_t launch_colsort3d(wpic_ctx3d* c);
+hipError_t launch_codes_identity3d(wpic_ctx3d* c);
+// transfers and initial state
+hipError_t launch_up_to_soa3d(wpic_ctx3d* c, const double* aos, long long n, int which);
+hipError_t launch_soa_to_aos3d(wpic_ctx3d* c, double* aos, int which);
+hipError_t launch_planes_from_aos3d(wpic_ctx3d* c, const double* aos, double* planes, int ncomp);
+hipError_t launch_planes_to_aos3d(wpic_ctx3d* c, const double* planes, double* aos, int ncomp);
+hipError_t launch_ic_beams3d(wpic_ctx3d* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
+                             double uth_i);
+
+}  // namespace wpic
