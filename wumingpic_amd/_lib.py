@@ -42,6 +42,13 @@ EXPORTS3D = (
 )
 
 
+# every symbol include/wpic_adj.h declares
+EXPORTS_ADJ = (
+    "wpic_set_cg_tol2d", "wpic_perturb_bz2d", "wpic_adj_forward2d", "wpic_adj_backward2d",
+    "wpic_adj_param_grads2d", "wpic_adj_download2d",
+)
+
+
 class Params2D(C.Structure):
     _fields_ = [
         ("ndim", C.c_int), ("nsp", C.c_int),
@@ -111,6 +118,12 @@ def lib():
         "wpic_ic_beams2d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
         "wpic_set_timing": [vp, C.c_int],
         "wpic_timing": [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
+        "wpic_set_cg_tol2d": [vp, C.c_double],
+        "wpic_perturb_bz2d": [vp, C.c_double, C.c_int],
+        "wpic_adj_forward2d": [vp, C.c_int, C.c_int, C.POINTER(C.c_double)],
+        "wpic_adj_backward2d": [vp, C.c_double],
+        "wpic_adj_param_grads2d": [vp, C.c_double, C.c_double, C.c_int, np.ctypeslib.ndpointer(np.float64)],
+        "wpic_adj_download2d": [vp, d, d, C.c_int],
         "wpic_create3d": [C.POINTER(vp), C.POINTER(Params3D)],
         "wpic_destroy3d": [vp],
         "wpic_set_mode3d": [vp, C.c_int],

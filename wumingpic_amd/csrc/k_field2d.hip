@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void k_cg_fin0(CgArgs a) {
   }
   if (threadIdx.x == 0) {
     a.sc[l].sum_g = t;
-    a.sc[l].eps = sqrt(t) * 1e-6;
+    a.sc[l].eps = sqrt(t) * a.g.cg_err;
   }
 }
 
@@ -473,6 +473,15 @@ hipError_t launch_dfield_bc(wpic_ctx* c) {
 hipError_t launch_cg_init(wpic_ctx* c) {
   CgArgs a = cg_args(c);
   hipLaunchKernelGGL(k_cg_init, dim3(NPB), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg_fin0, dim3(3), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg_r0, dim3(NPB), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg_fin1, dim3(3), dim3(256), 0, c->st, a);
+  return hipGetLastError();
+}
+
+// the solve's start after b and phi are set elsewhere (the adjoint's transposed solves)
+hipError_t launch_cg_start(wpic_ctx* c) {
+  CgArgs a = cg_args(c);
   hipLaunchKernelGGL(k_cg_fin0, dim3(3), dim3(256), 0, c->st, a);
   hipLaunchKernelGGL(k_cg_r0, dim3(NPB), dim3(256), 0, c->st, a);
   hipLaunchKernelGGL(k_cg_fin1, dim3(3), dim3(256), 0, c->st, a);

@@ -324,12 +324,14 @@ struct MoveF64 {
   Parts src, dst;
   const uint64_t* idsrc;
   uint64_t* iddst;
+  int* perm;  // optional: the gp slot each sorted slot was gathered from
   struct Rec {
     double x, y, ux, uy, uz;
     uint64_t id;
   };
   __device__ Rec load(int s) const { return {src.x[s], src.y[s], src.ux[s], src.uy[s], src.uz[s], idsrc[s]}; }
-  __device__ void store(int d, const Rec& r) const {
+  __device__ void store(int d, const Rec& r, int s) const {
+    if (perm) perm[d] = s;
     dst.x[d] = r.x;
     dst.y[d] = r.y;
     dst.ux[d] = r.ux;
@@ -351,7 +353,7 @@ struct MoveMixed {
   __device__ Rec load(int s) const {
     return {src.ix[s], src.ox[s], src.oy[s], src.ux[s], src.uy[s], src.uz[s], idsrc[s]};
   }
-  __device__ void store(int d, const Rec& r) const {
+  __device__ void store(int d, const Rec& r, int) const {
     dst.ix[d] = r.ix;
     dst.ox[d] = r.ox;
     dst.oy[d] = r.oy;
@@ -461,7 +463,7 @@ __global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, M mv, int nbits) {
       int pos = 0;
       if (act[u]) pos = hw[key[u]] + rank;
       if (act[u] && rank == 0) hw[key[u]] = pos + cnt;
-      if (act[u]) mv.store(nb + pos, v[u]);
+      if (act[u]) mv.store(nb + pos, v[u], src[u]);
     }
   }
 }
@@ -549,7 +551,7 @@ hipError_t launch_rowsort(wpic_ctx* c) {
     MoveMixed mv{c->MB, c->MA, a.idsrc, a.iddst};
     hipLaunchKernelGGL(k_rowsort<MoveMixed>, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, mv, nbits);
   } else {
-    MoveF64 mv{c->B, c->A, a.idsrc, a.iddst};
+    MoveF64 mv{c->B, c->A, a.idsrc, a.iddst, c->perm_out};
     hipLaunchKernelGGL(k_rowsort<MoveF64>, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, mv, nbits);
   }
   return hipGetLastError();

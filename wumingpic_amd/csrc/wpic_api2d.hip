@@ -154,28 +154,37 @@ int ensure_pjbuf(wpic_ctx* c) {
 }
 
 // field__fdtd_i after the deposit: current BCs, RHS, CG, E, BCs, update
+int cg_loop_impl(wpic_ctx* c) {
+  // iterations are launched NB at a time; the three components stop on their
+  // own device flags, the host polls once per batch
+  const int NB = 4;
+  int it = 0;
+  struct {
+    CgScal s[3];
+    int err;
+  } h;
+  for (;;) {
+    for (int b = 0; b < NB && it < 100; b++, it++) HIPCHK(launch_cg_iter(c, it));
+    HIPCHK(hipMemcpyAsync(h.s, c->scal, sizeof(h.s), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&h.err, c->err, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (h.err) return check_err(c);
+    if (!h.s[0].active && !h.s[1].active && !h.s[2].active) break;
+    if (it >= 100) return set_error(WPIC_ECG, "********** stop at cgm after ite_max **********");
+  }
+  for (int l = 0; l < 3; l++) c->cg_iters[l] = h.s[l].ite;
+  return 0;
+}
+
+// field__fdtd_i after the deposit: current BCs, RHS, CG, E, BCs, update
 int field_after_deposit(wpic_ctx* c) {
   Timed t(c, "field");
   HIPCHK(launch_curre_bc(c));
   {
     Timed tc(c, "cg");
     HIPCHK(launch_cg_init(c));
-    const int NB = 4;
-    int it = 0;
-    struct {
-      CgScal s[3];
-      int err;
-    } h;
-    for (;;) {
-      for (int b = 0; b < NB && it < 100; b++, it++) HIPCHK(launch_cg_iter(c, it));
-      HIPCHK(hipMemcpyAsync(h.s, c->scal, sizeof(h.s), hipMemcpyDeviceToHost, c->st));
-      HIPCHK(hipMemcpyAsync(&h.err, c->err, sizeof(int), hipMemcpyDeviceToHost, c->st));
-      HIPCHK(hipStreamSynchronize(c->st));
-      if (h.err) return check_err(c);
-      if (!h.s[0].active && !h.s[1].active && !h.s[2].active) break;
-      if (it >= 100) return set_error(WPIC_ECG, "********** stop at cgm after ite_max **********");
-    }
-    for (int l = 0; l < 3; l++) c->cg_iters[l] = h.s[l].ite;
+    int rc = cg_loop_impl(c);
+    if (rc) return rc;
     HIPCHK(launch_cg_store(c));
   }
   HIPCHK(launch_dfield_bc(c));
@@ -245,6 +254,33 @@ int sort_into_a(wpic_ctx* c) {
 
 }  // namespace
 
+namespace wpic {
+
+int step_once(wpic_ctx* c) {
+  // B shares A's slots and rows; the exchange writes rows N
+  c->rb = c->ra;
+  c->rn = (c->ra + 1) % 3;
+  set_rows(c);
+  c->idb = c->ida;
+  int rc = deposit(c, true, true);
+  if (rc) return rc;
+  c->b_valid = true;
+  c->b_aligned = true;
+  rc = field_after_deposit(c);
+  if (rc) return rc;
+  rc = exchange(c, false);
+  if (rc) return rc;
+  rc = sort_into_a(c);
+  if (rc) return rc;
+  c->h_rows_ok[c->ra] = false;
+  return 0;
+}
+
+int cg_loop(wpic_ctx* c) { return cg_loop_impl(c); }
+int check_err_ctx(wpic_ctx* c) { return check_err(c); }
+
+}  // namespace wpic
+
 extern "C" {
 
 const char* wpic_last_error(void) { return g_err.c_str(); }
@@ -310,6 +346,7 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
     g.r[s] = p->r[s];
   }
   g.ncap = p->capacity;
+  g.cg_err = 1e-6;
 
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
@@ -373,6 +410,7 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
 int wpic_destroy(wpic_ctx* c) {
   if (!c) return 0;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  adj_free(c);
   for (auto& t : c->timers) {
     if (t.a) (void)hipEventDestroy(t.a);
     if (t.b) (void)hipEventDestroy(t.b);
@@ -673,22 +711,8 @@ int wpic_cg_iters2d(wpic_ctx* c, int iters[3]) {
 int wpic_step2d(wpic_ctx* c, int nsteps) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
   for (int it = 0; it < nsteps; it++) {
-    // B shares A's slots and rows; the exchange writes rows N
-    c->rb = c->ra;
-    c->rn = (c->ra + 1) % 3;
-    set_rows(c);
-    c->idb = c->ida;
-    int rc = deposit(c, true, true);
+    int rc = step_once(c);
     if (rc) return rc;
-    c->b_valid = true;
-    c->b_aligned = true;
-    rc = field_after_deposit(c);
-    if (rc) return rc;
-    rc = exchange(c, false);
-    if (rc) return rc;
-    rc = sort_into_a(c);
-    if (rc) return rc;
-    c->h_rows_ok[c->ra] = false;
   }
   return check_err(c);
 }

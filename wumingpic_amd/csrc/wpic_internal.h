@@ -28,6 +28,7 @@ struct Grid2D {
   double wall_lo, wall_hi;           // 2*(nxs+1)*delx, 2*(nxe-1)*delx
   double q[WPIC_MAXSP], r[WPIC_MAXSP];
   long long ncap;                    // particle capacity per species
+  double cg_err;                     // CG relative tolerance (field.f90:339: 1e-6)
 };
 
 __host__ __device__ inline int fidx(const Grid2D& g, int i, int j) {
@@ -182,6 +183,9 @@ struct wpic_ctx {
   bool timing;
   std::vector<wpic::TimerRec> timers;
   std::vector<std::pair<std::string, std::pair<double, long long>>> tsum;
+
+  int* perm_out;          // when set, the sort records the gp slot of every sorted slot
+  struct AdjState* adj;   // recorded run and adjoint arrays (wpic_adj.h), lazy
 };
 
 namespace wpic {
@@ -228,6 +232,27 @@ hipError_t launch_mixed_to_aos(wpic_ctx* c, double* aos);
 hipError_t launch_ic_beams_mixed(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
                                  double uth_i);
 hipError_t launch_energy_mixed(wpic_ctx* c, double* dout);
+
+// one fused step (wpic_step2d's body) and the CG iteration loop of field__fdtd_i
+int step_once(wpic_ctx* c);
+int cg_loop(wpic_ctx* c);
+int check_err_ctx(wpic_ctx* c);
+hipError_t launch_cg_start(wpic_ctx* c);
+// adjoint
+hipError_t launch_adj_unsort(wpic_ctx* c, const int* rows, const int* perm, const double* lamA, double* lamB,
+                             size_t N);
+hipError_t launch_adj_field1(wpic_ctx* c, const double* lamF, double* lamF2, double* lamJ);
+hipError_t launch_adj_field2(wpic_ctx* c, double* lamF2, double* lamJ);
+hipError_t launch_adj_particles(wpic_ctx* c, const Parts& P, const int* rows, const double* uf, const double* lamB,
+                                const double* lamJ, double* lamA, double* lamT, size_t N);
+hipError_t launch_adj_t2f(wpic_ctx* c, const double* lamT, double* lamF);
+hipError_t launch_adj_seed(wpic_ctx* c, double* lamF, double seed);
+hipError_t launch_adj_params(wpic_ctx* c, const Parts& P, const uint64_t* id, const int* rows, const double* lamA,
+                             size_t N, const double* lamF, double udrift, double uth_e, int mode, int n_ppc,
+                             double* out);
+hipError_t launch_perturb_bz(wpic_ctx* c, double amp, int mode);
+hipError_t launch_adj_lamF_to_planes(wpic_ctx* c, const double* lamF, double* planes);
+void adj_free(wpic_ctx* c);
 
 void timer_begin(wpic_ctx* c, const char* name, hipEvent_t* a);
 void timer_end(wpic_ctx* c, hipEvent_t a);

@@ -174,6 +174,34 @@ class Pic2D:
         check(self._L.wpic_ic_beams2d(self._h, int(seed), int(n_ppc), float(udrift), float(uth_e),
                                       float(uth_i), float(b0)))
 
+    # -- adjoint (include/wpic_adj.h) ---------------------------------------------
+    def set_cg_tol(self, err):
+        """CG relative tolerance (the reference's err = 1e-6, 2d/common/field.f90:339)"""
+        check(self._L.wpic_set_cg_tol2d(self._h, float(err)))
+
+    def perturb_bz(self, amp, mode=1):
+        check(self._L.wpic_perturb_bz2d(self._h, float(amp), int(mode)))
+
+    def adj_forward(self, nsteps, every):
+        """nsteps with a checkpoint every `every` steps; returns the final W_B"""
+        wb = C.c_double()
+        check(self._L.wpic_adj_forward2d(self._h, int(nsteps), int(every), C.byref(wb)))
+        return wb.value
+
+    def adj_backward(self, seed=1.0):
+        check(self._L.wpic_adj_backward2d(self._h, float(seed)))
+
+    def adj_param_grads(self, udrift, uth_e, bz_mode=1):
+        out = np.zeros(3)
+        check(self._L.wpic_adj_param_grads2d(self._h, float(udrift), float(uth_e), int(bz_mode), out))
+        return out
+
+    def adj_download(self, np_cap):
+        lam_up = np.zeros((6, np_cap, self.ny, self.nsp), order="F")
+        lam_uf = np.zeros((6, self.nx + 4, self.ny + 4), order="F")
+        check(self._L.wpic_adj_download2d(self._h, lam_up, lam_uf, np_cap))
+        return lam_up, lam_uf
+
     def set_timing(self, on=True):
         check(self._L.wpic_set_timing(self._h, 1 if on else 0))
 
