@@ -49,3 +49,22 @@ def test_adjoint_of_the_initial_state_downloads_in_reference_layout():
     assert np.abs(lam_uf[:, 2:18, 2:18]).max() > 0
     assert np.abs(lam_uf[:, :2, :]).max() == 0 and np.abs(lam_uf[:, :, :2]).max() == 0
     assert np.abs(lam_up[2:5]).max() > 0
+
+
+def test_adjoint_called_from_fortran_matches_the_harness():
+    """the adjoint entry points through their ISO_C_BINDING interfaces
+    (wumingpic_amd/fortran/adj_driver) give the harness's gradients"""
+    import os
+    import subprocess
+
+    from wumingpic_amd.adjoint import WeibelCase, WeibelMagneticEnergy
+    exe = os.path.join(os.path.dirname(__file__), "..", "wumingpic_amd", "fortran", "adj_driver")
+    out = subprocess.run([exe, "32", "32", "8", "12", "4", "0.5", "0.1", "1e-3"], capture_output=True, text=True,
+                         check=True, timeout=120).stdout.split()
+    wb_f, g_f = float(out[0]), np.array([float(v) for v in out[1:4]])
+    case = WeibelCase(nx=32, ny=32, n_ppc=8, nsteps=12, every=4)
+    t = torch.tensor([0.5, 0.1, 1e-3], dtype=torch.float64, requires_grad=True)
+    wb = WeibelMagneticEnergy.apply(t, case)
+    wb.backward()
+    assert abs(wb_f - float(wb.detach())) <= 1e-12 * abs(wb_f)
+    assert np.allclose(g_f, t.grad.numpy(), rtol=1e-9, atol=0)

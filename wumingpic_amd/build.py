@@ -44,6 +44,7 @@ FORT = os.path.join(HERE, "fortran")
 FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/amdflang")
 FSRC = ["wpic_capi.f90", "wpic_shim_state.f90", "shim_modules.f90"]
 FDRIVER = "shim_step2d_driver.f90"
+FADJ = "adj_driver.f90"
 
 
 def build_fortran():
@@ -68,6 +69,17 @@ def build_fortran():
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link of the Fortran driver failed:\n{r.stdout}\n{r.stderr}")
+    # the adjoint called from Fortran
+    aobj = os.path.join(mod, FADJ + ".o")
+    cmd = [FLANG, "-O2", "-fPIC", "-module-dir", mod, "-I", mod, "-c", os.path.join(FORT, FADJ), "-o", aobj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed for {FADJ}:\n{r.stdout}\n{r.stderr}")
+    aexe = os.path.join(FORT, "adj_driver")
+    cmd = [FLANG, "-o", aexe, aobj, archive, "-L" + HERE, "-lwpic", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link of the Fortran adjoint driver failed:\n{r.stdout}\n{r.stderr}")
     return exe
 
 

@@ -170,7 +170,7 @@ struct Cg3Args {
   const double *uf, *uj;
   double* df;
   double *phi, *pold, *pnew, *r, *b, *ap;  // [3][nx*ny*nz]
-  double* part;                            // [3][NPB][2]
+  double* part;                            // [3][NPB3][2]
   CgScal* sc;
   int* err;
   int parity;
@@ -200,8 +200,13 @@ __device__ inline double serial_dot3(const double* x, const double* y, size_t n)
   return s;
 }
 
+// the NPB3 block partials: each thread adds NPB3/256 consecutive ones in
+// order, then a fixed tree over the block
 __device__ inline double part_sum3(const Cg3Args& a, int l, int slot, double* sh) {
-  return bsum256(a.part[((size_t)l * NPB + threadIdx.x) * 2 + slot], sh);
+  constexpr int PER = NPB3 / 256;
+  double v = 0.0;
+  for (int q = 0; q < PER; q++) v = v + a.part[((size_t)l * NPB3 + threadIdx.x * PER + q) * 2 + slot];
+  return bsum256(v, sh);
 }
 
 // gkl (field.f90:128-158), b = f5*gkl, warm start phi = df(l), sum b^2
@@ -214,7 +219,7 @@ __global__ __launch_bounds__(256) void k_cg3_init(Cg3Args a) {
   const double *J1 = a.uj, *J2 = a.uj + pl, *J3 = a.uj + 2 * pl;
   const double f1 = g.f1, f2 = g.f2, f3 = g.f3, f5 = g.f5;
   double s[3] = {0.0, 0.0, 0.0};
-  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)NPB * 256) {
+  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)NPB3 * 256) {
     int i, j, k;
     decode3(g, q, i, j, k);
 #define F(P, ii, jj, kk) P[fidx3(g, ii, jj, kk)]
@@ -244,7 +249,7 @@ __global__ __launch_bounds__(256) void k_cg3_init(Cg3Args a) {
   if (!a.parity) {
     for (int l = 0; l < 3; l++) {
       const double t = bsum256(s[l], sh);
-      if (threadIdx.x == 0) a.part[((size_t)l * NPB + blockIdx.x) * 2] = t;
+      if (threadIdx.x == 0) a.part[((size_t)l * NPB3 + blockIdx.x) * 2] = t;
     }
   }
 }
@@ -272,7 +277,7 @@ __global__ __launch_bounds__(256) void k_cg3_r0(Cg3Args a) {
   const size_t n = (size_t)g.nx * g.ny * g.nz;
   const double f4 = g.f4;
   double s[3] = {0.0, 0.0, 0.0};
-  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)NPB * 256) {
+  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)NPB3 * 256) {
     int i, j, k;
     decode3(g, q, i, j, k);
 #pragma unroll
@@ -289,7 +294,7 @@ __global__ __launch_bounds__(256) void k_cg3_r0(Cg3Args a) {
   if (!a.parity) {
     for (int l = 0; l < 3; l++) {
       const double t = bsum256(s[l], sh);
-      if (threadIdx.x == 0) a.part[((size_t)l * NPB + blockIdx.x) * 2] = t;
+      if (threadIdx.x == 0) a.part[((size_t)l * NPB3 + blockIdx.x) * 2] = t;
     }
   }
 }
@@ -325,7 +330,7 @@ __global__ __launch_bounds__(256) void k_cg3_a(Cg3Args a) {
   double* pn = a.pnew + (size_t)l * n;
   double* ap = a.ap + (size_t)l * n;
   double s_r = 0.0, s_2 = 0.0;
-  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)NPB * 256) {
+  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)NPB3 * 256) {
     int i, j, k;
     decode3(g, q, i, j, k);
     size_t m;
@@ -344,8 +349,8 @@ __global__ __launch_bounds__(256) void k_cg3_a(Cg3Args a) {
     const double t1 = bsum256(s_r, sh);
     const double t2 = bsum256(s_2, sh);
     if (threadIdx.x == 0) {
-      a.part[((size_t)l * NPB + blockIdx.x) * 2] = t1;
-      a.part[((size_t)l * NPB + blockIdx.x) * 2 + 1] = t2;
+      a.part[((size_t)l * NPB3 + blockIdx.x) * 2] = t1;
+      a.part[((size_t)l * NPB3 + blockIdx.x) * 2 + 1] = t2;
     }
   }
 }
@@ -383,7 +388,7 @@ __global__ __launch_bounds__(256) void k_cg3_b(Cg3Args a) {
   double *ph = a.phi + (size_t)l * n, *r = a.r + (size_t)l * n;
   const double *p = a.pnew + (size_t)l * n, *ap = a.ap + (size_t)l * n;
   double s = 0.0;
-  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)NPB * 256) {
+  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)NPB3 * 256) {
     ph[q] = ph[q] + av * p[q];
     const double rv = r[q] - av * ap[q];
     r[q] = rv;
@@ -391,7 +396,7 @@ __global__ __launch_bounds__(256) void k_cg3_b(Cg3Args a) {
   }
   if (!a.parity) {
     const double t = bsum256(s, sh);
-    if (threadIdx.x == 0) a.part[((size_t)l * NPB + blockIdx.x) * 2] = t;
+    if (threadIdx.x == 0) a.part[((size_t)l * NPB3 + blockIdx.x) * 2] = t;
   }
 }
 
@@ -521,9 +526,9 @@ hipError_t launch_dfield_bc3d(wpic_ctx3d* c) {
 
 hipError_t launch_cg_init3d(wpic_ctx3d* c) {
   Cg3Args a = cg3_args(c);
-  hipLaunchKernelGGL(k_cg3_init, dim3(NPB), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg3_init, dim3(NPB3), dim3(256), 0, c->st, a);
   hipLaunchKernelGGL(k_cg3_fin0, dim3(3), dim3(256), 0, c->st, a);
-  hipLaunchKernelGGL(k_cg3_r0, dim3(NPB), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg3_r0, dim3(NPB3), dim3(256), 0, c->st, a);
   hipLaunchKernelGGL(k_cg3_fin1, dim3(3), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }
@@ -534,9 +539,9 @@ hipError_t launch_cg_iter3d(wpic_ctx3d* c, int it) {
     a.pold = c->pb;
     a.pnew = c->pa;
   }
-  hipLaunchKernelGGL(k_cg3_a, dim3(NPB, 3), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg3_a, dim3(NPB3, 3), dim3(256), 0, c->st, a);
   hipLaunchKernelGGL(k_cg3_fa, dim3(3), dim3(256), 0, c->st, a);
-  hipLaunchKernelGGL(k_cg3_b, dim3(NPB, 3), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg3_b, dim3(NPB3, 3), dim3(256), 0, c->st, a);
   hipLaunchKernelGGL(k_cg3_fb, dim3(3), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }

@@ -14,8 +14,9 @@ struct XferArgs {
   Parts P;
   uint64_t* id;
   const int *rowbase, *np2;
-  const int* R;  // non-null: B seen through the exchange permutation
+  const int* R;  // non-null: records seen through a permutation (the exchange's R, or A's aperm)
   const long long* aosbase;  // [nsp*ny] start of each row in the AoS buffer
+  bool id_direct;            // ids stored in slot order (A's ids, sorted by the sort itself)
 };
 
 __global__ void k_aos_to_soa(XferArgs a, const double* __restrict__ aos) {
@@ -49,7 +50,7 @@ __global__ void k_soa_to_aos(XferArgs a, double* __restrict__ aos) {
     d[2] = a.P.ux[s];
     d[3] = a.P.uy[s];
     d[4] = a.P.uz[s];
-    const uint64_t b = a.id[s];
+    const uint64_t b = a.id[a.id_direct ? base + k : s];
     __builtin_memcpy(&d[5], &b, 8);
   }
 }
@@ -91,6 +92,7 @@ hipError_t launch_up_to_soa(wpic_ctx* c, const double* aos, long long n, int whi
   a.np2 = which == 0 ? c->np2A : c->np2B;
   a.R = nullptr;
   a.aosbase = c->aosbase;
+  a.id_direct = true;
   hipLaunchKernelGGL(k_aos_to_soa, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, aos);
   return hipGetLastError();
 }
@@ -103,7 +105,7 @@ hipError_t launch_soa_to_aos(wpic_ctx* c, double* aos, int which) {
   if (which == 0) {
     a.rowbase = c->rowbaseA;
     a.np2 = c->np2A;
-    a.R = nullptr;
+    a.R = c->aperm;
   } else if (which == 1) {
     a.rowbase = c->rowbaseN;
     a.np2 = c->np2N;
@@ -114,6 +116,7 @@ hipError_t launch_soa_to_aos(wpic_ctx* c, double* aos, int which) {
     a.R = nullptr;
   }
   a.aosbase = c->aosbase;
+  a.id_direct = which == 0;
   hipLaunchKernelGGL(k_soa_to_aos, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, aos);
   return hipGetLastError();
 }
@@ -147,14 +150,15 @@ __global__ void k_mixed_to_aos(XferArgs a, PartsM M, double* __restrict__ aos) {
   const int j = a.g.nys + rw % a.g.ny;
   double* dst = aos + a.aosbase[rw] * 6;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const int s = base + k;
+    int s = a.R ? a.R[base + k] : base + k;
+    if ((unsigned)s >= (unsigned)(a.g.ncap * a.g.nsp)) s = 0;
     double* d = dst + (size_t)k * 6;
     d[0] = (double)M.ix[s] + (double)M.ox[s];
     d[1] = (double)j + (double)M.oy[s];
     d[2] = M.ux[s];
     d[3] = M.uy[s];
     d[4] = M.uz[s];
-    const uint64_t b = a.id[s];
+    const uint64_t b = a.id[base + k];
     __builtin_memcpy(&d[5], &b, 8);
   }
 }
@@ -167,6 +171,7 @@ hipError_t launch_up_to_mixed(wpic_ctx* c, const double* aos) {
   a.np2 = c->np2A;
   a.R = nullptr;
   a.aosbase = c->aosbase;
+  a.id_direct = true;
   hipLaunchKernelGGL(k_aos_to_mixed, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, c->MA, aos);
   return hipGetLastError();
 }
@@ -177,9 +182,32 @@ hipError_t launch_mixed_to_aos(wpic_ctx* c, double* aos) {
   a.id = c->id[c->ida];
   a.rowbase = c->rowbaseA;
   a.np2 = c->np2A;
-  a.R = nullptr;
+  a.R = c->aperm;
   a.aosbase = c->aosbase;
+  a.id_direct = true;
   hipLaunchKernelGGL(k_mixed_to_aos, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, c->MA, aos);
+  return hipGetLastError();
+}
+
+// the sorted state in slot order: dst[slot] = src[ap[slot]] over the active
+// slots of the rows (rowbase then np2)
+__global__ void k_gather_rows(Grid2D g, Parts src, const int* ap, Parts dst, const int* rows) {
+  const int nr = g.nsp * g.ny;
+  const int rw = blockIdx.x;
+  const int base = rows[rw], n = rows[nr + rw];
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int d = base + k;
+    const int s = ap ? ap[d] : d;
+    dst.x[d] = src.x[s];
+    dst.y[d] = src.y[s];
+    dst.ux[d] = src.ux[s];
+    dst.uy[d] = src.uy[s];
+    dst.uz[d] = src.uz[s];
+  }
+}
+
+hipError_t launch_gather_rows(wpic_ctx* c, const Parts& src, const int* ap, const Parts& dst, const int* rows) {
+  hipLaunchKernelGGL(k_gather_rows, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, c->g, src, ap, dst, rows);
   return hipGetLastError();
 }
 
@@ -313,7 +341,7 @@ __device__ inline double bsum(double v, double* sh) {
 // out[blk*3 + {0,1,2}] = partial {kinetic, E^2, B^2}
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_energy(Grid2D g, Parts A, PartsM M, const int* rowbase, const int* np2,
-                                                const double* uf, double* out) {
+                                                const int* ap, const double* uf, double* out) {
   __shared__ double sh[4];
   double kin = 0.0, e2 = 0.0, b2 = 0.0;
   const int nrow = g.nsp * g.ny;
@@ -321,7 +349,7 @@ __global__ __launch_bounds__(256) void k_energy(Grid2D g, Parts A, PartsM M, con
     const int s = rw / g.ny;
     const int base = rowbase[rw], n = np2[rw];
     for (int k = threadIdx.x; k < n; k += 256) {
-      const int p = base + k;
+      const int p = ap ? ap[base + k] : base + k;
       const double ux = MIXED ? (double)M.ux[p] : A.ux[p];
       const double uy = MIXED ? (double)M.uy[p] : A.uy[p];
       const double uz = MIXED ? (double)M.uz[p] : A.uz[p];
@@ -348,28 +376,28 @@ __global__ __launch_bounds__(256) void k_energy(Grid2D g, Parts A, PartsM M, con
 }
 
 hipError_t launch_energy(wpic_ctx* c, double* dout) {
-  hipLaunchKernelGGL(k_energy<false>, dim3(RB), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA, c->np2A, c->uf,
-                     dout);
+  hipLaunchKernelGGL(k_energy<false>, dim3(RB), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA, c->np2A,
+                     c->aperm, c->uf, dout);
   return hipGetLastError();
 }
 
 hipError_t launch_energy_mixed(wpic_ctx* c, double* dout) {
-  hipLaunchKernelGGL(k_energy<true>, dim3(RB), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA, c->np2A, c->uf,
-                     dout);
+  hipLaunchKernelGGL(k_energy<true>, dim3(RB), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA, c->np2A,
+                     c->aperm, c->uf, dout);
   return hipGetLastError();
 }
 
 // rho with the quadratic shape at cell centres, then G = div E - 4 pi rho
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_rho(Grid2D g, Parts A, PartsM M, const int* rowbase, const int* np2,
-                                             double* rho) {
+                                             const int* ap, double* rho) {
   const int rw = blockIdx.x;
   const int s = rw / g.ny;
   const int base = rowbase[rw], n = np2[rw];
   const int gw = g.fw;
   const int jrow = g.nys + rw % g.ny;
   for (int k = threadIdx.x; k < n; k += 256) {
-    const int p = base + k;
+    const int p = ap ? ap[base + k] : base + k;
     const double x = MIXED ? (double)M.ix[p] + (double)M.ox[p] : A.x[p];
     const double y = MIXED ? (double)jrow + (double)M.oy[p] : A.y[p];
     const int i0 = (int)(x * g.d_delx), j0 = (int)(y * g.d_delx);
@@ -418,10 +446,10 @@ hipError_t launch_gauss(wpic_ctx* c, double* G, double* rho) {
   if (e != hipSuccess) return e;
   if (c->mixed)
     hipLaunchKernelGGL(k_rho<true>, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA,
-                       c->np2A, rho);
+                       c->np2A, c->aperm, rho);
   else
     hipLaunchKernelGGL(k_rho<false>, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA,
-                       c->np2A, rho);
+                       c->np2A, c->aperm, rho);
   if (c->g.bc == WPIC_BC_PERIODIC) {
     hipLaunchKernelGGL(k_rho_fold_y, dim3((c->g.nx + 2 + 255) / 256), dim3(256), 0, c->st, c->g, rho);
     hipLaunchKernelGGL(k_rho_fold_x, dim3((c->g.ny + 255) / 256), dim3(256), 0, c->st, c->g, rho);

@@ -25,6 +25,7 @@ struct MixArgs {
   PartsM A, B;
   const int* rowbase;
   const int* cumcnt;
+  const int* ap;  // A's slot -> record (null: identity)
   const double* uf;
   double* uj;
   int* code;
@@ -118,7 +119,8 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
       const float qk = (float)(g.q[s] * g.delx * g.d_delt);
       const float qf = (float)g.q[s];
       for (int pi = pbeg + lg; pi < pend; pi += G) {
-        const float ox = a.A.ox[pi], oy = a.A.oy[pi];
+        const int ia = a.ap ? a.ap[pi] : pi;
+        const float ox = a.A.ox[ia], oy = a.A.oy[ia];
         float sx0, sx1, sx2, sy0, sy1, sy2;
         shape3f(ox - 0.5f, sx0, sx1, sx2);
         shape3f(oy - 0.5f, sy0, sy1, sy2);
@@ -133,9 +135,9 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
                    (f[SFW - 1] * sx0 + f[SFW] * sx1 + f[SFW + 1] * sx2) * sy2;
         }
         const float bpx = fl[0], bpy = fl[1], bpz = fl[2], epx = fl[3], epy = fl[4], epz = fl[5];
-        float uvm1 = a.A.ux[pi] + fac1 * epx;
-        float uvm2 = a.A.uy[pi] + fac1 * epy;
-        float uvm3 = a.A.uz[pi] + fac1 * epz;
+        float uvm1 = a.A.ux[ia] + fac1 * epx;
+        float uvm2 = a.A.uy[ia] + fac1 * epy;
+        float uvm3 = a.A.uz[ia] + fac1 * epz;
         const float gam = sqrtf(cc2 + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
         const float igam = 1.0f / gam;
         const float fac1r = fac1 * igam;
@@ -258,6 +260,7 @@ hipError_t launch_push_deposit_mixed(wpic_ctx* c) {
   a.B = c->MB;
   a.rowbase = c->rowbaseA;
   a.cumcnt = c->cumcnt;
+  a.ap = c->aperm;
   a.uf = c->uf;
   a.uj = c->uj;
   a.code = c->code;

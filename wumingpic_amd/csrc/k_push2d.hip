@@ -29,6 +29,7 @@ struct PushArgs {
   Parts A, B;
   const int* rowbase;
   const int* cumcnt;
+  const int* ap;  // A's slot -> record (null: identity)
   const double* uf;
   double* uj;
   double* pjbuf;
@@ -235,8 +236,9 @@ __device__ inline void deposit_pass(const PushArgs& a, double (*acc)[ACW * ACH],
     const double qk = q * g.delx * g.d_delt;
     for (int pi = pbeg + lg; pi < pend; pi += G) {
       double nxp = a.B.x[pi], nyp = a.B.y[pi];
+      const int ia = a.ap ? a.ap[pi] : pi;
       double s0x[5], s0y[5], dsx[5], dsy[5];
-      esirkepov_shapes(g, a.A.x[pi], a.A.y[pi], nxp, nyp, i, j, s0x, s0y, dsx, dsy);
+      esirkepov_shapes(g, a.A.x[ia], a.A.y[ia], nxp, nyp, i, j, s0x, s0y, dsx, dsy);
       if (DZ) {
         const double g3 = a.B.ux[pi], g4 = a.B.uy[pi], g5 = a.B.uz[pi];
         const double u2 = +g3 * g3 + g4 * g4 + g5 * g5;
@@ -341,12 +343,13 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
       const double txxx = fac1 * fac1;
       const double fac2 = q * delt / rm;
       for (int pi = pbeg + lg; pi < pend; pi += G) {
-        const double x = a.A.x[pi], y = a.A.y[pi];
+        const int ia = a.ap ? a.ap[pi] : pi;
+        const double x = a.A.x[ia], y = a.A.y[ia];
         double nxp, nyp, g3, g4, g5;
         if (PUSH) {
           double gmv;
-          boris_push(sf, li, lj, i, j, d_delx, delt, c, unit_c, fac1, txxx, fac2, x, y, a.A.ux[pi], a.A.uy[pi],
-                     a.A.uz[pi], nxp, nyp, g3, g4, g5, gmv);
+          boris_push(sf, li, lj, i, j, d_delx, delt, c, unit_c, fac1, txxx, fac2, x, y, a.A.ux[ia], a.A.uy[ia],
+                     a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
         } else {
           nxp = a.B.x[pi];
           nyp = a.B.y[pi];
@@ -456,10 +459,11 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
       const double fac2 = q * delt / rm;
       const double qk = q * g.delx * g.d_delt;
       for (int pi = pbeg + lg; pi < pend; pi += G) {
-        const double x = a.A.x[pi], y = a.A.y[pi];
+        const int ia = a.ap ? a.ap[pi] : pi;
+        const double x = a.A.x[ia], y = a.A.y[ia];
         double nxp, nyp, g3, g4, g5, gmv;
-        boris_push(sf, li, lj, i, j, d_delx, delt, c, unit_c, fac1, txxx, fac2, x, y, a.A.ux[pi], a.A.uy[pi],
-                   a.A.uz[pi], nxp, nyp, g3, g4, g5, gmv);
+        boris_push(sf, li, lj, i, j, d_delx, delt, c, unit_c, fac1, txxx, fac2, x, y, a.A.ux[ia], a.A.uy[ia],
+                   a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
         if (FUSED && BC == WPIC_BC_WALLX) {
           const int ipos = (int)(nxp / g.delx);
           if (ipos < g.nxgs + 1) {
@@ -582,6 +586,7 @@ hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused)
   a.B = c->B;
   a.rowbase = c->rowbaseA;
   a.cumcnt = c->cumcnt;
+  a.ap = c->aperm;
   a.uf = c->uf;
   a.uj = c->uj;
   a.pjbuf = c->pjbuf;

@@ -310,56 +310,27 @@ __device__ inline uint64_t key_peers(int key, bool act, int nbits) {
   return peers;
 }
 
-// stable counting sort of one row by cell (sort.f90:52-78), gathering B -> A
-// (the FP64 or the mixed record, by the mover M).
+// stable counting sort of one row by cell (sort.f90:52-78): the permutation
+// and the ids, by the mover M.
 // Each wave owns a contiguous quarter of the row: its per-key histogram, then
 // (after a scan over keys and waves) its per-key write cursors.  Within a
 // wave, lanes with equal keys are ranked by one ballot per key bit.  Chunks
 // of 64 are processed U at a time so their loads overlap.
 constexpr int SU = 4;
 
-// particle record movers of the sort: the FP64 record (x, y, ux, uy, uz, id)
-// and the mixed record (x cell, FP32 offsets and momenta, id)
-struct MoveF64 {
-  Parts src, dst;
+// The sort moves no particle record: it writes, for every sorted slot, the gp
+// slot its particle comes from (the next push reads the records through it),
+// and gathers the ids into slot order.
+struct MovePerm {
   const uint64_t* idsrc;
   uint64_t* iddst;
-  int* perm;  // optional: the gp slot each sorted slot was gathered from
+  int* perm;
   struct Rec {
-    double x, y, ux, uy, uz;
     uint64_t id;
   };
-  __device__ Rec load(int s) const { return {src.x[s], src.y[s], src.ux[s], src.uy[s], src.uz[s], idsrc[s]}; }
+  __device__ Rec load(int s) const { return {idsrc[s]}; }
   __device__ void store(int d, const Rec& r, int s) const {
-    if (perm) perm[d] = s;
-    dst.x[d] = r.x;
-    dst.y[d] = r.y;
-    dst.ux[d] = r.ux;
-    dst.uy[d] = r.uy;
-    dst.uz[d] = r.uz;
-    iddst[d] = r.id;
-  }
-};
-
-struct MoveMixed {
-  PartsM src, dst;
-  const uint64_t* idsrc;
-  uint64_t* iddst;
-  struct Rec {
-    int ix;
-    float ox, oy, ux, uy, uz;
-    uint64_t id;
-  };
-  __device__ Rec load(int s) const {
-    return {src.ix[s], src.ox[s], src.oy[s], src.ux[s], src.uy[s], src.uz[s], idsrc[s]};
-  }
-  __device__ void store(int d, const Rec& r, int) const {
-    dst.ix[d] = r.ix;
-    dst.ox[d] = r.ox;
-    dst.oy[d] = r.oy;
-    dst.ux[d] = r.ux;
-    dst.uy[d] = r.uy;
-    dst.uz[d] = r.uz;
+    perm[d] = s;
     iddst[d] = r.id;
   }
 };
@@ -547,13 +518,8 @@ hipError_t launch_rowsort(wpic_ctx* c) {
   int nbits = 0;
   while ((1 << nbits) < c->g.nx) nbits++;
   const size_t shm = sizeof(int) * (SB / 64) * (size_t)c->g.nx;
-  if (c->mixed) {
-    MoveMixed mv{c->MB, c->MA, a.idsrc, a.iddst};
-    hipLaunchKernelGGL(k_rowsort<MoveMixed>, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, mv, nbits);
-  } else {
-    MoveF64 mv{c->B, c->A, a.idsrc, a.iddst, c->perm_out};
-    hipLaunchKernelGGL(k_rowsort<MoveF64>, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, mv, nbits);
-  }
+  MovePerm mv{a.idsrc, a.iddst, c->perm_out ? c->perm_out : c->permbuf};
+  hipLaunchKernelGGL(k_rowsort<MovePerm>, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, mv, nbits);
   return hipGetLastError();
 }
 

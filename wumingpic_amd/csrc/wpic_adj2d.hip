@@ -83,9 +83,8 @@ int copy_d(wpic_ctx* c, void* dst, const void* src, size_t bytes) {
 int save_state(wpic_ctx* c, double* part, uint64_t* id, int* rows, int* cumcnt, double* uf, double* df) {
   AdjState& a = *c->adj;
   const size_t N = a.N;
-  const double* src[5] = {c->A.x, c->A.y, c->A.ux, c->A.uy, c->A.uz};
-  for (int k = 0; k < 5; k++)
-    if (part && copy_d(c, part + k * N, src[k], sizeof(double) * N)) return WPIC_EHIP;
+  // the sorted particles in slot order (A is read through its permutation)
+  if (part) HIPCHK(launch_gather_rows(c, c->A, c->aperm, parts_of(part, N), c->rows[c->ra]));
   if (id && copy_d(c, id, c->id[c->ida], sizeof(uint64_t) * N)) return WPIC_EHIP;
   if (rows && copy_d(c, rows, c->rows[c->ra], sizeof(int) * 2 * a.nr)) return WPIC_EHIP;
   if (cumcnt && copy_d(c, cumcnt, c->cumcnt, sizeof(int) * a.ncc)) return WPIC_EHIP;
@@ -100,6 +99,7 @@ int restore_state(wpic_ctx* c, int s) {
   double* dst[5] = {c->A.x, c->A.y, c->A.ux, c->A.uy, c->A.uz};
   for (int k = 0; k < 5; k++)
     if (copy_d(c, dst[k], a.ck_part[s] + k * N, sizeof(double) * N)) return WPIC_EHIP;
+  c->aperm = nullptr;
   if (copy_d(c, c->id[c->ida], a.ck_id[s], sizeof(uint64_t) * N)) return WPIC_EHIP;
   if (copy_d(c, c->rows[c->ra], a.ck_rows[s], sizeof(int) * 2 * a.nr)) return WPIC_EHIP;
   if (copy_d(c, c->cumcnt, a.ck_cumcnt[s], sizeof(int) * a.ncc)) return WPIC_EHIP;

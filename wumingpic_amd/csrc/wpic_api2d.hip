@@ -245,6 +245,11 @@ int sort_into_a(wpic_ctx* c) {
   const int dst = c->idb == c->ida ? 1 - c->ida : c->ida;
   c->ida = dst;
   c->idb = dst;
+  // the sorted state is gp's buffer seen through the new permutation; the
+  // old sorted buffer becomes the next gp
+  c->aperm = c->perm_out ? c->perm_out : c->permbuf;
+  std::swap(c->A, c->B);
+  std::swap(c->MA, c->MB);
   set_rows(c);
   c->b_valid = false;
   c->b_aligned = false;
@@ -365,7 +370,7 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
       if (e == hipSuccess) e = dalloc(q, N);
   }
   for (int k = 0; k < 2 && e == hipSuccess; k++) e = dalloc(&c->id[k], N);
-  int** ip[] = {&c->code, &c->R, &c->holes, &c->ulist, &c->dlist};
+  int** ip[] = {&c->code, &c->R, &c->holes, &c->ulist, &c->dlist, &c->permbuf};
   for (auto q : ip)
     if (e == hipSuccess) e = dalloc(q, N);
   for (int k = 0; k < 3 && e == hipSuccess; k++) {
@@ -392,6 +397,7 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
   if (e == hipSuccess) e = hipMemset(c->uj, 0, sizeof(double) * 3 * pl);
   if (e == hipSuccess) e = hipMemset(c->R, 0, sizeof(int) * N);
   if (e == hipSuccess) e = hipMemset(c->code, 0, sizeof(int) * N);
+  if (e == hipSuccess) e = hipMemset(c->permbuf, 0, sizeof(int) * N);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     wpic_destroy(c);
@@ -420,7 +426,7 @@ int wpic_destroy(wpic_ctx* c) {
                 c->rows[2], c->cumcnt, c->rowcnt, c->aosbase, c->uf, c->df, c->uj, c->pjbuf, c->phi, c->pa,
                 c->pb, c->rr, c->bb, c->ap, c->part, c->scal, c->err, c->staging, c->gbuf, c->red,
                 c->MA.ix, c->MA.ox, c->MA.oy, c->MA.ux, c->MA.uy, c->MA.uz,
-                c->MB.ix, c->MB.ox, c->MB.oy, c->MB.ux, c->MB.uy, c->MB.uz};
+                c->MB.ix, c->MB.ox, c->MB.oy, c->MB.ux, c->MB.uy, c->MB.uz, c->permbuf};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -537,6 +543,7 @@ int wpic_upload_up2d(wpic_ctx* c, const double* up, const int* np2, const int* c
         if (cr[i + 1] < cr[i]) return set_error(WPIC_EINVAL, "cumcnt must be non-decreasing");
     }
   }
+  c->aperm = nullptr;
   int rc = upload_rows(c, c->ra, up, np2, np, 0);
   if (rc) return rc;
   const size_t ncc = (size_t)c->g.nsp * c->g.ny * (c->g.nx + 1);
@@ -783,6 +790,7 @@ int wpic_ic_beams2d(wpic_ctx* c, unsigned long long seed, int n_ppc, double udri
     }
   HIPCHK(hipMemcpyAsync(c->rows[c->ra], hrows.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->cumcnt, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice, c->st));
+  c->aperm = nullptr;
   if (c->mixed)
     HIPCHK(launch_ic_beams_mixed(c, seed, n_ppc, udrift, uth_e, uth_i));
   else

@@ -453,7 +453,7 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   double** cg[] = {&c->phi, &c->pa, &c->pb, &c->rr, &c->bb, &c->ap};
   for (auto q : cg)
     if (e == hipSuccess) e = dalloc(q, 3 * nc);
-  if (e == hipSuccess) e = dalloc(&c->part, (size_t)3 * NPB * 2);
+  if (e == hipSuccess) e = dalloc(&c->part, (size_t)3 * NPB3 * 2);
   if (e == hipSuccess) e = dalloc(&c->scal, 3);
   if (e == hipSuccess) e = dalloc(&c->err, 1);
   if (e == hipSuccess) e = dalloc(&c->red, (size_t)256 * 3);

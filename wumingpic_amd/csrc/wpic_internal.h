@@ -143,9 +143,15 @@ struct wpic_ctx {
   int device;
 
   // particles: species s occupies [s*ncap, (s+1)*ncap); rows compact within
+  // A holds the sorted particles (the reference's up) as aperm[k] -> slot of
+  // A's buffer (null: identity); B holds gp, slot-aligned with A after the
+  // push.  The sort writes only the permutation and the ids, then A and B
+  // swap buffers: the records are read through aperm by the next push.
   wpic::Parts A, B;
   bool mixed;            // WPIC_MODE_MIXED context: MA/MB hold the particles
   wpic::PartsM MA, MB;
+  int* aperm;            // A's slot -> record in A's buffer, or null
+  int* permbuf;          // [N] the sort's permutation
   uint64_t* id[2];
   int ida, idb;
   int *code, *R, *holes, *ulist, *dlist;
@@ -225,6 +231,7 @@ hipError_t launch_planes_to_aos(wpic_ctx* c, const double* planes, double* aos, 
 hipError_t launch_ic_beams(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
                            double uth_i);
 hipError_t launch_fill(wpic_ctx* c, double* p, size_t n, double v);
+hipError_t launch_gather_rows(wpic_ctx* c, const Parts& src, const int* ap, const Parts& dst, const int* rows);
 // mixed-precision form
 hipError_t launch_push_deposit_mixed(wpic_ctx* c);
 hipError_t launch_up_to_mixed(wpic_ctx* c, const double* aos);

@@ -38,6 +38,9 @@ struct Parts3 {
 // (new x cell - nxgs) << 4 | dir.
 constexpr int DIR_STAY = 4;
 constexpr int CODE_SHIFT3 = 4;
+// CG partial-sum blocks in 3D: enough blocks to cover 256^3 cells with
+// few cells per thread; the partials are summed in a fixed order
+constexpr int NPB3 = 2048;
 
 }  // namespace wpic
 

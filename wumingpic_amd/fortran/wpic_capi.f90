@@ -1,10 +1,11 @@
-! wpic_capi.f90 -- ISO_C_BINDING interfaces to the C-ABI of include/wpic.h.
+! wpic_capi.f90 -- ISO_C_BINDING interfaces to the C-ABI of include/wpic.h
+! and include/wpic_adj.h.
 module wpic_capi
   use iso_c_binding
   implicit none
 
   integer(c_int), parameter :: WPIC_BC_PERIODIC = 0, WPIC_BC_WALLX = 1
-  integer(c_int), parameter :: WPIC_MODE_FAST = 0, WPIC_MODE_PARITY = 1
+  integer(c_int), parameter :: WPIC_MODE_FAST = 0, WPIC_MODE_PARITY = 1, WPIC_MODE_MIXED = 2
 
   type, bind(C) :: wpic_params2d
      integer(c_int) :: ndim, nsp
@@ -105,6 +106,57 @@ module wpic_capi
        type(c_ptr), value :: ctx
        integer(c_int), value :: nsteps
      end function wpic_step2d
+
+     integer(c_int) function wpic_energy2d(ctx, out) bind(C, name='wpic_energy2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out(3)
+     end function wpic_energy2d
+
+     integer(c_int) function wpic_ic_beams2d(ctx, seed, n_ppc, udrift, uth_e, uth_i, b0) &
+          bind(C, name='wpic_ic_beams2d')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       integer(c_long_long), value :: seed
+       integer(c_int), value :: n_ppc
+       real(c_double), value :: udrift, uth_e, uth_i, b0
+     end function wpic_ic_beams2d
+
+     ! adjoint (include/wpic_adj.h)
+     integer(c_int) function wpic_set_cg_tol2d(ctx, err) bind(C, name='wpic_set_cg_tol2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: err
+     end function wpic_set_cg_tol2d
+
+     integer(c_int) function wpic_perturb_bz2d(ctx, amp, mode) bind(C, name='wpic_perturb_bz2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: amp
+       integer(c_int), value :: mode
+     end function wpic_perturb_bz2d
+
+     integer(c_int) function wpic_adj_forward2d(ctx, nsteps, every, wb) bind(C, name='wpic_adj_forward2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nsteps, every
+       real(c_double), intent(out) :: wb
+     end function wpic_adj_forward2d
+
+     integer(c_int) function wpic_adj_backward2d(ctx, seed) bind(C, name='wpic_adj_backward2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: seed
+     end function wpic_adj_backward2d
+
+     integer(c_int) function wpic_adj_param_grads2d(ctx, udrift, uth_e, bz_mode, out) &
+          bind(C, name='wpic_adj_param_grads2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: udrift, uth_e
+       integer(c_int), value :: bz_mode
+       real(c_double), intent(out) :: out(3)
+     end function wpic_adj_param_grads2d
   end interface
 
 contains
