@@ -49,8 +49,9 @@ WORKLOADS = {
                         name="3D Weibel beams 256x256x256, 16 ppc/species, FP64 path"),
     "weibel3d64": dict(nx=64, ny=64, nz=64, n_ppc=16, name="3D Weibel beams 64x64x64, 16 ppc/species, FP64 path"),
 }
-# algorithmic bytes of one 3D particle through the push (read x,y,z,u and
-# write them: 12 doubles) and through the deposit (read old and new x,y,z)
+# algorithmic bytes of one 3D particle through the push or the fused fast
+# push+deposit (read x,y,z,u and write them: 12 doubles) and through the
+# separate deposit (read old and new x,y,z)
 PUSH3D_BYTES = 96
 DEPOSIT3D_BYTES = 48
 
@@ -194,11 +195,16 @@ def main3d(a, w):
     log(f"initial state ready: {w['name']}, {npart} particles")
     elapsed = timed_run(sim, a.warmup, a.steps)
     log(f"{a.warmup} warmup + {a.steps} timed steps: {elapsed:.3f} s")
-    ph = {name: sim.timing(name) for name in ("push", "deposit", "field", "cg", "exchange", "sort")}
+    names = ("push_deposit", "push", "deposit", "field", "cg", "exchange", "sort")
+    ph = {name: sim.timing(name) for name in names}
+    ph = {k: v for k, v in ph.items() if v[1] > 0}
     cg = sim.cg_iters()
     sim.close()
-    push_s = ph["push"][0] / max(ph["push"][1], 1) / 1e3
-    dep_s = ph["deposit"][0] / max(ph["deposit"][1], 1) / 1e3
+    rates = {}
+    for name, nbytes in (("push_deposit", PUSH3D_BYTES), ("push", PUSH3D_BYTES), ("deposit", DEPOSIT3D_BYTES)):
+        if name in ph:
+            sec = ph[name][0] / ph[name][1] / 1e3
+            rates[name + "_GBs"] = nbytes * npart / sec / 1e9
     out = {
         "metric": "particle-pushes/s (push+deposit+sort) per GPU, 3D",
         "value": npart * a.steps / elapsed, "unit": "particle-pushes/s", "n_gpus": 1,
@@ -207,8 +213,7 @@ def main3d(a, w):
         "data": "synthetic (seeded counter-based initial state, generated on the device)",
         "config": {"workload": w["name"], "particles": npart, "mode": a.mode},
         "phase_ms_per_step": {k: v[0] / a.steps for k, v in ph.items()},
-        "push_GBs": PUSH3D_BYTES * npart / push_s / 1e9,
-        "deposit_GBs": DEPOSIT3D_BYTES * npart / dep_s / 1e9,
+        **rates,
         "cg_iters_last_step": list(cg),
     }
     print(json.dumps(out))

@@ -207,6 +207,40 @@ def test_fast3d_one_step_particles_exact_fields_close():
     assert np.abs(uf_dev - uf).max() <= 1e-12 * np.abs(uf).max()
 
 
+def test_fast3d_cell_changers_beyond_the_tile_queue():
+    """One 8x4x4 tile with 48 ppc per species and electrons drifting at u = 3:
+    about 2,900 particles change cell in one step, more than the fused fast
+    kernel queues per tile in LDS (2,048), so the rest go through its global
+    overflow list.  Particles stay bit-exact, the current agrees to rounding."""
+    from wumingpic_amd import MODE_FAST
+    rng = np.random.default_rng(41)
+    q, r = S.weibel_consts(48)
+    p = po.make_params3d(8, 4, 4, np_cap=3 * 48 * 8, delt=0.5, q=q, r=r)
+    up, cc, np2 = S.sorted_particles3d(p, 48, rng, u_scale=(0.02, 0.2), drift=(0.0, 3.0), colocate=True)
+    uf = S.periodic_fill_fields3d(p, S.random_fields3d(p, rng, scale=0.02))
+    o = po.Oracle3D(p)
+    gp = np.zeros_like(up, order="F")
+    o.particle_solv(gp, up, uf, cc)
+    uf_ref = uf.copy(order="F")
+    o.field_fdtd_i(uf_ref, up, gp, cc)
+    moved = 0
+    for k in range(p.nz):
+        for j in range(p.ny):
+            n = np2[j, k, 1]
+            old = np.floor(up[0:3, :n, j, k, 1] / p.delx)
+            new = np.floor(gp[0:3, :n, j, k, 1] / p.delx)
+            moved += int((old != new).any(axis=0).sum())
+    assert moved > 2048 + 200
+    with _pic(p, MODE_FAST) as sim:
+        sim.upload_up(up, np2, cc)
+        sim.upload_fields(uf)
+        sim.step(1)
+        uj = sim.download_current()
+        uf_dev = sim.download_fields()
+    assert np.abs(uj - o.uj).max() <= 1e-13 * np.abs(o.uj).max()
+    assert np.abs(uf_dev - uf_ref).max() <= 1e-12 * np.abs(uf_ref).max()
+
+
 def test_weibel3d_fast_mode_close_and_conserving():
     from wumingpic_amd import MODE_FAST
     sim, p, up, uf, cc, np2 = _weibel3d_device_state(12, 8, 6, 4, MODE_FAST)

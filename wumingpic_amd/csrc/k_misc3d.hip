@@ -9,8 +9,9 @@ struct Xfer3Args {
   Parts3 P;
   uint64_t* id;
   const int *colbase, *np2;
-  const int* R;              // non-null: B seen through the exchange permutation
+  const int* R;              // non-null: records seen through a permutation (the exchange's R, or A's aperm)
   const long long* aosbase;  // [nsp*ncol] start of each column in the AoS buffer
+  bool id_direct;            // ids stored in slot order (A's ids, sorted by the sort itself)
 };
 
 __global__ void k_aos_to_soa3(Xfer3Args a, const double* __restrict__ aos) {
@@ -46,7 +47,7 @@ __global__ void k_soa_to_aos3(Xfer3Args a, double* __restrict__ aos) {
     d[3] = a.P.ux[s];
     d[4] = a.P.uy[s];
     d[5] = a.P.uz[s];
-    const uint64_t b = a.id[s];
+    const uint64_t b = a.id[a.id_direct ? base + q : s];
     __builtin_memcpy(&d[6], &b, 8);
   }
 }
@@ -88,6 +89,7 @@ hipError_t launch_up_to_soa3d(wpic_ctx3d* c, const double* aos, long long n, int
   a.np2 = which == 0 ? c->np2A : c->np2B;
   a.R = nullptr;
   a.aosbase = c->aosbase;
+  a.id_direct = true;
   hipLaunchKernelGGL(k_aos_to_soa3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, a, aos);
   return hipGetLastError();
 }
@@ -100,7 +102,7 @@ hipError_t launch_soa_to_aos3d(wpic_ctx3d* c, double* aos, int which) {
   if (which == 0) {
     a.colbase = c->colbaseA;
     a.np2 = c->np2A;
-    a.R = nullptr;
+    a.R = c->aperm;
   } else if (which == 1) {
     a.colbase = c->colbaseN;
     a.np2 = c->np2N;
@@ -111,6 +113,7 @@ hipError_t launch_soa_to_aos3d(wpic_ctx3d* c, double* aos, int which) {
     a.R = nullptr;
   }
   a.aosbase = c->aosbase;
+  a.id_direct = which == 0;
   hipLaunchKernelGGL(k_soa_to_aos3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, a, aos);
   return hipGetLastError();
 }
@@ -218,7 +221,7 @@ __device__ inline double bsum3(double v, double* sh) {
 }  // namespace
 
 __global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, const int* colbase, const int* np2,
-                                                 const double* uf, double* out) {
+                                                 const int* ap, const double* uf, double* out) {
   __shared__ double sh[4];
   double kin = 0.0, e2 = 0.0, b2 = 0.0;
   const int ncols = g.nsp * g.ncol;
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, const int* 
     const int s = rw / g.ncol;
     const int base = colbase[rw], n = np2[rw];
     for (int q = threadIdx.x; q < n; q += 256) {
-      const int p = base + q;
+      const int p = ap ? ap[base + q] : base + q;
       const double u2 = A.ux[p] * A.ux[p] + A.uy[p] * A.uy[p] + A.uz[p] * A.uz[p];
       kin = kin + g.r[s] * (sqrt(1 + u2 / (g.c * g.c)) - 1);
     }
@@ -251,18 +254,20 @@ __global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, const int* 
 }
 
 hipError_t launch_energy3d(wpic_ctx3d* c, double* dout) {
-  hipLaunchKernelGGL(k_energy3, dim3(RB3), dim3(256), 0, c->st, c->g, c->A, c->colbaseA, c->np2A, c->uf, dout);
+  hipLaunchKernelGGL(k_energy3, dim3(RB3), dim3(256), 0, c->st, c->g, c->A, c->colbaseA, c->np2A, c->aperm,
+                     c->uf, dout);
   return hipGetLastError();
 }
 
 // rho at cell centres with the quadratic shape, wrapped periodically, on the
 // interior array [nx*ny*nz]
-__global__ __launch_bounds__(256) void k_rho3(Grid3D g, Parts3 A, const int* colbase, const int* np2, double* rho) {
+__global__ __launch_bounds__(256) void k_rho3(Grid3D g, Parts3 A, const int* colbase, const int* np2, const int* ap,
+                                              double* rho) {
   const int rw = blockIdx.x;
   const int s = rw / g.ncol;
   const int base = colbase[rw], n = np2[rw];
   for (int q = threadIdx.x; q < n; q += 256) {
-    const int p = base + q;
+    const int p = ap ? ap[base + q] : base + q;
     const double x = A.x[p], y = A.y[p], z = A.z[p];
     const int i0 = (int)(x * g.d_delx), j0 = (int)(y * g.d_delx), k0 = (int)(z * g.d_delx);
     if (i0 < g.nxgs || i0 > g.nxge || j0 < g.nys || j0 > g.nye || k0 < g.nzs || k0 > g.nze) continue;
@@ -303,7 +308,7 @@ hipError_t launch_gauss3d(wpic_ctx3d* c, double* G, double* rho) {
   hipError_t e = launch_fill3d(c, rho, n, 0.0);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_rho3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, c->g, c->A, c->colbaseA,
-                     c->np2A, rho);
+                     c->np2A, c->aperm, rho);
   hipLaunchKernelGGL(k_gauss3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, c->g, c->uf, rho, G);
   return hipGetLastError();
 }

@@ -401,6 +401,57 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
   }
 }
 
+// Current of one particle as the full 5x5 Esirkepov block in the factored
+// form below, added to the LDS tile at cell (li, lj) with atomics (zeros
+// skipped): the path of a particle that changes cell.
+__device__ inline void deposit2_full(double (*acc)[ACW * ACH], int li, int lj, const double s0x[5], const double s0y[5],
+                                     const double dsx[5], const double dsy[5], double qk, double qvz) {
+  double cx = 0.0, cy = 0.0;
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    cx -= dsx[m];
+    cy -= dsy[m];
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      const double vx = cx * (qk * fma(0.5, dsy[n], s0y[n]));  // x face m - 1, y node n - 2
+      const double vy = cy * (qk * fma(0.5, dsx[n], s0x[n]));  // x node n - 2, y face m - 1
+      if (vx != 0.0) atomicAdd(&acc[0][(lj + n) * ACW + (li + 1 + m)], vx);
+      if (vy != 0.0) atomicAdd(&acc[1][(lj + 1 + m) * ACW + (li + n)], vy);
+    }
+  }
+#pragma unroll
+  for (int jp = 0; jp < 5; jp++) {
+    const double A = qvz * fma(0.5, dsy[jp], s0y[jp]), B = qvz * fma(1.0 / 3.0, dsy[jp], 0.5 * s0y[jp]);
+#pragma unroll
+    for (int ip = 0; ip < 5; ip++) {
+      double v = dsx[ip] * B;
+      if (ip >= 1 && ip <= 3) v = fma(s0x[ip], A, v);
+      if (v != 0.0) atomicAdd(&acc[2][(lj + jp) * ACW + (li + ip)], v);
+    }
+  }
+}
+
+// Push of one particle of cell (i, j) and, with reflecting walls, the
+// reflection that precedes the deposit in the reconnection order
+// (2d/proj/reconnection/app.f90:101-103; boundary_reconnection.f90:79-91).
+template <int BC>
+__device__ inline void push_reflect(const Grid2D& g, const double (*sf)[SFW * SFH], int li, int lj, int i, int j,
+                                    bool unit_c, const double* kc, double x, double y, double ux, double uy, double uz,
+                                    double& nxp, double& nyp, double& g3, double& g4, double& g5, double& gmv) {
+  boris_push(sf, li, lj, i, j, g.d_delx, g.delt, g.c, unit_c, kc[0], kc[1], kc[2], x, y, ux, uy, uz, nxp, nyp, g3, g4,
+             g5, gmv);
+  if (BC == WPIC_BC_WALLX) {
+    const int ipos = (int)(nxp / g.delx);
+    if (ipos < g.nxgs + 1) {
+      nxp = g.wall_lo - nxp;
+      g3 = -g3; g4 = -g4; g5 = -g5;
+    } else if (ipos >= g.nxge - 1) {
+      nxp = g.wall_hi - nxp;
+      g3 = -g3; g4 = -g4; g5 = -g5;
+    }
+  }
+}
+
 // Fast mode, push and deposit in one pass over each particle.  The push is the
 // shared bit-exact boris_push; the current is the same Esirkepov weights in a
 // factored form that reorders the reference's sums (so uj agrees to rounding,
@@ -409,13 +460,25 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
 //   Jy(ip, jp) += cy(jp) * wx(ip),  likewise with x and y exchanged
 //   Jz(ip, jp) += s0x(ip) * A(jp) + dsx(ip) * B(jp),
 //                 A = q vz (s0y + dsy/2),  B = q vz (s0y/2 + dsy/3)
-// which expands to field.f90:274-298 term by term.  With FUSED the wraps and
-// the exchange code follow in the same pass; otherwise gp keeps the unwrapped
-// positions as particle__solv leaves them.
+// which expands to field.f90:274-298 term by term.  A particle that stays in
+// its cell has ds = 0 on the outer nodes and cx = 0 on the outer faces (the
+// last up to rounding), so its current lives on 6 + 6 + 9 nodes: each lane
+// accumulates those 21 values in registers over the particles of the cell it
+// handles and the group reduces them once per cell.  A particle that changes
+// cell is queued in LDS and, after the tile's cells, pushed again (the same
+// bits) and added with its full block by all lanes (deposit2_full); beyond the
+// queue's capacity it is added at once.  With FUSED the wraps and the exchange
+// code follow in the same pass; otherwise gp keeps the unwrapped positions as
+// particle__solv leaves them.
+constexpr int NXQ2 = 3072;  // queued cell-changing particles per tile
+
 template <int G, bool FUSED, int BC>
 __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
   __shared__ double sf[6][SFW * SFH];
   __shared__ double acc[3][ACW * ACH];
+  __shared__ double spc[WPIC_MAXSP][5];  // fac1, txxx, fac2, qk, q
+  __shared__ int2 xq[NXQ2];              // slot, cell in the tile | species << 8
+  __shared__ int nxq;
 
   const Grid2D& g = a.g;
   const int ntiles = a.ntx * a.nty;
@@ -431,49 +494,52 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
     if (i <= g.nxge + 1 && j <= g.nye + 1) sf[c][rem] = cell_field(g, a.uf, c, i, j);
   }
   for (int k = tid; k < 3 * ACW * ACH; k += NT) (&acc[0][0])[k] = 0.0;
+  if (tid == 0) nxq = 0;
+  // per-species constants (particle.f90:92-94, field.f90:274)
+#pragma unroll
+  for (int s = 0; s < WPIC_MAXSP; s++) {
+    if (tid == s && s < g.nsp) {
+      const double q = g.q[s], rm = g.r[s];
+      const double fac1 = q / rm * 0.5 * g.delt;
+      spc[s][0] = fac1;
+      spc[s][1] = fac1 * fac1;
+      spc[s][2] = q * g.delt / rm;
+      spc[s][3] = q * g.delx * g.d_delt;
+      spc[s][4] = q;
+    }
+  }
   __syncthreads();
 
   const int grp = tid / G, lg = tid % G;
   constexpr int NGRP = NT / G;
-  const double d_delx = g.d_delx, delt = g.delt, c = g.c;
-  const bool unit_c = c * c == 1.0;
+  const bool unit_c = g.c * g.c == 1.0;
 
   for (int cc = grp; cc < TX * TY; cc += NGRP) {
     const int li = cc % TX, lj = cc / TX;
     const int i = i0 + li, j = j0 + lj;
     if (i > g.nxge || j > g.nye) continue;  // uniform within a group
-    double ax[20], ay[20], az[25];
+    // stayers: sx[m][u] Jx at x face m, y node u - 1; sy[m][u] Jy at y face
+    // m, x node u - 1; sz[v][u] Jz at x node u - 1, y node v - 1
+    double sx[2][3], sy[2][3], sz[3][3];
 #pragma unroll
-    for (int k = 0; k < 20; k++) ax[k] = ay[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < 25; k++) az[k] = 0.0;
+    for (int u = 0; u < 3; u++) {
+      sx[0][u] = sx[1][u] = sy[0][u] = sy[1][u] = 0.0;
+      sz[0][u] = sz[1][u] = sz[2][u] = 0.0;
+    }
 
     for (int s = 0; s < g.nsp; s++) {
       const int rw = s * g.ny + (j - g.nys);
       const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
       const int base = a.rowbase[rw];
       const int pbeg = base + cr[i - g.nxgs], pend = base + cr[i - g.nxgs + 1];
-      const double q = g.q[s], rm = g.r[s];
-      const double fac1 = q / rm * 0.5 * delt;
-      const double txxx = fac1 * fac1;
-      const double fac2 = q * delt / rm;
-      const double qk = q * g.delx * g.d_delt;
+      const double* kc = spc[s];
+      const double qk = kc[3], q = kc[4];
       for (int pi = pbeg + lg; pi < pend; pi += G) {
         const int ia = a.ap ? a.ap[pi] : pi;
         const double x = a.A.x[ia], y = a.A.y[ia];
         double nxp, nyp, g3, g4, g5, gmv;
-        boris_push(sf, li, lj, i, j, d_delx, delt, c, unit_c, fac1, txxx, fac2, x, y, a.A.ux[ia], a.A.uy[ia],
-                   a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
-        if (FUSED && BC == WPIC_BC_WALLX) {
-          const int ipos = (int)(nxp / g.delx);
-          if (ipos < g.nxgs + 1) {
-            nxp = g.wall_lo - nxp;
-            g3 = -g3; g4 = -g4; g5 = -g5;
-          } else if (ipos >= g.nxge - 1) {
-            nxp = g.wall_hi - nxp;
-            g3 = -g3; g4 = -g4; g5 = -g5;
-          }
-        }
+        push_reflect<FUSED ? BC : WPIC_BC_PERIODIC>(g, sf, li, lj, i, j, unit_c, kc, x, y, a.A.ux[ia], a.A.uy[ia],
+                                                    a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
         a.B.ux[pi] = g3;
         a.B.uy[pi] = g4;
         a.B.uz[pi] = g5;
@@ -481,33 +547,33 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
         double s0x[5], s0y[5], dsx[5], dsy[5];
         esirkepov_shapes(g, x, y, nxp, nyp, i, j, s0x, s0y, dsx, dsy);
         const double qvz = q * (g5 * gmv);
-        double cx = 0.0, cy = 0.0;
-        double wx[5], wy[5], A[5], B[5];
+        const int i2 = (int)(nxp * g.d_delx), j2 = (int)(nyp * g.d_delx);
+        if (i2 == i && j2 == j) {
+          const double cx1 = (0.0 - dsx[0]) - dsx[1], cx2 = cx1 - dsx[2];
+          const double cy1 = (0.0 - dsy[0]) - dsy[1], cy2 = cy1 - dsy[2];
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
-          const double ty = fma(0.5, dsy[k], s0y[k]);
-          wy[k] = qk * ty;
-          A[k] = qvz * ty;
-          B[k] = qvz * fma(1.0 / 3.0, dsy[k], 0.5 * s0y[k]);
-          wx[k] = qk * fma(0.5, dsx[k], s0x[k]);
-        }
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-          cx -= dsx[m];
-          cy -= dsy[m];
-#pragma unroll
-          for (int jp = 0; jp < 5; jp++) ax[jp * 4 + m] = fma(cx, wy[jp], ax[jp * 4 + m]);
-#pragma unroll
-          for (int ip = 0; ip < 5; ip++) ay[m * 5 + ip] = fma(cy, wx[ip], ay[m * 5 + ip]);
-        }
-#pragma unroll
-        for (int jp = 0; jp < 5; jp++)
-#pragma unroll
-          for (int ip = 0; ip < 5; ip++) {
-            double t = fma(dsx[ip], B[jp], az[jp * 5 + ip]);
-            if (ip >= 1 && ip <= 3) t = fma(s0x[ip], A[jp], t);
-            az[jp * 5 + ip] = t;
+          for (int u = 0; u < 3; u++) {
+            const double wy = qk * fma(0.5, dsy[u + 1], s0y[u + 1]);
+            const double wx = qk * fma(0.5, dsx[u + 1], s0x[u + 1]);
+            sx[0][u] = fma(cx1, wy, sx[0][u]);
+            sx[1][u] = fma(cx2, wy, sx[1][u]);
+            sy[0][u] = fma(cy1, wx, sy[0][u]);
+            sy[1][u] = fma(cy2, wx, sy[1][u]);
           }
+#pragma unroll
+          for (int v = 0; v < 3; v++) {
+            const double A = qvz * fma(0.5, dsy[v + 1], s0y[v + 1]);
+            const double B = qvz * fma(1.0 / 3.0, dsy[v + 1], 0.5 * s0y[v + 1]);
+#pragma unroll
+            for (int u = 0; u < 3; u++) sz[v][u] = fma(s0x[u + 1], A, fma(dsx[u + 1], B, sz[v][u]));
+          }
+        } else {
+          const int slot = atomicAdd(&nxq, 1);
+          if (slot < NXQ2)
+            xq[slot] = make_int2(pi, cc | (s << 8));
+          else
+            deposit2_full(acc, li, lj, s0x, s0y, dsx, dsy, qk, qvz);
+        }
 
         if (FUSED) {
           const int code = wrap_code<BC>(g, nxp, nyp, j, a.err);
@@ -517,10 +583,44 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
         a.B.y[pi] = nyp;
       }
     }
-    const int cellidx = cidx(g, i, j);
-    flush_block<25, 40, G, false>(az, a.pjbuf, acc, cellidx, li, lj, lg);
-    flush_block<20, 0, G, false>(ax, a.pjbuf, acc, cellidx, li, lj, lg);
-    flush_block<20, 20, G, false>(ay, a.pjbuf, acc, cellidx, li, lj, lg);
+    // reduce the stayers' 21 values over the group, spread the LDS atomics
+#pragma unroll
+    for (int q = 0; q < 21; q++) {
+      double v = q < 6 ? sx[q / 3][q % 3] : (q < 12 ? sy[(q - 6) / 3][(q - 6) % 3] : sz[(q - 12) / 3][(q - 12) % 3]);
+#pragma unroll
+      for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, G);
+      if ((q % G) == lg && v != 0.0) {
+        int comp, xo, yo;
+        if (q < 6) {
+          comp = 0; xo = q / 3; yo = q % 3 - 1;
+        } else if (q < 12) {
+          comp = 1; yo = (q - 6) / 3; xo = (q - 6) % 3 - 1;
+        } else {
+          comp = 2; yo = (q - 12) / 3 - 1; xo = (q - 12) % 3 - 1;
+        }
+        atomicAdd(&acc[comp][(lj + 2 + yo) * ACW + (li + 2 + xo)], v);
+      }
+    }
+  }
+
+  __syncthreads();
+  // the queued particles that changed cell: the push again (the same bits, its
+  // stores already made) and the full block
+  const int nq = nxq < NXQ2 ? nxq : NXQ2;
+  for (int k = tid; k < nq; k += NT) {
+    const int2 e = xq[k];
+    const int pi = e.x, cc = e.y & 255, s = e.y >> 8;
+    const int li = cc % TX, lj = cc / TX;
+    const int i = i0 + li, j = j0 + lj;
+    const double* kc = spc[s];
+    const int ia = a.ap ? a.ap[pi] : pi;
+    const double x = a.A.x[ia], y = a.A.y[ia];
+    double nxp, nyp, g3, g4, g5, gmv;
+    push_reflect<FUSED ? BC : WPIC_BC_PERIODIC>(g, sf, li, lj, i, j, unit_c, kc, x, y, a.A.ux[ia], a.A.uy[ia],
+                                                a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
+    double s0x[5], s0y[5], dsx[5], dsy[5];
+    esirkepov_shapes(g, x, y, nxp, nyp, i, j, s0x, s0y, dsx, dsy);
+    deposit2_full(acc, li, lj, s0x, s0y, dsx, dsy, kc[3], kc[4] * (g5 * gmv));
   }
 
   __syncthreads();

@@ -12,7 +12,9 @@
 // compute the old/new shape arrays s0 and ds of up to 16 particles into LDS;
 // then lane (jp, kp) (25 of the 32) walks those particles in slot order and
 // accumulates the cell's pjx(:, jp, kp), pjy(:, jp, kp), pjz(:, jp, kp) with
-// the reference's dstmp / running pjtmp arithmetic (field.f90:335-380).
+// the reference's dstmp / running pjtmp arithmetic (field.f90:335-380) in
+// parity mode, or in a factored form with FMA (the running sums of ds
+// precomputed per particle) in fast mode.
 // Parity mode writes the 300-entry block per cell; k_gather_pj3d then adds the
 // blocks into uj in the reference's (k, j, i) cell order with the scatter
 // permutation uj(2) += pjy(jp, ip, kp), uj(3) += pjz(kp, ip, jp) (:390-398),
@@ -23,11 +25,11 @@
 namespace wpic {
 
 namespace p3 {
-constexpr int TX = 8, TY = 4, TZ = 4, NT = 256, G = 4;
+constexpr int TX = 8, TY = 4, TZ = 4, NT = 256, G = 16;
 constexpr int SW = TX + 2, SH = TY + 2, SD = TZ + 2, SN = SW * SH * SD;  // staged tmpf
 constexpr int AW = TX + 4, AH = TY + 4, AD = TZ + 4, AN = AW * AH * AD;  // current tile
 constexpr int NB = 16;   // particles per shape batch of a half-wave
-constexpr int NSV = 30;  // s0x s0y s0z dsx dsy dsz, 5 each
+constexpr int NSV = 42;  // s0x s0y s0z dsx dsy dsz (5 each); fast mode adds cx cy cz (4 each)
 }  // namespace p3
 
 struct Push3Args {
@@ -35,6 +37,9 @@ struct Push3Args {
   Parts3 A, B;
   const int* colbase;
   const int* cumcnt;
+  const int* ap;  // A's slot -> record in A's buffer (null: identity)
+  int* nov;       // fused fast kernel: count of the overflow list below
+  int *ovslot, *ovcell;
   const double* uf;
   double* uj;
   double* pjbuf;
@@ -62,6 +67,58 @@ __device__ inline void tile_origin(const Grid3D& g, int tile, int ntx, int nty, 
   i0 = g.nxgs + tx * p3::TX;
   j0 = g.nys + (rest % nty) * p3::TY;
   k0 = g.nzs + (rest / nty) * p3::TZ;
+}
+
+// Buneman-Boris push of one particle of cell (i, j, k) (particle.f90:93-217)
+// from the staged tmpf; so is the LDS offset of the cell.  sh returns the
+// particle's old shape (x, y, z triplets), which the deposit reuses.  Shared by
+// k_push3d and the fused fast kernel, so both give the same gp bit for bit.
+__device__ inline void boris_push3(const double (*sf)[p3::SN], int so, int i, int j, int k, double d_delx, double delt,
+                                   double c, double fac1, double txxx, double fac2, double x, double y, double z,
+                                   double ux, double uy, double uz, double& nxp, double& nyp, double& nzp, double& g4,
+                                   double& g5, double& g6, double sh[9]) {
+  using namespace p3;
+  double shxm, shx, shxp, shym, shy, shyp, shzm, shz, shzp;
+  shape3(x * d_delx - 5e-1 - i, shxm, shx, shxp);
+  shape3(y * d_delx - 5e-1 - j, shym, shy, shyp);
+  shape3(z * d_delx - 5e-1 - k, shzm, shz, shzp);
+  // re-read the cell's staged fields from LDS for every particle instead of
+  // holding all 162 in registers across the particle loop
+  asm volatile("" : "+v"(so));
+  double fl[6];
+#pragma unroll
+  for (int cm = 0; cm < 6; cm++) {
+    const double* f = &sf[cm][so];
+#define RX(o) (+f[(o)-1] * shxm + f[o] * shx + f[(o) + 1] * shxp)
+#define RY(o) (+RX((o)-SW) * shym + RX(o) * shy + RX((o) + SW) * shyp)
+    fl[cm] = RY(-SW * SH) * shzm + RY(0) * shz + RY(SW * SH) * shzp;
+#undef RY
+#undef RX
+  }
+  const double bpx = fl[0], bpy = fl[1], bpz = fl[2], epx = fl[3], epy = fl[4], epz = fl[5];
+  double uvm1 = ux + fac1 * epx;
+  double uvm2 = uy + fac1 * epy;
+  double uvm3 = uz + fac1 * epz;
+  double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+  const double igam = 1e0 / gam;
+  const double fac1r = fac1 * igam;
+  const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+  const double uvm4 = uvm1 + fac1r * (+uvm2 * bpz - uvm3 * bpy);
+  const double uvm5 = uvm2 + fac1r * (+uvm3 * bpx - uvm1 * bpz);
+  const double uvm6 = uvm3 + fac1r * (+uvm1 * bpy - uvm2 * bpx);
+  uvm1 = uvm1 + fac2r * (+uvm5 * bpz - uvm6 * bpy);
+  uvm2 = uvm2 + fac2r * (+uvm6 * bpx - uvm4 * bpz);
+  uvm3 = uvm3 + fac2r * (+uvm4 * bpy - uvm5 * bpx);
+  g4 = uvm1 + fac1 * epx;
+  g5 = uvm2 + fac1 * epy;
+  g6 = uvm3 + fac1 * epz;
+  gam = 1e0 / sqrt(1e0 + (+g4 * g4 + g5 * g5 + g6 * g6) / (c * c));
+  nxp = x + g4 * delt * gam;
+  nyp = y + g5 * delt * gam;
+  nzp = z + g6 * delt * gam;
+  sh[0] = shxm; sh[1] = shx; sh[2] = shxp;
+  sh[3] = shym; sh[4] = shy; sh[5] = shyp;
+  sh[6] = shzm; sh[7] = shz; sh[8] = shzp;
 }
 
 __global__ __launch_bounds__(p3::NT) void k_push3d(Push3Args a) {
@@ -99,44 +156,14 @@ __global__ __launch_bounds__(p3::NT) void k_push3d(Push3Args a) {
       const double txxx = fac1 * fac1;
       const double fac2 = q * delt / rm;
       for (int pi = pbeg + lg; pi < pend; pi += G) {
-        const double x = a.A.x[pi], y = a.A.y[pi], z = a.A.z[pi];
-        double shxm, shx, shxp, shym, shy, shyp, shzm, shz, shzp;
-        shape3(x * d_delx - 5e-1 - i, shxm, shx, shxp);
-        shape3(y * d_delx - 5e-1 - j, shym, shy, shyp);
-        shape3(z * d_delx - 5e-1 - k, shzm, shz, shzp);
-        int so = (lk + 1) * (SW * SH) + (lj + 1) * SW + (li + 1);
-        asm volatile("" : "+v"(so));
-        double fl[6];
-#pragma unroll
-        for (int cm = 0; cm < 6; cm++) {
-          const double* f = &sf[cm][so];
-#define RX(o) (+f[(o)-1] * shxm + f[o] * shx + f[(o) + 1] * shxp)
-#define RY(o) (+RX((o)-SW) * shym + RX(o) * shy + RX((o) + SW) * shyp)
-          fl[cm] = RY(-SW * SH) * shzm + RY(0) * shz + RY(SW * SH) * shzp;
-#undef RY
-#undef RX
-        }
-        const double bpx = fl[0], bpy = fl[1], bpz = fl[2], epx = fl[3], epy = fl[4], epz = fl[5];
-        double uvm1 = a.A.ux[pi] + fac1 * epx;
-        double uvm2 = a.A.uy[pi] + fac1 * epy;
-        double uvm3 = a.A.uz[pi] + fac1 * epz;
-        double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
-        const double igam = 1e0 / gam;
-        const double fac1r = fac1 * igam;
-        const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
-        const double uvm4 = uvm1 + fac1r * (+uvm2 * bpz - uvm3 * bpy);
-        const double uvm5 = uvm2 + fac1r * (+uvm3 * bpx - uvm1 * bpz);
-        const double uvm6 = uvm3 + fac1r * (+uvm1 * bpy - uvm2 * bpx);
-        uvm1 = uvm1 + fac2r * (+uvm5 * bpz - uvm6 * bpy);
-        uvm2 = uvm2 + fac2r * (+uvm6 * bpx - uvm4 * bpz);
-        uvm3 = uvm3 + fac2r * (+uvm4 * bpy - uvm5 * bpx);
-        const double g4 = uvm1 + fac1 * epx;
-        const double g5 = uvm2 + fac1 * epy;
-        const double g6 = uvm3 + fac1 * epz;
-        gam = 1e0 / sqrt(1e0 + (+g4 * g4 + g5 * g5 + g6 * g6) / (c * c));
-        a.B.x[pi] = x + g4 * delt * gam;
-        a.B.y[pi] = y + g5 * delt * gam;
-        a.B.z[pi] = z + g6 * delt * gam;
+        const int ia = a.ap ? a.ap[pi] : pi;
+        const double x = a.A.x[ia], y = a.A.y[ia], z = a.A.z[ia];
+        double nxp, nyp, nzp, g4, g5, g6, sh[9];
+        boris_push3(sf, (lk + 1) * (SW * SH) + (lj + 1) * SW + (li + 1), i, j, k, d_delx, delt, c, fac1, txxx, fac2,
+                    x, y, z, a.A.ux[ia], a.A.uy[ia], a.A.uz[ia], nxp, nyp, nzp, g4, g5, g6, sh);
+        a.B.x[pi] = nxp;
+        a.B.y[pi] = nyp;
+        a.B.z[pi] = nzp;
         a.B.ux[pi] = g4;
         a.B.uy[pi] = g5;
         a.B.uz[pi] = g6;
@@ -221,18 +248,51 @@ __global__ __launch_bounds__(p3::NT) void k_deposit3d(Push3Args a) {
         const int nb = pend - b0 < NB ? pend - b0 : NB;
         if (hl < nb) {
           const int pi = b0 + hl;
-          const double old[3] = {a.A.x[pi], a.A.y[pi], a.A.z[pi]};
+          const int ia = a.ap ? a.ap[pi] : pi;
+          const double old[3] = {a.A.x[ia], a.A.y[ia], a.A.z[ia]};
           const double nw[3] = {a.B.x[pi], a.B.y[pi], a.B.z[pi]};
           double v[NSV];
           shapes3(g, old, nw, cell, v);
+          if (!PARITY) {
+            // running sums c(m) = -sum_{k<=m} ds(k) of the factored form
 #pragma unroll
-          for (int q = 0; q < NSV; q++) sh[hl][q] = v[q];
+            for (int d = 0; d < 3; d++) {
+              double run = 0.0;
+#pragma unroll
+              for (int m = 0; m < 4; m++) {
+                run -= v[15 + 5 * d + m];
+                v[30 + 4 * d + m] = run;
+              }
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < (PARITY ? 30 : NSV); q++) sh[hl][q] = v[q];
         }
         // the shapes were written by other lanes of this wave
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (accl) {
+        if (accl && !PARITY) {
+          // fast mode: pj(m-1, jp, kp) += c(m) * w(jp, kp), the dstmp of the
+          // reference written as w = (s0 + ds/2) s0' + (s0/2 + ds/3) ds' (FMA)
+          for (int p = 0; p < nb; p++) {
+            const double* v = sh[p];
+            const double *s0x = v, *s0y = v + 5, *s0z = v + 10, *dsx = v + 15, *dsy = v + 20, *dsz = v + 25;
+            const double *cxs = v + 30, *cys = v + 34, *czs = v + 38;
+            const double axj = fma(0.5, dsx[ja], s0x[ja]), bxj = fma(fac, dsx[ja], 0.5 * s0x[ja]);
+            const double ayj = fma(0.5, dsy[ja], s0y[ja]), byj = fma(fac, dsy[ja], 0.5 * s0y[ja]);
+            const double wx = fma(ayj, s0z[kb], byj * dsz[kb]) * qdxdt;
+            const double wy = fma(axj, s0z[kb], bxj * dsz[kb]) * qdxdt;
+            const double wz = fma(axj, s0y[kb], bxj * dsy[kb]) * qdxdt;
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+              px[m] = fma(cxs[m], wx, px[m]);
+              py[m] = fma(cys[m], wy, py[m]);
+              pz[m] = fma(czs[m], wz, pz[m]);
+            }
+          }
+        }
+        if (accl && PARITY) {
           for (int p = 0; p < nb; p++) {
             const double* v = sh[p];
             const double *s0x = v, *s0y = v + 5, *s0z = v + 10, *dsx = v + 15, *dsy = v + 20, *dsz = v + 25;
@@ -339,6 +399,337 @@ __global__ __launch_bounds__(256) void k_gather_pj3d(Grid3D g, const double* __r
   uj[2 * n + q] = sz;
 }
 
+// Current of one particle that changes cell (field.f90:253-380, factored as
+// below): the 4-node window [lo, lo+3] (of the 5 nodes -2..2) per axis holds
+// every non-zero s0 and ds when |inc| <= 1; smo_* as field.f90:258-266.
+// add(comp, xo, yo, zo, v) receives each non-zero value with its node offset
+// from the particle's cell (i, j, k).
+template <class Add>
+__device__ inline void deposit3_window(const Grid3D& g, const Parts3& A, const Parts3& B, const int* ap, int pc,
+                                       const int cell[3], double qd, Add add) {
+  const int ia = ap ? ap[pc] : pc;
+  const double old[3] = {A.x[ia], A.y[ia], A.z[ia]};
+  const double nw[3] = {B.x[pc], B.y[pc], B.z[pc]};
+  const double d_delx = g.d_delx;
+  double S0[3][4], DS[3][4], cq[3][3];
+  int lo[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double s0m, s00, s0p, s11, s12, s13;
+    shape3(old[d] * d_delx - 5e-1 - cell[d], s0m, s00, s0p);
+    const int i2 = (int)(nw[d] * d_delx);
+    const double dh = nw[d] * d_delx - 5e-1 - i2;
+    const int in = i2 - cell[d];
+    shape3(dh, s11, s12, s13);
+    const double smo1 = in < 0 ? (double)(-in) : 0.0;
+    const double smo2 = (double)(-abs(in) + 1);
+    const double smo3 = in > 0 ? (double)in : 0.0;
+    const double f0 = s11 * smo1 - 0e0;
+    const double f1 = s11 * smo2 + s12 * smo1 - s0m;
+    const double f2 = s12 * smo2 + s13 * smo1 + s11 * smo3 - s00;
+    const double f3 = s13 * smo2 + s12 * smo3 - s0p;
+    const double f4 = s13 * smo3 - 0e0;
+    const bool left = in < 0;
+    lo[d] = left ? 0 : 1;
+    S0[d][0] = left ? 0.0 : s0m;
+    S0[d][1] = left ? s0m : s00;
+    S0[d][2] = left ? s00 : s0p;
+    S0[d][3] = left ? s0p : 0.0;
+    DS[d][0] = left ? f0 : f1;
+    DS[d][1] = left ? f1 : f2;
+    DS[d][2] = left ? f2 : f3;
+    DS[d][3] = left ? f3 : f4;
+    double run = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      run = run - DS[d][m];
+      cq[d][m] = run * qd;
+    }
+  }
+  // window node u of an axis is node lo + u - 2 of the cell; face m lies
+  // between window nodes m and m + 1, at offset lo + m - 1
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const double ax = fma(0.5, DS[0][u], S0[0][u]), bxu = fma(1.0 / 3.0, DS[0][u], 0.5 * S0[0][u]);
+    const double ay = fma(0.5, DS[1][u], S0[1][u]), byu = fma(1.0 / 3.0, DS[1][u], 0.5 * S0[1][u]);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const double wx = fma(ay, S0[2][v], byu * DS[2][v]);  // y node u, z node v
+      const double wy = fma(ax, S0[2][v], bxu * DS[2][v]);  // x node u, z node v
+      const double wz = fma(ax, S0[1][v], bxu * DS[1][v]);  // x node u, y node v
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        const double vx = cq[0][m] * wx, vy = cq[1][m] * wy, vz = cq[2][m] * wz;
+        if (vx != 0.0) add(0, lo[0] + m - 1, lo[1] + u - 2, lo[2] + v - 2, vx);
+        if (vy != 0.0) add(1, lo[0] + u - 2, lo[1] + m - 1, lo[2] + v - 2, vy);
+        if (vz != 0.0) add(2, lo[0] + u - 2, lo[1] + v - 2, lo[2] + m - 1, vz);
+      }
+    }
+  }
+}
+
+// Fast mode, push and deposit in one pass over each particle.  The push is the
+// shared bit-exact boris_push3; the deposit uses the factored weights of
+// k_deposit3d's fast path,
+//   Jx(m, j, k) += cx(m) (ay(j) s0z(k) + by(j) dsz(k)),  cx(m) = -sum_{n<=m} dsx(n),
+//   ay = s0y + dsy/2,  by = s0y/2 + dsy/3,
+// and Jy, Jz with the axes permuted (field.f90:335-380 expanded term by term).
+// A particle that stays in its cell along every axis has ds = 0 on the outer
+// nodes and cx = 0 on the outer faces (the last one up to rounding), so its
+// current lives on 2 x 3 x 3 nodes per component: every lane accumulates
+// those 54 values in registers over the particles of the cell it handles and
+// the lane group reduces them once per cell.  A particle that changes cell is
+// queued in LDS; after the tile's cells, all lanes add the queued particles'
+// 3 x 4 x 4 windows per component (deposit3_window) to the LDS tile.  The
+// tile is added to uj with one HBM atomic per node.
+namespace p3f {
+constexpr int G = 8;       // lanes per cell
+constexpr int NXQ = 2048;  // queued cell-changing particles per tile (beyond: a global list)
+}
+
+__global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) {
+  using namespace p3;
+  constexpr int GF = p3f::G;
+  __shared__ double sf[6][SN];
+  __shared__ double acc[3][AN];
+  __shared__ double spc[WPIC_MAXSP][4];  // fac1, txxx, fac2, q delx / delt
+  __shared__ int2 xq[p3f::NXQ];  // slot, cell in the tile | species << 8
+  __shared__ int nxq;
+  using p3f::NXQ;
+  const Grid3D& g = a.g;
+  const int ntiles = a.ntx * a.nty * a.ntz;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  int i0, j0, k0;
+  tile_origin(g, tile, a.ntx, a.nty, i0, j0, k0);
+  const int tid = threadIdx.x;
+  for (int q = tid; q < 6 * SN; q += NT) {
+    const int c = q / SN, rem = q - c * SN;
+    const int i = i0 - 1 + rem % SW, j = j0 - 1 + (rem / SW) % SH, k = k0 - 1 + rem / (SW * SH);
+    if (i <= g.nxge + 1 && j <= g.nye + 1 && k <= g.nze + 1) sf[c][rem] = tmpf3(g, a.uf, c, i, j, k);
+  }
+  for (int q = tid; q < 3 * AN; q += NT) (&acc[0][0])[q] = 0.0;
+  if (tid == 0) nxq = 0;
+#pragma unroll
+  for (int s = 0; s < WPIC_MAXSP; s++) {
+    if (tid == s && s < g.nsp) {
+      const double q = g.q[s], rm = g.r[s];
+      const double fac1 = q / rm * 5e-1 * g.delt;
+      spc[s][0] = fac1;
+      spc[s][1] = fac1 * fac1;
+      spc[s][2] = q * g.delt / rm;
+      spc[s][3] = q * g.delx * g.d_delt;
+    }
+  }
+  __syncthreads();
+
+  const int grp = tid / GF, lg = tid % GF;
+  constexpr int NGRP = NT / GF;
+  const double d_delx = g.d_delx, delt = g.delt, c = g.c;
+  for (int cc = grp; cc < TX * TY * TZ; cc += NGRP) {
+    const int li = cc % TX, lj = (cc / TX) % TY, lk = cc / (TX * TY);
+    const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+    if (i > g.nxge || j > g.nye || k > g.nze) continue;  // uniform within a group
+    const int col = (j - g.nys) + g.ny * (k - g.nzs);
+    CellList cl;
+    int ntot = 0;
+#pragma unroll
+    for (int s = 0; s < WPIC_MAXSP; s++) {
+      if (s < g.nsp) {
+        const int rw = s * g.ncol + col;
+        const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
+        const int c0 = cr[i - g.nxgs], c1 = cr[i - g.nxgs + 1];
+        cl.beg[s] = a.colbase[rw] + c0;
+        cl.cum[s] = ntot;
+        ntot += c1 - c0;
+      } else {
+        cl.beg[s] = 0;
+        cl.cum[s] = 0x7fffffff;
+      }
+    }
+    // stayers' current: [m][a][b] with m the face (offset m along the
+    // component's axis) and a, b the nodes (offsets a-1, b-1) of the other two
+    // axes in the order (y, z) for Jx, (x, z) for Jy, (x, y) for Jz
+    double jx[18], jy[18], jz[18];
+#pragma unroll
+    for (int q = 0; q < 18; q++) jx[q] = jy[q] = jz[q] = 0.0;
+    const int so = (lk + 1) * (SW * SH) + (lj + 1) * SW + (li + 1);
+    const int bx = li + 2, by = lj + 2, bz = lk + 2;  // the cell in the current tile
+
+    // software pipeline: the record of the next particle and the slot of the
+    // one after are loaded while the current particle is computed
+    int t = lg, ian = 0;
+    double x = 0.0, y = 0.0, z = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
+    if (t < ntot) {
+      int s0, p0;
+      cl.slot(t, s0, p0);
+      const int ia = a.ap ? a.ap[p0] : p0;
+      x = a.A.x[ia];
+      y = a.A.y[ia];
+      z = a.A.z[ia];
+      ux = a.A.ux[ia];
+      uy = a.A.uy[ia];
+      uz = a.A.uz[ia];
+    }
+    if (t + GF < ntot) {
+      int s1, p1;
+      cl.slot(t + GF, s1, p1);
+      ian = a.ap ? a.ap[p1] : p1;
+    }
+    for (; t < ntot; t += GF) {
+      double xn = 0.0, yn = 0.0, zn = 0.0, uxn = 0.0, uyn = 0.0, uzn = 0.0;
+      if (t + GF < ntot) {
+        xn = a.A.x[ian];
+        yn = a.A.y[ian];
+        zn = a.A.z[ian];
+        uxn = a.A.ux[ian];
+        uyn = a.A.uy[ian];
+        uzn = a.A.uz[ian];
+      }
+      int iann = 0;
+      if (t + 2 * GF < ntot) {
+        int s2, p2;
+        cl.slot(t + 2 * GF, s2, p2);
+        iann = a.ap ? a.ap[p2] : p2;
+      }
+      int sc, pc;
+      cl.slot(t, sc, pc);
+      const double* kc = spc[sc];
+      double nw[3], g4, g5, g6, sh[9];
+      boris_push3(sf, so, i, j, k, d_delx, delt, c, kc[0], kc[1], kc[2], x, y, z, ux, uy, uz, nw[0], nw[1], nw[2], g4,
+                  g5, g6, sh);
+      a.B.x[pc] = nw[0];
+      a.B.y[pc] = nw[1];
+      a.B.z[pc] = nw[2];
+      a.B.ux[pc] = g4;
+      a.B.uy[pc] = g5;
+      a.B.uz[pc] = g6;
+
+      // new shape and cell increment per axis (field.f90:253-329)
+      const double qd = kc[3];
+      const int cell[3] = {i, j, k};
+      double s1[3][3];
+      int inc[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const int i2 = (int)(nw[d] * d_delx);
+        const double dh = nw[d] * d_delx - 5e-1 - i2;
+        inc[d] = i2 - cell[d];
+        shape3(dh, s1[d][0], s1[d][1], s1[d][2]);
+      }
+      if ((inc[0] | inc[1] | inc[2]) == 0) {
+        // nodes -1..1 of each axis: s0, ds = s1 - s0, a = s0 + ds/2, b = s0/2 + ds/3
+        double s0[3][3], ds[3][3], A[3][3], B[3][3], cq[3][2];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+#pragma unroll
+          for (int n = 0; n < 3; n++) {
+            s0[d][n] = sh[3 * d + n];
+            ds[d][n] = s1[d][n] - s0[d][n];
+            A[d][n] = fma(0.5, ds[d][n], s0[d][n]);
+            B[d][n] = fma(1.0 / 3.0, ds[d][n], 0.5 * s0[d][n]);
+          }
+          cq[d][0] = (0.0 - ds[d][0]) * qd;
+          cq[d][1] = ((0.0 - ds[d][0]) - ds[d][1]) * qd;
+        }
+#pragma unroll
+        for (int u = 0; u < 3; u++)
+#pragma unroll
+          for (int v = 0; v < 3; v++) {
+            const double wx = fma(A[1][u], s0[2][v], B[1][u] * ds[2][v]);  // y node u, z node v
+            const double wy = fma(A[0][u], s0[2][v], B[0][u] * ds[2][v]);  // x node u, z node v
+            const double wz = fma(A[0][u], s0[1][v], B[0][u] * ds[1][v]);  // x node u, y node v
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+              const int q = (m * 3 + u) * 3 + v;
+              jx[q] = fma(cq[0][m], wx, jx[q]);
+              jy[q] = fma(cq[1][m], wy, jy[q]);
+              jz[q] = fma(cq[2][m], wz, jz[q]);
+            }
+          }
+      } else {
+        // a particle that changes cell: its window is added after the cell loop
+        const int slot = atomicAdd(&nxq, 1);
+        if (slot < NXQ) {
+          xq[slot] = make_int2(pc, cc | (sc << 8));
+        } else {
+          // beyond the LDS queue: listed for k_deposit3d_overflow
+          const int o = atomicAdd(a.nov, 1);
+          a.ovslot[o] = pc;
+          a.ovcell[o] = (int)cidx3(g, i, j, k) | (sc << 28);
+        }
+      }
+
+      x = xn; y = yn; z = zn; ux = uxn; uy = uyn; uz = uzn;
+      ian = iann;
+    }
+    // reduce the stayers' current over the group, spread the LDS atomics
+#pragma unroll
+    for (int q = 0; q < 54; q++) {
+      double v = q < 18 ? jx[q] : (q < 36 ? jy[q - 18] : jz[q - 36]);
+#pragma unroll
+      for (int off = GF / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, GF);
+      if ((q % GF) == lg && v != 0.0) {
+        const int comp = q / 18, r = q % 18, m = r / 9, u = (r / 3) % 3, w = r % 3;
+        int xo, yo, zo;
+        if (comp == 0) {
+          xo = m; yo = u - 1; zo = w - 1;
+        } else if (comp == 1) {
+          xo = u - 1; yo = m; zo = w - 1;
+        } else {
+          xo = u - 1; yo = w - 1; zo = m;
+        }
+        atomicAdd(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
+      }
+    }
+  }
+
+  __syncthreads();
+  // the queued particles that changed cell
+  const int nq = nxq < NXQ ? nxq : NXQ;
+  for (int q = tid; q < nq; q += NT) {
+    const int2 e = xq[q];
+    const int cc = e.y & 255, s = e.y >> 8;
+    const int li = cc % TX, lj = (cc / TX) % TY, lk = cc / (TX * TY);
+    const int cell[3] = {i0 + li, j0 + lj, k0 + lk};
+    const int bx = li + 2, by = lj + 2, bz = lk + 2;
+    deposit3_window(g, a.A, a.B, a.ap, e.x, cell, spc[s][3], [&](int comp, int xo, int yo, int zo, double v) {
+      atomicAdd(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
+    });
+  }
+  __syncthreads();
+  const size_t pl = plane3(g);
+  for (int q = tid; q < 3 * AN; q += NT) {
+    const int comp = q / AN, rem = q - comp * AN;
+    const double v = acc[comp][rem];
+    if (v == 0.0) continue;
+    const int I = i0 - 2 + rem % AW, J = j0 - 2 + (rem / AW) % AH, K = k0 - 2 + rem / (AW * AH);
+    if (I > g.nxge + 2 || J > g.nye + 2 || K > g.nze + 2) continue;
+    unsafeAtomicAdd(a.uj + pl * comp + fidx3(g, I, J, K), v);
+  }
+}
+
+// The cell-changing particles that did not fit a tile's LDS queue, added to
+// uj with HBM atomics (ghost nodes included; the current boundary folds them).
+__global__ __launch_bounds__(256) void k_deposit3d_overflow(Push3Args a) {
+  const Grid3D& g = a.g;
+  const int n = *a.nov;
+  const size_t pl = plane3(g);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int pc = a.ovslot[q], pk = a.ovcell[q];
+    const int s = pk >> 28, cix = pk & ((1 << 28) - 1);
+    const int cell[3] = {g.nxgs + cix % g.nx, g.nys + (cix / g.nx) % g.ny, g.nzs + cix / (g.nx * g.ny)};
+    double qs = g.q[0];
+#pragma unroll
+    for (int k = 1; k < WPIC_MAXSP; k++)
+      if (s == k) qs = g.q[k];
+    const double qd = qs * g.delx * g.d_delt;
+    deposit3_window(g, a.A, a.B, a.ap, pc, cell, qd, [&](int comp, int xo, int yo, int zo, double v) {
+      unsafeAtomicAdd(a.uj + pl * comp + fidx3(g, cell[0] + xo, cell[1] + yo, cell[2] + zo), v);
+    });
+  }
+}
+
 static Push3Args push_args(wpic_ctx3d* c) {
   Push3Args a;
   a.g = c->g;
@@ -346,6 +737,9 @@ static Push3Args push_args(wpic_ctx3d* c) {
   a.B = c->B;
   a.colbase = c->colbaseA;
   a.cumcnt = c->cumcnt;
+  a.ap = c->aperm;
+  a.nov = nullptr;
+  a.ovslot = a.ovcell = nullptr;
   a.uf = c->uf;
   a.uj = c->uj;
   a.pjbuf = c->pjbuf;
@@ -369,6 +763,22 @@ hipError_t launch_deposit3d(wpic_ctx3d* c) {
     hipLaunchKernelGGL(k_deposit3d<true>, dim3(nblk), dim3(p3::NT), 0, c->st, a);
   else
     hipLaunchKernelGGL(k_deposit3d<false>, dim3(nblk), dim3(p3::NT), 0, c->st, a);
+  return hipGetLastError();
+}
+
+__global__ void k_zero_int3(int* p) { *p = 0; }
+
+// The exchange scratch (R, holes) is free between the sort and the next
+// exchange; the fused kernel lists its queue overflow there.
+hipError_t launch_push_deposit3d(wpic_ctx3d* c) {
+  Push3Args a = push_args(c);
+  a.nov = c->nov;
+  a.ovslot = c->R;
+  a.ovcell = c->holes;
+  const int nblk = xcd_grid(a.ntx * a.nty * a.ntz);
+  hipLaunchKernelGGL(k_zero_int3, dim3(1), dim3(1), 0, c->st, c->nov);
+  hipLaunchKernelGGL(k_push_deposit3d_fast, dim3(nblk), dim3(p3::NT), 0, c->st, a);
+  hipLaunchKernelGGL(k_deposit3d_overflow, dim3(256), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }
 

@@ -15,7 +15,9 @@
 //   k_colcount3d  arrivals, new column lengths and bases;
 //   k_colbuild3d  R = for each slot of the post-exchange column the particle
 //                 that lands there (hole filling as :418-455 does it);
-//   k_colsort3d   stable counting sort of R by x cell, gathering B into A.
+//   k_colsort3d   stable counting sort of R by x cell: for every sorted slot
+//                 the gp slot its particle comes from (the next push reads
+//                 the records through it), and the ids gathered into order.
 #include "wpic_internal3d.h"
 
 namespace wpic {
@@ -30,6 +32,7 @@ struct Sort3Args {
   Parts3 A, B;
   const uint64_t* idsrc;
   uint64_t* iddst;
+  int* perm;
   int *code, *R, *holes, *mlist;
   int *colbaseB, *np2B, *colbaseN, *np2N, *cumcnt;
   int *dcnt, *dstart, *colkm;
@@ -419,20 +422,10 @@ __global__ __launch_bounds__(SB) void k_colsort3d(Sort3Args a, int nbits) {
     }
 #pragma unroll
     for (int u = 0; u < SU; u++) key[u] = act[u] ? (a.code[src[u]] >> CODE_SHIFT3) : 0;
-    double vx[SU], vy[SU], vz[SU], vux[SU], vuy[SU], vuz[SU];
     uint64_t vid[SU];
 #pragma unroll
-    for (int u = 0; u < SU; u++) {
-      if (act[u]) {
-        vx[u] = a.B.x[src[u]];
-        vy[u] = a.B.y[src[u]];
-        vz[u] = a.B.z[src[u]];
-        vux[u] = a.B.ux[src[u]];
-        vuy[u] = a.B.uy[src[u]];
-        vuz[u] = a.B.uz[src[u]];
-        vid[u] = a.idsrc[src[u]];
-      }
-    }
+    for (int u = 0; u < SU; u++)
+      if (act[u]) vid[u] = a.idsrc[src[u]];
 #pragma unroll
     for (int u = 0; u < SU; u++) {
       const uint64_t peers = peers_of(key[u], act[u], nbits);
@@ -443,12 +436,7 @@ __global__ __launch_bounds__(SB) void k_colsort3d(Sort3Args a, int nbits) {
       if (act[u] && rank == 0) hw[key[u]] = pos + cnt;
       if (act[u]) {
         const int d = nb + pos;
-        a.A.x[d] = vx[u];
-        a.A.y[d] = vy[u];
-        a.A.z[d] = vz[u];
-        a.A.ux[d] = vux[u];
-        a.A.uy[d] = vuy[u];
-        a.A.uz[d] = vuz[u];
+        a.perm[d] = src[u];
         a.iddst[d] = vid[u];
       }
     }
@@ -478,6 +466,7 @@ static Sort3Args sort3_args(wpic_ctx3d* c) {
   a.B = c->B;
   a.idsrc = c->id[c->idb];
   a.iddst = c->idb == c->ida ? c->id[1 - c->ida] : c->id[c->ida];
+  a.perm = c->permbuf;
   a.code = c->code;
   a.R = c->R;
   a.holes = c->holes;

@@ -225,6 +225,23 @@ int push3(wpic_ctx3d* c) {
   return 0;
 }
 
+// fast mode: particle__solv and ele_cur in one fused pass
+int push_deposit3(wpic_ctx3d* c) {
+  c->cb = c->ca;
+  if (c->cn == c->ca) c->cn = pick_free3(c->ca, c->cb);
+  set_cols(c);
+  c->idb = c->ida;
+  HIPCHK(launch_fill3d(c, c->uj, 3 * plane3(c->g), 0.0));
+  {
+    Timed3 t(c, "push_deposit");
+    HIPCHK(launch_push_deposit3d(c));
+  }
+  c->b_valid = true;
+  c->b_aligned = true;
+  c->exch_pending = false;
+  return 0;
+}
+
 int exchange3(wpic_ctx3d* c) {
   Timed3 t(c, "exchange");
   HIPCHK(launch_colscan3d(c));
@@ -247,6 +264,10 @@ int sort_into_a3(wpic_ctx3d* c) {
   const int dst = c->idb == c->ida ? 1 - c->ida : c->ida;
   c->ida = dst;
   c->idb = dst;
+  // the sorted state is gp's buffer seen through the new permutation; the
+  // old sorted buffer becomes the next gp
+  c->aperm = c->permbuf;
+  std::swap(c->A, c->B);
   set_cols(c);
   c->b_valid = false;
   c->b_aligned = false;
@@ -430,7 +451,7 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   for (auto q : dp)
     if (e == hipSuccess) e = dalloc(q, N);
   for (int k = 0; k < 2 && e == hipSuccess; k++) e = dalloc(&c->id[k], N);
-  int** ip[] = {&c->code, &c->R, &c->holes, &c->mlist};
+  int** ip[] = {&c->code, &c->R, &c->holes, &c->mlist, &c->permbuf};
   for (auto q : ip)
     if (e == hipSuccess) e = dalloc(q, N);
   for (int k = 0; k < 3 && e == hipSuccess; k++) {
@@ -456,6 +477,7 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   if (e == hipSuccess) e = dalloc(&c->part, (size_t)3 * NPB3 * 2);
   if (e == hipSuccess) e = dalloc(&c->scal, 3);
   if (e == hipSuccess) e = dalloc(&c->err, 1);
+  if (e == hipSuccess) e = dalloc(&c->nov, 1);
   if (e == hipSuccess) e = dalloc(&c->red, (size_t)256 * 3);
   if (e == hipSuccess) e = hipMemset(c->err, 0, sizeof(int));
   if (e == hipSuccess) e = hipMemset(c->uf, 0, sizeof(double) * 6 * pl);
@@ -463,6 +485,7 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   if (e == hipSuccess) e = hipMemset(c->uj, 0, sizeof(double) * 3 * pl);
   if (e == hipSuccess) e = hipMemset(c->R, 0, sizeof(int) * N);
   if (e == hipSuccess) e = hipMemset(c->code, 0, sizeof(int) * N);
+  if (e == hipSuccess) e = hipMemset(c->permbuf, 0, sizeof(int) * N);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     wpic_destroy3d(c);
@@ -490,7 +513,7 @@ int wpic_destroy3d(wpic_ctx3d* c) {
                 c->holes,  c->mlist, c->cols[0],  c->cols[1], c->cols[2], c->cumcnt, c->dcnt,   c->dstart,
                 c->colkm,  c->seg,   c->nseg,     c->aosbase, c->uf,     c->df,      c->uj,     c->pjbuf,
                 c->phi,    c->pa,    c->pb,       c->rr,      c->bb,     c->ap,      c->part,   c->scal,
-                c->err,    c->staging, c->gbuf,   c->red};
+                c->err,    c->staging, c->gbuf,   c->red,     c->permbuf, c->nov};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -520,6 +543,7 @@ int wpic_upload_up3d(wpic_ctx3d* c, const double* up, const int* np2, const int*
     for (int i = 0; i < nx; i++)
       if (cr[i + 1] < cr[i]) return set_error(WPIC_EINVAL, "cumcnt must be non-decreasing");
   }
+  c->aperm = nullptr;
   int rc = upload_cols(c, c->ca, up, np2, np, 0);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(c->cumcnt, cumcnt, sizeof(int) * (size_t)nr * (nx + 1), hipMemcpyHostToDevice, c->st));
@@ -649,12 +673,18 @@ int wpic_step3d(wpic_ctx3d* c, int nsteps) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
   for (int it = 0; it < nsteps; it++) {
     c->cn = (c->ca + 1) % 3;
-    int rc = push3(c);
-    if (rc) return rc;
-    c->cn = (c->ca + 1) % 3;
-    set_cols(c);
-    rc = deposit3(c);
-    if (rc) return rc;
+    int rc;
+    if (c->mode == WPIC_MODE_FAST) {
+      rc = push_deposit3(c);
+      if (rc) return rc;
+    } else {
+      rc = push3(c);
+      if (rc) return rc;
+      c->cn = (c->ca + 1) % 3;
+      set_cols(c);
+      rc = deposit3(c);
+      if (rc) return rc;
+    }
     rc = field_after_deposit3(c);
     if (rc) return rc;
     {
@@ -729,6 +759,7 @@ int wpic_ic_beams3d(wpic_ctx3d* c, unsigned long long seed, int n_ppc, double ud
     }
   HIPCHK(hipMemcpyAsync(c->cols[c->ca], hcols.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->cumcnt, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice, c->st));
+  c->aperm = nullptr;
   HIPCHK(launch_ic_beams3d(c, seed, n_ppc, udrift, uth_e, uth_i));
   const size_t pl = plane3(g);
   HIPCHK(launch_fill3d(c, c->uf, 6 * pl, 0.0));

@@ -111,6 +111,25 @@ __host__ __device__ inline void shape3(double dh, double& s0, double& s1, double
   s2 = 0.5 * (0.5 + dh) * (0.5 + dh);
 }
 
+// The particles of one cell over all species, as one list: element t is slot
+// beg[s] + t - cum[s] of species s, the last s with cum[s] <= t (species
+// beyond nsp carry cum = INT_MAX).
+struct CellList {
+  int beg[WPIC_MAXSP], cum[WPIC_MAXSP];
+  __device__ void slot(int t, int& s, int& p) const {
+    int ss = 0, o = t, b = beg[0];
+#pragma unroll
+    for (int k = 1; k < WPIC_MAXSP; k++)
+      if (t >= cum[k]) {
+        ss = k;
+        o = t - cum[k];
+        b = beg[k];
+      }
+    s = ss;
+    p = b + o;
+  }
+};
+
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous band of logical tiles so neighbouring tiles (which share halo
 // atomics) meet in the same L2.  Launch 8*ceil(n/8) groups; extra ones exit.

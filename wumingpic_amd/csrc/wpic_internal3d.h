@@ -51,8 +51,14 @@ struct wpic_ctx3d {
   hipStream_t st;
   int device;
 
-  // particles: species s occupies [s*ncap, (s+1)*ncap); columns compact within
+  // particles: species s occupies [s*ncap, (s+1)*ncap); columns compact within.
+  // A holds the sorted particles (the reference's up) as aperm[k] -> slot of
+  // A's buffer (null: identity); B (gp) shares A's slots after the push.  The
+  // sort writes the permutation and swaps the buffers.
   wpic::Parts3 A, B;
+  int* aperm;    // A's slot -> record in A's buffer, or null
+  int* permbuf;  // [N] the sort's permutation
+  int* nov;      // [1] overflow count of the fused push+deposit
   uint64_t* id[2];
   int ida, idb;
   int *code, *R, *holes, *mlist;  // [N]
@@ -98,6 +104,7 @@ int hip_fail(hipError_t e, const char* what);
 // push / deposit
 hipError_t launch_push3d(wpic_ctx3d* c);
 hipError_t launch_deposit3d(wpic_ctx3d* c);
+hipError_t launch_push_deposit3d(wpic_ctx3d* c);
 hipError_t launch_gather_pj3d(wpic_ctx3d* c);
 // fields
 hipError_t launch_fill3d(wpic_ctx3d* c, double* p, size_t n, double v);
