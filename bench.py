@@ -29,16 +29,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of one particle through the push+deposit kernel: read
-# x, y, ux, uy, uz and write them back (5 doubles each way)
-PUSH_DEPOSIT_BYTES = 80
-# one particle-step of the whole loop: the 48-byte record read once and
-# written once to its sorted slot
+# algorithmic bytes of one particle through the push+deposit kernel: the
+# 48-byte record (x, y, ux, uy, uz, id) read once and written once, the
+# per-particle-step figure of the whole loop (the sort moves no record)
+PUSH_DEPOSIT_BYTES = 96
 STEP_BYTES = 96
 # the kernel that implements the fused push + deposit of the FP64 fast mode
 PUSH_DEPOSIT_KERNEL = "k_push_deposit_fast<16, true, 0>"
-# mixed form: read and write the 24-byte record (cell, 2 offsets, 3 momenta)
-PUSH_DEPOSIT_BYTES_MIXED = 48
+# mixed form: read and write the 32-byte record (cell, 2 offsets, 3 momenta, id)
+PUSH_DEPOSIT_BYTES_MIXED = 64
 
 WORKLOADS = {
     "weibel1024": dict(nx=1024, ny=1024, n_ppc=64,
@@ -50,10 +49,12 @@ WORKLOADS = {
     "weibel3d64": dict(nx=64, ny=64, nz=64, n_ppc=16, name="3D Weibel beams 64x64x64, 16 ppc/species, FP64 path"),
 }
 # algorithmic bytes of one 3D particle through the push or the fused fast
-# push+deposit (read x,y,z,u and write them: 12 doubles) and through the
-# separate deposit (read old and new x,y,z)
-PUSH3D_BYTES = 96
+# push+deposit (read the 56-byte record x,y,z,u,id and write it) and through
+# the separate deposit (read old and new x,y,z)
+PUSH3D_BYTES = 112
 DEPOSIT3D_BYTES = 48
+# 3D mixed form: the 36-byte record (cell, 3 offsets, 3 momenta, id) each way
+PUSH3D_BYTES_MIXED = 72
 
 
 def parse():
@@ -149,9 +150,16 @@ def mixed_run(w, a, npart, elapsed64, uf64, energy64):
     import numpy as np
 
     from wumingpic_amd import MODE_MIXED
-    from wumingpic_amd.pic2d import weibel_beams_2d
 
-    sim = weibel_beams_2d(w["nx"], w["ny"], w["n_ppc"], mode=MODE_MIXED)
+    if "nz" in w:
+        from wumingpic_amd.pic3d import weibel_beams_3d
+        sim = weibel_beams_3d(w["nx"], w["ny"], w["nz"], w["n_ppc"], mode=MODE_MIXED,
+                              capacity=int(w["n_ppc"] * w["nx"] * w["ny"] * w["nz"] * 1.05))
+        nbytes = PUSH3D_BYTES_MIXED
+    else:
+        from wumingpic_amd.pic2d import weibel_beams_2d
+        sim = weibel_beams_2d(w["nx"], w["ny"], w["n_ppc"], mode=MODE_MIXED)
+        nbytes = PUSH_DEPOSIT_BYTES_MIXED
     elapsed = timed_run(sim, a.warmup, a.steps)
     log(f"mixed form: {a.warmup} warmup + {a.steps} timed steps: {elapsed:.3f} s")
     pd_ms, pd_n = sim.timing("push_deposit")
@@ -173,8 +181,8 @@ def mixed_run(w, a, npart, elapsed64, uf64, energy64):
         "ms_per_step": elapsed / a.steps * 1e3,
         "speedup_vs_f64": elapsed64 / elapsed,
         "dtype": "int32 cell + f32 offsets/momenta; f32 push/deposit; f64 field solve and current sum",
-        "push_deposit": {"avg_launch_ms": pd_avg_s * 1e3, "bytes_per_particle": PUSH_DEPOSIT_BYTES_MIXED,
-                         "achieved_GBs": PUSH_DEPOSIT_BYTES_MIXED * npart / pd_avg_s / 1e9},
+        "push_deposit": {"avg_launch_ms": pd_avg_s * 1e3, "bytes_per_particle": nbytes,
+                         "achieved_GBs": nbytes * npart / pd_avg_s / 1e9},
         "phase_ms_per_step": phases,
         "steps_compared": a.warmup + a.steps,
         "field_rel_err_vs_f64": err,
@@ -199,6 +207,9 @@ def main3d(a, w):
     ph = {name: sim.timing(name) for name in names}
     ph = {k: v for k, v in ph.items() if v[1] > 0}
     cg = sim.cg_iters()
+    sim.set_timing(False)
+    energy64 = sim.energy()
+    uf64 = sim.download_fields() if not a.no_mixed else None
     sim.close()
     rates = {}
     for name, nbytes in (("push_deposit", PUSH3D_BYTES), ("push", PUSH3D_BYTES), ("deposit", DEPOSIT3D_BYTES)):
@@ -216,6 +227,8 @@ def main3d(a, w):
         **rates,
         "cg_iters_last_step": list(cg),
     }
+    if not a.no_mixed:
+        out["mixed"] = mixed_run(w, a, npart, elapsed, uf64, energy64)
     print(json.dumps(out))
 
 

@@ -32,12 +32,13 @@ extern "C" {
 /* arithmetic modes of the FP64 path */
 #define WPIC_MODE_FAST 0   /* tiled deposit with atomics, tree dot products */
 #define WPIC_MODE_PARITY 1 /* reference summation order: bit-exact vs 1 thread */
-/* Mixed precision, fixed at wpic_create2d: particles stored as an int32 x cell
- * plus FP32 in-cell offsets and FP32 momenta (32 B with the id), push and
- * deposit in FP32 with FP32 LDS accumulation, FP64 global current sum and
- * FP64 field solve.  Runs the fused loop (wpic_step2d) with the up/fields
- * transfers, diagnostics and seeded state; the single-routine entry points
- * need an FP64 mode. */
+/* Mixed precision, fixed at wpic_create2d / wpic_create3d: particles stored
+ * as an int32 x cell plus FP32 in-cell offsets and FP32 momenta (32 B with the
+ * id in 2D, 36 B in 3D), push and deposit in FP32 (the tile's current
+ * accumulated in FP32 in 2D, FP64 in 3D), FP64 global current sum and FP64
+ * field solve.  Runs the fused loop
+ * (wpic_step2d / wpic_step3d) with the up/fields transfers, diagnostics and
+ * seeded state; the single-routine entry points need an FP64 mode. */
 #define WPIC_MODE_MIXED 2
 
 /* error codes */

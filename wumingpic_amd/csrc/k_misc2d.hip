@@ -12,11 +12,9 @@ namespace wpic {
 struct XferArgs {
   Grid2D g;
   Parts P;
-  uint64_t* id;
   const int *rowbase, *np2;
   const int* R;  // non-null: records seen through a permutation (the exchange's R, or A's aperm)
   const long long* aosbase;  // [nsp*ny] start of each row in the AoS buffer
-  bool id_direct;            // ids stored in slot order (A's ids, sorted by the sort itself)
 };
 
 __global__ void k_aos_to_soa(XferArgs a, const double* __restrict__ aos) {
@@ -33,7 +31,7 @@ __global__ void k_aos_to_soa(XferArgs a, const double* __restrict__ aos) {
     a.P.uz[d] = s[4];
     uint64_t b;
     __builtin_memcpy(&b, &s[5], 8);
-    a.id[d] = b;
+    a.P.id[d] = b;
   }
 }
 
@@ -50,7 +48,7 @@ __global__ void k_soa_to_aos(XferArgs a, double* __restrict__ aos) {
     d[2] = a.P.ux[s];
     d[3] = a.P.uy[s];
     d[4] = a.P.uz[s];
-    const uint64_t b = a.id[a.id_direct ? base + k : s];
+    const uint64_t b = a.P.id[s];
     __builtin_memcpy(&d[5], &b, 8);
   }
 }
@@ -87,12 +85,10 @@ hipError_t launch_up_to_soa(wpic_ctx* c, const double* aos, long long n, int whi
   XferArgs a;
   a.g = c->g;
   a.P = which == 0 ? c->A : c->B;
-  a.id = which == 0 ? c->id[c->ida] : c->id[c->idb];
   a.rowbase = which == 0 ? c->rowbaseA : c->rowbaseB;
   a.np2 = which == 0 ? c->np2A : c->np2B;
   a.R = nullptr;
   a.aosbase = c->aosbase;
-  a.id_direct = true;
   hipLaunchKernelGGL(k_aos_to_soa, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, aos);
   return hipGetLastError();
 }
@@ -101,7 +97,6 @@ hipError_t launch_soa_to_aos(wpic_ctx* c, double* aos, int which) {
   XferArgs a;
   a.g = c->g;
   a.P = which == 0 ? c->A : c->B;
-  a.id = which == 0 ? c->id[c->ida] : c->id[c->idb];
   if (which == 0) {
     a.rowbase = c->rowbaseA;
     a.np2 = c->np2A;
@@ -116,7 +111,6 @@ hipError_t launch_soa_to_aos(wpic_ctx* c, double* aos, int which) {
     a.R = nullptr;
   }
   a.aosbase = c->aosbase;
-  a.id_direct = which == 0;
   hipLaunchKernelGGL(k_soa_to_aos, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, aos);
   return hipGetLastError();
 }
@@ -140,7 +134,7 @@ __global__ void k_aos_to_mixed(XferArgs a, PartsM M, const double* __restrict__ 
     M.uz[d] = (float)s[4];
     uint64_t b;
     __builtin_memcpy(&b, &s[5], 8);
-    a.id[d] = b;
+    M.id[d] = b;
   }
 }
 
@@ -158,7 +152,7 @@ __global__ void k_mixed_to_aos(XferArgs a, PartsM M, double* __restrict__ aos) {
     d[2] = M.ux[s];
     d[3] = M.uy[s];
     d[4] = M.uz[s];
-    const uint64_t b = a.id[base + k];
+    const uint64_t b = M.id[s];
     __builtin_memcpy(&d[5], &b, 8);
   }
 }
@@ -166,12 +160,11 @@ __global__ void k_mixed_to_aos(XferArgs a, PartsM M, double* __restrict__ aos) {
 hipError_t launch_up_to_mixed(wpic_ctx* c, const double* aos) {
   XferArgs a;
   a.g = c->g;
-  a.id = c->id[c->ida];
+  a.P = c->A;
   a.rowbase = c->rowbaseA;
   a.np2 = c->np2A;
   a.R = nullptr;
   a.aosbase = c->aosbase;
-  a.id_direct = true;
   hipLaunchKernelGGL(k_aos_to_mixed, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, c->MA, aos);
   return hipGetLastError();
 }
@@ -179,12 +172,11 @@ hipError_t launch_up_to_mixed(wpic_ctx* c, const double* aos) {
 hipError_t launch_mixed_to_aos(wpic_ctx* c, double* aos) {
   XferArgs a;
   a.g = c->g;
-  a.id = c->id[c->ida];
+  a.P = c->A;
   a.rowbase = c->rowbaseA;
   a.np2 = c->np2A;
   a.R = c->aperm;
   a.aosbase = c->aosbase;
-  a.id_direct = true;
   hipLaunchKernelGGL(k_mixed_to_aos, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a, c->MA, aos);
   return hipGetLastError();
 }
@@ -203,6 +195,7 @@ __global__ void k_gather_rows(Grid2D g, Parts src, const int* ap, Parts dst, con
     dst.ux[d] = src.ux[s];
     dst.uy[d] = src.uy[s];
     dst.uz[d] = src.uz[s];
+    if (dst.id) dst.id[d] = src.id[s];
   }
 }
 
@@ -299,7 +292,7 @@ static IcArgs ic_args(wpic_ctx* c, unsigned long long seed, int n_ppc, double ud
   a.g = c->g;
   a.A = c->A;
   a.M = c->MA;
-  a.id = c->id[c->ida];
+  a.id = c->mixed ? c->MA.id : c->A.id;
   a.seed = seed;
   a.n_ppc = n_ppc;
   a.udrift = udrift;

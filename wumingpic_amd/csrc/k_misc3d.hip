@@ -7,11 +7,9 @@ namespace wpic {
 struct Xfer3Args {
   Grid3D g;
   Parts3 P;
-  uint64_t* id;
   const int *colbase, *np2;
   const int* R;              // non-null: records seen through a permutation (the exchange's R, or A's aperm)
   const long long* aosbase;  // [nsp*ncol] start of each column in the AoS buffer
-  bool id_direct;            // ids stored in slot order (A's ids, sorted by the sort itself)
 };
 
 __global__ void k_aos_to_soa3(Xfer3Args a, const double* __restrict__ aos) {
@@ -29,7 +27,7 @@ __global__ void k_aos_to_soa3(Xfer3Args a, const double* __restrict__ aos) {
     a.P.uz[d] = s[5];
     uint64_t b;
     __builtin_memcpy(&b, &s[6], 8);
-    a.id[d] = b;
+    a.P.id[d] = b;
   }
 }
 
@@ -47,7 +45,7 @@ __global__ void k_soa_to_aos3(Xfer3Args a, double* __restrict__ aos) {
     d[3] = a.P.ux[s];
     d[4] = a.P.uy[s];
     d[5] = a.P.uz[s];
-    const uint64_t b = a.id[a.id_direct ? base + q : s];
+    const uint64_t b = a.P.id[s];
     __builtin_memcpy(&d[6], &b, 8);
   }
 }
@@ -84,12 +82,10 @@ hipError_t launch_up_to_soa3d(wpic_ctx3d* c, const double* aos, long long n, int
   Xfer3Args a;
   a.g = c->g;
   a.P = which == 0 ? c->A : c->B;
-  a.id = which == 0 ? c->id[c->ida] : c->id[c->idb];
   a.colbase = which == 0 ? c->colbaseA : c->colbaseB;
   a.np2 = which == 0 ? c->np2A : c->np2B;
   a.R = nullptr;
   a.aosbase = c->aosbase;
-  a.id_direct = true;
   hipLaunchKernelGGL(k_aos_to_soa3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, a, aos);
   return hipGetLastError();
 }
@@ -98,7 +94,6 @@ hipError_t launch_soa_to_aos3d(wpic_ctx3d* c, double* aos, int which) {
   Xfer3Args a;
   a.g = c->g;
   a.P = which == 0 ? c->A : c->B;
-  a.id = which == 0 ? c->id[c->ida] : c->id[c->idb];
   if (which == 0) {
     a.colbase = c->colbaseA;
     a.np2 = c->np2A;
@@ -113,8 +108,77 @@ hipError_t launch_soa_to_aos3d(wpic_ctx3d* c, double* aos, int which) {
     a.R = nullptr;
   }
   a.aosbase = c->aosbase;
-  a.id_direct = which == 0;
   hipLaunchKernelGGL(k_soa_to_aos3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, a, aos);
+  return hipGetLastError();
+}
+
+// mixed-precision transfers: the reference's FP64 AoS columns <-> x cell +
+// FP32 offsets (y, z relative to the column) and momenta
+__global__ void k_aos_to_mixed3(Xfer3Args a, PartsM3 M, const double* __restrict__ aos) {
+  const int rw = blockIdx.x;
+  const int col = rw % a.g.ncol;
+  const int j = a.g.nys + col % a.g.ny, k = a.g.nzs + col / a.g.ny;
+  const int base = a.colbase[rw], n = a.np2[rw];
+  const double* src = aos + a.aosbase[rw] * 7;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const double* s = src + (size_t)q * 7;
+    const int d = base + q;
+    const int ix = (int)s[0];
+    M.ix[d] = ix;
+    M.ox[d] = to_offset(s[0] - ix);
+    M.oy[d] = to_offset(s[1] - j);
+    M.oz[d] = to_offset(s[2] - k);
+    M.ux[d] = (float)s[3];
+    M.uy[d] = (float)s[4];
+    M.uz[d] = (float)s[5];
+    uint64_t b;
+    __builtin_memcpy(&b, &s[6], 8);
+    M.id[d] = b;
+  }
+}
+
+__global__ void k_mixed_to_aos3(Xfer3Args a, PartsM3 M, double* __restrict__ aos) {
+  const int rw = blockIdx.x;
+  const int col = rw % a.g.ncol;
+  const int j = a.g.nys + col % a.g.ny, k = a.g.nzs + col / a.g.ny;
+  const int base = a.colbase[rw], n = a.np2[rw];
+  double* dst = aos + a.aosbase[rw] * 7;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    int s = a.R ? a.R[base + q] : base + q;
+    if ((unsigned)s >= (unsigned)(a.g.ncap * a.g.nsp)) s = 0;
+    double* d = dst + (size_t)q * 7;
+    d[0] = (double)M.ix[s] + (double)M.ox[s];
+    d[1] = (double)j + (double)M.oy[s];
+    d[2] = (double)k + (double)M.oz[s];
+    d[3] = M.ux[s];
+    d[4] = M.uy[s];
+    d[5] = M.uz[s];
+    const uint64_t b = M.id[s];
+    __builtin_memcpy(&d[6], &b, 8);
+  }
+}
+
+hipError_t launch_up_to_mixed3d(wpic_ctx3d* c, const double* aos) {
+  Xfer3Args a;
+  a.g = c->g;
+  a.P = c->A;
+  a.colbase = c->colbaseA;
+  a.np2 = c->np2A;
+  a.R = nullptr;
+  a.aosbase = c->aosbase;
+  hipLaunchKernelGGL(k_aos_to_mixed3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, a, c->MA, aos);
+  return hipGetLastError();
+}
+
+hipError_t launch_mixed_to_aos3d(wpic_ctx3d* c, double* aos) {
+  Xfer3Args a;
+  a.g = c->g;
+  a.P = c->A;
+  a.colbase = c->colbaseA;
+  a.np2 = c->np2A;
+  a.R = c->aperm;
+  a.aosbase = c->aosbase;
+  hipLaunchKernelGGL(k_mixed_to_aos3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, a, c->MA, aos);
   return hipGetLastError();
 }
 
@@ -153,12 +217,15 @@ __device__ inline double in_cell(int cell, double u, double delx, double d_delx)
 struct Ic3Args {
   Grid3D g;
   Parts3 A;
-  uint64_t* id;
+  PartsM3 M;
   unsigned long long seed;
   int n_ppc;
   double udrift, uth_e, uth_i;
 };
 
+// MIXED: the same draws stored as x cell + FP32 offsets and momenta, so the
+// FP64 and mixed forms start from the same state up to FP32 rounding
+template <bool MIXED>
 __global__ void k_ic_beams3(Ic3Args a) {
   const Grid3D& g = a.g;
   const long long colen = (long long)a.n_ppc * g.nx;
@@ -172,9 +239,9 @@ __global__ void k_ic_beams3(Ic3Args a) {
   const int ci = (int)(m / a.n_ppc), sl = (int)(m - (long long)ci * a.n_ppc);
   const long long d = s * g.ncap + q;
   const int i = g.nxgs + ci, j = g.nys + col % g.ny, k = g.nzs + col / g.ny;
-  a.A.x[d] = in_cell(i, u01_3(rkey3(a.seed, 1, col, ci, sl)), g.delx, g.d_delx);
-  a.A.y[d] = in_cell(j, u01_3(rkey3(a.seed, 2, col, ci, sl)), g.delx, g.d_delx);
-  a.A.z[d] = in_cell(k, u01_3(rkey3(a.seed, 3, col, ci, sl)), g.delx, g.d_delx);
+  const double x = in_cell(i, u01_3(rkey3(a.seed, 1, col, ci, sl)), g.delx, g.d_delx);
+  const double y = in_cell(j, u01_3(rkey3(a.seed, 2, col, ci, sl)), g.delx, g.d_delx);
+  const double z = in_cell(k, u01_3(rkey3(a.seed, 3, col, ci, sl)), g.delx, g.d_delx);
   double n0, n1, n2, n3;
   normal2_3(rkey3(a.seed, 10 + 4 * s, col, ci, sl), rkey3(a.seed, 11 + 4 * s, col, ci, sl), n0, n1);
   normal2_3(rkey3(a.seed, 12 + 4 * s, col, ci, sl), rkey3(a.seed, 13 + 4 * s, col, ci, sl), n2, n3);
@@ -182,10 +249,24 @@ __global__ void k_ic_beams3(Ic3Args a) {
   const double th = electron ? a.uth_e : a.uth_i;
   double ux = th * n0;
   if (electron) ux += (sl & 1) ? -a.udrift : a.udrift;
-  a.A.ux[d] = ux;
-  a.A.uy[d] = th * n1;
-  a.A.uz[d] = th * n2;
-  a.id[d] = (uint64_t)(-(q + 1));
+  if (MIXED) {
+    a.M.ix[d] = i;
+    a.M.ox[d] = to_offset(x - i);
+    a.M.oy[d] = to_offset(y - j);
+    a.M.oz[d] = to_offset(z - k);
+    a.M.ux[d] = (float)ux;
+    a.M.uy[d] = (float)(th * n1);
+    a.M.uz[d] = (float)(th * n2);
+    a.M.id[d] = (uint64_t)(-(q + 1));
+  } else {
+    a.A.x[d] = x;
+    a.A.y[d] = y;
+    a.A.z[d] = z;
+    a.A.ux[d] = ux;
+    a.A.uy[d] = th * n1;
+    a.A.uz[d] = th * n2;
+    a.A.id[d] = (uint64_t)(-(q + 1));
+  }
 }
 
 hipError_t launch_ic_beams3d(wpic_ctx3d* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
@@ -193,14 +274,17 @@ hipError_t launch_ic_beams3d(wpic_ctx3d* c, unsigned long long seed, int n_ppc, 
   Ic3Args a;
   a.g = c->g;
   a.A = c->A;
-  a.id = c->id[c->ida];
+  a.M = c->MA;
   a.seed = seed;
   a.n_ppc = n_ppc;
   a.udrift = udrift;
   a.uth_e = uth_e;
   a.uth_i = uth_i;
   const long long n = (long long)n_ppc * c->g.nx * c->g.ncol * c->g.nsp;
-  hipLaunchKernelGGL(k_ic_beams3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  if (c->mixed)
+    hipLaunchKernelGGL(k_ic_beams3<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  else
+    hipLaunchKernelGGL(k_ic_beams3<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }
 
@@ -220,7 +304,8 @@ __device__ inline double bsum3(double v, double* sh) {
 }
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, const int* colbase, const int* np2,
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, PartsM3 M, const int* colbase, const int* np2,
                                                  const int* ap, const double* uf, double* out) {
   __shared__ double sh[4];
   double kin = 0.0, e2 = 0.0, b2 = 0.0;
@@ -230,7 +315,10 @@ __global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, const int* 
     const int base = colbase[rw], n = np2[rw];
     for (int q = threadIdx.x; q < n; q += 256) {
       const int p = ap ? ap[base + q] : base + q;
-      const double u2 = A.ux[p] * A.ux[p] + A.uy[p] * A.uy[p] + A.uz[p] * A.uz[p];
+      const double ux = MIXED ? (double)M.ux[p] : A.ux[p];
+      const double uy = MIXED ? (double)M.uy[p] : A.uy[p];
+      const double uz = MIXED ? (double)M.uz[p] : A.uz[p];
+      const double u2 = ux * ux + uy * uy + uz * uz;
       kin = kin + g.r[s] * (sqrt(1 + u2 / (g.c * g.c)) - 1);
     }
   }
@@ -254,21 +342,29 @@ __global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, const int* 
 }
 
 hipError_t launch_energy3d(wpic_ctx3d* c, double* dout) {
-  hipLaunchKernelGGL(k_energy3, dim3(RB3), dim3(256), 0, c->st, c->g, c->A, c->colbaseA, c->np2A, c->aperm,
-                     c->uf, dout);
+  if (c->mixed)
+    hipLaunchKernelGGL(k_energy3<true>, dim3(RB3), dim3(256), 0, c->st, c->g, c->A, c->MA, c->colbaseA, c->np2A,
+                       c->aperm, c->uf, dout);
+  else
+    hipLaunchKernelGGL(k_energy3<false>, dim3(RB3), dim3(256), 0, c->st, c->g, c->A, c->MA, c->colbaseA, c->np2A,
+                       c->aperm, c->uf, dout);
   return hipGetLastError();
 }
 
 // rho at cell centres with the quadratic shape, wrapped periodically, on the
 // interior array [nx*ny*nz]
-__global__ __launch_bounds__(256) void k_rho3(Grid3D g, Parts3 A, const int* colbase, const int* np2, const int* ap,
-                                              double* rho) {
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_rho3(Grid3D g, Parts3 A, PartsM3 M, const int* colbase, const int* np2,
+                                              const int* ap, double* rho) {
   const int rw = blockIdx.x;
   const int s = rw / g.ncol;
   const int base = colbase[rw], n = np2[rw];
   for (int q = threadIdx.x; q < n; q += 256) {
     const int p = ap ? ap[base + q] : base + q;
-    const double x = A.x[p], y = A.y[p], z = A.z[p];
+    const int jc = g.nys + (rw % g.ncol) % g.ny, kc = g.nzs + (rw % g.ncol) / g.ny;
+    const double x = MIXED ? (double)M.ix[p] + (double)M.ox[p] : A.x[p];
+    const double y = MIXED ? (double)jc + (double)M.oy[p] : A.y[p];
+    const double z = MIXED ? (double)kc + (double)M.oz[p] : A.z[p];
     const int i0 = (int)(x * g.d_delx), j0 = (int)(y * g.d_delx), k0 = (int)(z * g.d_delx);
     if (i0 < g.nxgs || i0 > g.nxge || j0 < g.nys || j0 > g.nye || k0 < g.nzs || k0 > g.nze) continue;
     double sx[3], sy[3], sz[3];
@@ -307,8 +403,12 @@ hipError_t launch_gauss3d(wpic_ctx3d* c, double* G, double* rho) {
   const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
   hipError_t e = launch_fill3d(c, rho, n, 0.0);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_rho3, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, c->g, c->A, c->colbaseA,
-                     c->np2A, c->aperm, rho);
+  if (c->mixed)
+    hipLaunchKernelGGL(k_rho3<true>, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, c->g, c->A, c->MA,
+                       c->colbaseA, c->np2A, c->aperm, rho);
+  else
+    hipLaunchKernelGGL(k_rho3<false>, dim3((unsigned)(c->g.nsp * c->g.ncol)), dim3(256), 0, c->st, c->g, c->A, c->MA,
+                       c->colbaseA, c->np2A, c->aperm, rho);
   hipLaunchKernelGGL(k_gauss3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, c->g, c->uf, rho, G);
   return hipGetLastError();
 }

@@ -171,6 +171,7 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
         a.B.ux[pi] = g3;
         a.B.uy[pi] = g4;
         a.B.uz[pi] = g5;
+        a.B.id[pi] = a.A.id[ia];
         if (incy < -1 || incy > 1) {
           atomicOr(a.err, ERR_DOMAIN);
           incy = incy < -1 ? -1 : 1;

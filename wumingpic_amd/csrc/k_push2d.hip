@@ -187,13 +187,13 @@ __device__ inline void flush_block(double (&v)[N], double* pjbuf, double (*acc)[
 template <int BC>
 __device__ inline int wrap_code(const Grid2D& g, double& nxp, double& nyp, int j, int* err) {
   if (BC == WPIC_BC_PERIODIC) {
-    const int ipos = (int)rd_div(nxp, g.delx);
+    const int ipos = cell_rd(nxp, g.delx);
     if (ipos < g.nxgs)
       nxp = rd_add(nxp, g.spanx);
     else if (ipos >= g.nxge + 1)
       nxp = rd_add(nxp, -g.spanx);
   }
-  const int jpos = (int)rd_div(nyp, g.delx);
+  const int jpos = cell_rd(nyp, g.delx);
   if (jpos != j) {
     if (jpos <= g.nygs - 1)
       nyp = rd_add(nyp, g.spany);
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
         // reflecting walls act before the deposit in the reconnection order
         // (2d/proj/reconnection/app.f90:101-103; boundary_reconnection.f90:79-91)
         if (FUSED && BC == WPIC_BC_WALLX) {
-          const int ipos = (int)(nxp / g.delx);
+          const int ipos = cell_rn(nxp, g.delx);
           if (ipos < g.nxgs + 1) {
             nxp = g.wall_lo - nxp;
             g3 = -g3; g4 = -g4; g5 = -g5;
@@ -375,6 +375,7 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
           a.B.ux[pi] = g3;
           a.B.uy[pi] = g4;
           a.B.uz[pi] = g5;
+          a.B.id[pi] = a.A.id[ia];  // particle.f90:173-177
         }
       }
     }
@@ -441,7 +442,7 @@ __device__ inline void push_reflect(const Grid2D& g, const double (*sf)[SFW * SF
   boris_push(sf, li, lj, i, j, g.d_delx, g.delt, g.c, unit_c, kc[0], kc[1], kc[2], x, y, ux, uy, uz, nxp, nyp, g3, g4,
              g5, gmv);
   if (BC == WPIC_BC_WALLX) {
-    const int ipos = (int)(nxp / g.delx);
+    const int ipos = cell_rn(nxp, g.delx);
     if (ipos < g.nxgs + 1) {
       nxp = g.wall_lo - nxp;
       g3 = -g3; g4 = -g4; g5 = -g5;
@@ -543,6 +544,7 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
         a.B.ux[pi] = g3;
         a.B.uy[pi] = g4;
         a.B.uz[pi] = g5;
+        a.B.id[pi] = a.A.id[ia];  // particle.f90:173-177
 
         double s0x[5], s0y[5], dsx[5], dsy[5];
         esirkepov_shapes(g, x, y, nxp, nyp, i, j, s0x, s0y, dsx, dsy);
@@ -568,7 +570,7 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
             for (int u = 0; u < 3; u++) sz[v][u] = fma(s0x[u + 1], A, fma(dsx[u + 1], B, sz[v][u]));
           }
         } else {
-          const int slot = atomicAdd(&nxq, 1);
+          const int slot = wave_enqueue(&nxq);
           if (slot < NXQ2)
             xq[slot] = make_int2(pi, cc | (s << 8));
           else
@@ -607,7 +609,12 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
   // the queued particles that changed cell: the push again (the same bits, its
   // stores already made) and the full block
   const int nq = nxq < NXQ2 ? nxq : NXQ2;
-  for (int k = tid; k < nq; k += NT) {
+  // thread t drains entries [t per, (t + 1) per): the lanes of a wave work on
+  // entries far apart, mostly of different cells, so their atomics rarely meet
+  const int per = (nq + NT - 1) / NT;
+  for (int it = 0; it < per; it++) {
+    const int k = tid * per + it;
+    if (k >= nq) break;
     const int2 e = xq[k];
     const int pi = e.x, cc = e.y & 255, s = e.y >> 8;
     const int li = cc % TX, lj = cc / TX;

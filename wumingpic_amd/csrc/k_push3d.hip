@@ -167,6 +167,7 @@ __global__ __launch_bounds__(p3::NT) void k_push3d(Push3Args a) {
         a.B.ux[pi] = g4;
         a.B.uy[pi] = g5;
         a.B.uz[pi] = g6;
+        a.B.id[pi] = a.A.id[ia];  // particle.f90:227-231
       }
     }
   }
@@ -556,44 +557,12 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
     const int so = (lk + 1) * (SW * SH) + (lj + 1) * SW + (li + 1);
     const int bx = li + 2, by = lj + 2, bz = lk + 2;  // the cell in the current tile
 
-    // software pipeline: the record of the next particle and the slot of the
-    // one after are loaded while the current particle is computed
-    int t = lg, ian = 0;
-    double x = 0.0, y = 0.0, z = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
-    if (t < ntot) {
-      int s0, p0;
-      cl.slot(t, s0, p0);
-      const int ia = a.ap ? a.ap[p0] : p0;
-      x = a.A.x[ia];
-      y = a.A.y[ia];
-      z = a.A.z[ia];
-      ux = a.A.ux[ia];
-      uy = a.A.uy[ia];
-      uz = a.A.uz[ia];
-    }
-    if (t + GF < ntot) {
-      int s1, p1;
-      cl.slot(t + GF, s1, p1);
-      ian = a.ap ? a.ap[p1] : p1;
-    }
-    for (; t < ntot; t += GF) {
-      double xn = 0.0, yn = 0.0, zn = 0.0, uxn = 0.0, uyn = 0.0, uzn = 0.0;
-      if (t + GF < ntot) {
-        xn = a.A.x[ian];
-        yn = a.A.y[ian];
-        zn = a.A.z[ian];
-        uxn = a.A.ux[ian];
-        uyn = a.A.uy[ian];
-        uzn = a.A.uz[ian];
-      }
-      int iann = 0;
-      if (t + 2 * GF < ntot) {
-        int s2, p2;
-        cl.slot(t + 2 * GF, s2, p2);
-        iann = a.ap ? a.ap[p2] : p2;
-      }
+    for (int t = lg; t < ntot; t += GF) {
       int sc, pc;
       cl.slot(t, sc, pc);
+      const int ia = a.ap ? a.ap[pc] : pc;
+      const double x = a.A.x[ia], y = a.A.y[ia], z = a.A.z[ia];
+      const double ux = a.A.ux[ia], uy = a.A.uy[ia], uz = a.A.uz[ia];
       const double* kc = spc[sc];
       double nw[3], g4, g5, g6, sh[9];
       boris_push3(sf, so, i, j, k, d_delx, delt, c, kc[0], kc[1], kc[2], x, y, z, ux, uy, uz, nw[0], nw[1], nw[2], g4,
@@ -604,6 +573,7 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
       a.B.ux[pc] = g4;
       a.B.uy[pc] = g5;
       a.B.uz[pc] = g6;
+      a.B.id[pc] = a.A.id[ia];
 
       // new shape and cell increment per axis (field.f90:253-329)
       const double qd = kc[3];
@@ -649,7 +619,7 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
           }
       } else {
         // a particle that changes cell: its window is added after the cell loop
-        const int slot = atomicAdd(&nxq, 1);
+        const int slot = wave_enqueue(&nxq);
         if (slot < NXQ) {
           xq[slot] = make_int2(pc, cc | (sc << 8));
         } else {
@@ -660,8 +630,6 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
         }
       }
 
-      x = xn; y = yn; z = zn; ux = uxn; uy = uyn; uz = uzn;
-      ian = iann;
     }
     // reduce the stayers' current over the group, spread the LDS atomics
 #pragma unroll
@@ -687,7 +655,12 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
   __syncthreads();
   // the queued particles that changed cell
   const int nq = nxq < NXQ ? nxq : NXQ;
-  for (int q = tid; q < nq; q += NT) {
+  // thread t drains entries [t per, (t + 1) per): the lanes of a wave work on
+  // entries far apart, mostly of different cells, so their atomics rarely meet
+  const int per = (nq + NT - 1) / NT;
+  for (int it = 0; it < per; it++) {
+    const int q = tid * per + it;
+    if (q >= nq) break;
     const int2 e = xq[q];
     const int cc = e.y & 255, s = e.y >> 8;
     const int li = cc % TX, lj = (cc / TX) % TY, lk = cc / (TX * TY);

@@ -14,7 +14,7 @@
 //               row, the index of the particle that lands there;
 //   k_rowsort   one workgroup per row: stable counting sort of R by cell
 //               (per-wave histograms, then a wave multi-split by ballots on
-//               the key bits), gathering the particles into sorted order.
+//               the key bits), written as a permutation.
 #include "wpic_internal.h"
 
 namespace wpic {
@@ -24,8 +24,6 @@ constexpr int SB = 256;  // threads per row workgroup
 struct SortArgs {
   Grid2D g;
   Parts A, B;
-  const uint64_t* idsrc;
-  uint64_t* iddst;
   int *code, *R, *holes, *ulist, *dlist;
   int *rowbaseB, *np2B, *rowbaseN, *np2N, *rowbaseA, *np2A, *cumcnt, *rowcnt;
   int* err;
@@ -43,7 +41,7 @@ __global__ __launch_bounds__(SB) void k_wrapx(SortArgs a) {
     const int p = base + k;
     double x = a.B.x[p];
     if (g.bc == WPIC_BC_WALLX) {
-      const int ipos = (int)(x / g.delx);
+      const int ipos = cell_rn(x, g.delx);
       if (ipos < g.nxgs + 1) {
         a.B.x[p] = g.wall_lo - x;
         a.B.ux[p] = -a.B.ux[p];
@@ -56,7 +54,7 @@ __global__ __launch_bounds__(SB) void k_wrapx(SortArgs a) {
         a.B.uz[p] = -a.B.uz[p];
       }
     } else {
-      const int ipos = (int)rd_div(x, g.delx);
+      const int ipos = cell_rd(x, g.delx);
       if (ipos < g.nxgs)
         a.B.x[p] = rd_add(x, g.spanx);
       else if (ipos >= g.nxge + 1)
@@ -85,7 +83,7 @@ __global__ __launch_bounds__(SB) void k_rowscan(SortArgs a) {
       const int p = base + k;
       if (YWRAP) {
         double y = a.B.y[p];
-        const int jpos = (int)rd_div(y, g.delx);
+        const int jpos = cell_rd(y, g.delx);
         if (jpos != j) {
           if (jpos <= g.nygs - 1)
             a.B.y[p] = rd_add(y, g.spany);
@@ -310,33 +308,17 @@ __device__ inline uint64_t key_peers(int key, bool act, int nbits) {
   return peers;
 }
 
-// stable counting sort of one row by cell (sort.f90:52-78): the permutation
-// and the ids, by the mover M.
+// stable counting sort of one row by cell (sort.f90:52-78) as a permutation.
 // Each wave owns a contiguous quarter of the row: its per-key histogram, then
 // (after a scan over keys and waves) its per-key write cursors.  Within a
 // wave, lanes with equal keys are ranked by one ballot per key bit.  Chunks
 // of 64 are processed U at a time so their loads overlap.
 constexpr int SU = 4;
 
-// The sort moves no particle record: it writes, for every sorted slot, the gp
-// slot its particle comes from (the next push reads the records through it),
-// and gathers the ids into slot order.
-struct MovePerm {
-  const uint64_t* idsrc;
-  uint64_t* iddst;
-  int* perm;
-  struct Rec {
-    uint64_t id;
-  };
-  __device__ Rec load(int s) const { return {idsrc[s]}; }
-  __device__ void store(int d, const Rec& r, int s) const {
-    perm[d] = s;
-    iddst[d] = r.id;
-  }
-};
-
-template <class M>
-__global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, M mv, int nbits) {
+// The sort moves no particle record: for every sorted slot it writes the gp
+// slot its particle comes from; the next push reads the record and its id
+// through it.
+__global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, int* perm, int nbits) {
   extern __shared__ int hist[];  // [SB/64][nx]
   __shared__ int wsum[SB / 64];
   __shared__ int carry;
@@ -422,10 +404,6 @@ __global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, M mv, int nbits) {
     }
 #pragma unroll
     for (int u = 0; u < SU; u++) key[u] = act[u] ? (a.code[src[u]] >> 2) : 0;
-    typename M::Rec v[SU];
-#pragma unroll
-    for (int u = 0; u < SU; u++)
-      if (act[u]) v[u] = mv.load(src[u]);
 #pragma unroll
     for (int u = 0; u < SU; u++) {
       const uint64_t peers = key_peers(key[u], act[u], nbits);
@@ -434,7 +412,7 @@ __global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, M mv, int nbits) {
       int pos = 0;
       if (act[u]) pos = hw[key[u]] + rank;
       if (act[u] && rank == 0) hw[key[u]] = pos + cnt;
-      if (act[u]) mv.store(nb + pos, v[u], src[u]);
+      if (act[u]) perm[nb + pos] = src[u];
     }
   }
 }
@@ -461,8 +439,6 @@ static SortArgs sort_args(wpic_ctx* c) {
   a.g = c->g;
   a.A = c->A;
   a.B = c->B;
-  a.idsrc = c->id[c->idb];
-  a.iddst = c->idb == c->ida ? c->id[1 - c->ida] : c->id[c->ida];
   a.code = c->code;
   a.R = c->R;
   a.holes = c->holes;
@@ -518,8 +494,8 @@ hipError_t launch_rowsort(wpic_ctx* c) {
   int nbits = 0;
   while ((1 << nbits) < c->g.nx) nbits++;
   const size_t shm = sizeof(int) * (SB / 64) * (size_t)c->g.nx;
-  MovePerm mv{a.idsrc, a.iddst, c->perm_out ? c->perm_out : c->permbuf};
-  hipLaunchKernelGGL(k_rowsort<MovePerm>, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, mv, nbits);
+  int* perm = c->perm_out ? c->perm_out : c->permbuf;
+  hipLaunchKernelGGL(k_rowsort, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, perm, nbits);
   return hipGetLastError();
 }
 

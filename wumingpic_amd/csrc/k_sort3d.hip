@@ -17,7 +17,7 @@
 //                 that lands there (hole filling as :418-455 does it);
 //   k_colsort3d   stable counting sort of R by x cell: for every sorted slot
 //                 the gp slot its particle comes from (the next push reads
-//                 the records through it), and the ids gathered into order.
+//                 the record and its id through it).
 #include "wpic_internal3d.h"
 
 namespace wpic {
@@ -30,8 +30,6 @@ __device__ inline uint64_t lt_mask(int lane) { return lane == 0 ? 0ull : (~0ull 
 struct Sort3Args {
   Grid3D g;
   Parts3 A, B;
-  const uint64_t* idsrc;
-  uint64_t* iddst;
   int* perm;
   int *code, *R, *holes, *mlist;
   int *colbaseB, *np2B, *colbaseN, *np2N, *cumcnt;
@@ -422,10 +420,6 @@ __global__ __launch_bounds__(SB) void k_colsort3d(Sort3Args a, int nbits) {
     }
 #pragma unroll
     for (int u = 0; u < SU; u++) key[u] = act[u] ? (a.code[src[u]] >> CODE_SHIFT3) : 0;
-    uint64_t vid[SU];
-#pragma unroll
-    for (int u = 0; u < SU; u++)
-      if (act[u]) vid[u] = a.idsrc[src[u]];
 #pragma unroll
     for (int u = 0; u < SU; u++) {
       const uint64_t peers = peers_of(key[u], act[u], nbits);
@@ -434,11 +428,7 @@ __global__ __launch_bounds__(SB) void k_colsort3d(Sort3Args a, int nbits) {
       int pos = 0;
       if (act[u]) pos = hw[key[u]] + rank;
       if (act[u] && rank == 0) hw[key[u]] = pos + cnt;
-      if (act[u]) {
-        const int d = nb + pos;
-        a.perm[d] = src[u];
-        a.iddst[d] = vid[u];
-      }
+      if (act[u]) a.perm[nb + pos] = src[u];
     }
   }
 }
@@ -464,8 +454,6 @@ static Sort3Args sort3_args(wpic_ctx3d* c) {
   a.g = c->g;
   a.A = c->A;
   a.B = c->B;
-  a.idsrc = c->id[c->idb];
-  a.iddst = c->idb == c->ida ? c->id[1 - c->ida] : c->id[c->ida];
   a.perm = c->permbuf;
   a.code = c->code;
   a.R = c->R;
@@ -492,8 +480,9 @@ hipError_t launch_wrapx3d(wpic_ctx3d* c) {
   return hipGetLastError();
 }
 
-hipError_t launch_colscan3d(wpic_ctx3d* c) {
-  hipLaunchKernelGGL(k_colscan3d, ncols(c), dim3(SB), 0, c->st, sort3_args(c), 1);
+// wrap = 0: the codes come from the mixed-precision push (cell wrap done)
+hipError_t launch_colscan3d(wpic_ctx3d* c, int wrap) {
+  hipLaunchKernelGGL(k_colscan3d, ncols(c), dim3(SB), 0, c->st, sort3_args(c), wrap);
   return hipGetLastError();
 }
 

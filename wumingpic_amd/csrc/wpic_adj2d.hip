@@ -72,7 +72,9 @@ hipError_t dal(T** p, size_t n) {
   return hipMalloc((void**)p, (n ? n : 1) * sizeof(T));
 }
 
-Parts parts_of(double* base, size_t N) { return Parts{base, base + N, base + 2 * N, base + 3 * N, base + 4 * N}; }
+Parts parts_of(double* base, uint64_t* id, size_t N) {
+  return Parts{base, base + N, base + 2 * N, base + 3 * N, base + 4 * N, id};
+}
 
 int copy_d(wpic_ctx* c, void* dst, const void* src, size_t bytes) {
   HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
@@ -83,9 +85,9 @@ int copy_d(wpic_ctx* c, void* dst, const void* src, size_t bytes) {
 int save_state(wpic_ctx* c, double* part, uint64_t* id, int* rows, int* cumcnt, double* uf, double* df) {
   AdjState& a = *c->adj;
   const size_t N = a.N;
-  // the sorted particles in slot order (A is read through its permutation)
-  if (part) HIPCHK(launch_gather_rows(c, c->A, c->aperm, parts_of(part, N), c->rows[c->ra]));
-  if (id && copy_d(c, id, c->id[c->ida], sizeof(uint64_t) * N)) return WPIC_EHIP;
+  // the sorted particles and their ids in slot order (A is read through its
+  // permutation)
+  if (part) HIPCHK(launch_gather_rows(c, c->A, c->aperm, parts_of(part, id, N), c->rows[c->ra]));
   if (rows && copy_d(c, rows, c->rows[c->ra], sizeof(int) * 2 * a.nr)) return WPIC_EHIP;
   if (cumcnt && copy_d(c, cumcnt, c->cumcnt, sizeof(int) * a.ncc)) return WPIC_EHIP;
   if (uf && copy_d(c, uf, c->uf, sizeof(double) * 6 * a.pl)) return WPIC_EHIP;
@@ -100,7 +102,7 @@ int restore_state(wpic_ctx* c, int s) {
   for (int k = 0; k < 5; k++)
     if (copy_d(c, dst[k], a.ck_part[s] + k * N, sizeof(double) * N)) return WPIC_EHIP;
   c->aperm = nullptr;
-  if (copy_d(c, c->id[c->ida], a.ck_id[s], sizeof(uint64_t) * N)) return WPIC_EHIP;
+  if (copy_d(c, c->A.id, a.ck_id[s], sizeof(uint64_t) * N)) return WPIC_EHIP;
   if (copy_d(c, c->rows[c->ra], a.ck_rows[s], sizeof(int) * 2 * a.nr)) return WPIC_EHIP;
   if (copy_d(c, c->cumcnt, a.ck_cumcnt[s], sizeof(int) * a.ncc)) return WPIC_EHIP;
   if (copy_d(c, c->uf, a.ck_uf[s], sizeof(double) * 6 * a.pl)) return WPIC_EHIP;
@@ -248,7 +250,7 @@ int wpic_adj_backward2d(wpic_ctx* c, double seed) {
       if (rc) return rc;
       HIPCHK(launch_adj_field2(c, a.lamF2, a.lamJ));
       HIPCHK(hipMemsetAsync(a.lamT, 0, sizeof(double) * 6 * a.nc, c->st));
-      HIPCHK(launch_adj_particles(c, parts_of(a.st_part[t], N), a.st_rows[t], a.st_uf[t], a.lamB, a.lamJ, a.lamA2,
+      HIPCHK(launch_adj_particles(c, parts_of(a.st_part[t], nullptr, N), a.st_rows[t], a.st_uf[t], a.lamB, a.lamJ, a.lamA2,
                                   a.lamT, N));
       HIPCHK(launch_adj_t2f(c, a.lamT, a.lamF2));
       std::swap(a.lamF, a.lamF2);
@@ -270,7 +272,7 @@ int wpic_adj_param_grads2d(wpic_ctx* c, double udrift, double uth_e, int bz_mode
   const int n_ppc = rows[a.nr] / c->g.nx;
   if (n_ppc < 1 || rows[a.nr] != n_ppc * c->g.nx)
     return set_error(WPIC_EINVAL, "the recorded run did not start from a wpic_ic_beams2d state");
-  HIPCHK(launch_adj_params(c, parts_of(a.ck_part[0], a.N), a.ck_id[0], a.ck_rows[0], a.lamA, a.N, a.lamF, udrift,
+  HIPCHK(launch_adj_params(c, parts_of(a.ck_part[0], a.ck_id[0], a.N), a.ck_id[0], a.ck_rows[0], a.lamA, a.N, a.lamF, udrift,
                            uth_e, bz_mode, n_ppc, a.red));
   std::vector<double> h(256 * 3);
   HIPCHK(hipMemcpyAsync(h.data(), a.red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));

@@ -242,9 +242,6 @@ int sort_into_a(wpic_ctx* c) {
   c->ra = c->rn;
   c->rn = old_a;
   if (c->rb == c->ra) c->rb = pick_free(c->ra, c->rn);
-  const int dst = c->idb == c->ida ? 1 - c->ida : c->ida;
-  c->ida = dst;
-  c->idb = dst;
   // the sorted state is gp's buffer seen through the new permutation; the
   // old sorted buffer becomes the next gp
   c->aperm = c->perm_out ? c->perm_out : c->permbuf;
@@ -266,7 +263,6 @@ int step_once(wpic_ctx* c) {
   c->rb = c->ra;
   c->rn = (c->ra + 1) % 3;
   set_rows(c);
-  c->idb = c->ida;
   int rc = deposit(c, true, true);
   if (rc) return rc;
   c->b_valid = true;
@@ -407,7 +403,8 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
   c->rb = 1;
   c->rn = 2;
   set_rows(c);
-  c->ida = c->idb = 0;
+  c->A.id = c->MA.id = c->id[0];
+  c->B.id = c->MB.id = c->id[1];
   for (int k = 0; k < 3; k++) c->h_rows_ok[k] = false;
   *out = c;
   return 0;
@@ -571,7 +568,6 @@ int wpic_upload_gp2d(wpic_ctx* c, const double* gp, const int* np2, int np) {
   if (!c || !gp || !np2) return set_error(WPIC_EINVAL, "null argument");
   if (c->rb == c->ra) c->rb = pick_free(c->ra, c->rn);
   set_rows(c);
-  if (c->idb == c->ida) c->idb = 1 - c->ida;
   int rc = upload_rows(c, c->rb, gp, np2, np, 1);
   if (rc) return rc;
   c->b_valid = true;
@@ -648,7 +644,6 @@ int wpic_particle_solv2d(wpic_ctx* c) {
   c->rb = c->ra;
   if (c->rn == c->ra) c->rn = pick_free(c->ra, c->rb);
   set_rows(c);
-  c->idb = c->ida;
   {
     Timed t(c, "push");
     HIPCHK(launch_push_deposit(c, true, false, false));

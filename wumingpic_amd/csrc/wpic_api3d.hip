@@ -214,7 +214,6 @@ int push3(wpic_ctx3d* c) {
   c->cb = c->ca;
   if (c->cn == c->ca) c->cn = pick_free3(c->ca, c->cb);
   set_cols(c);
-  c->idb = c->ida;
   {
     Timed3 t(c, "push");
     HIPCHK(launch_push3d(c));
@@ -225,16 +224,19 @@ int push3(wpic_ctx3d* c) {
   return 0;
 }
 
-// fast mode: particle__solv and ele_cur in one fused pass
+// fast and mixed modes: particle__solv and ele_cur in one fused pass (the
+// mixed one also writes the exchange codes with the x cell wrapped)
 int push_deposit3(wpic_ctx3d* c) {
   c->cb = c->ca;
   if (c->cn == c->ca) c->cn = pick_free3(c->ca, c->cb);
   set_cols(c);
-  c->idb = c->ida;
   HIPCHK(launch_fill3d(c, c->uj, 3 * plane3(c->g), 0.0));
   {
     Timed3 t(c, "push_deposit");
-    HIPCHK(launch_push_deposit3d(c));
+    if (c->mixed)
+      HIPCHK(launch_push_deposit_mixed3d(c));
+    else
+      HIPCHK(launch_push_deposit3d(c));
   }
   c->b_valid = true;
   c->b_aligned = true;
@@ -244,7 +246,7 @@ int push_deposit3(wpic_ctx3d* c) {
 
 int exchange3(wpic_ctx3d* c) {
   Timed3 t(c, "exchange");
-  HIPCHK(launch_colscan3d(c));
+  HIPCHK(launch_colscan3d(c, c->mixed ? 0 : 1));
   HIPCHK(launch_colcount3d(c));
   HIPCHK(launch_colbuild3d(c));
   c->h_cols_ok[c->cn] = false;
@@ -261,19 +263,22 @@ int sort_into_a3(wpic_ctx3d* c) {
   c->ca = c->cn;
   c->cn = old_a;
   if (c->cb == c->ca) c->cb = pick_free3(c->ca, c->cn);
-  const int dst = c->idb == c->ida ? 1 - c->ida : c->ida;
-  c->ida = dst;
-  c->idb = dst;
   // the sorted state is gp's buffer seen through the new permutation; the
   // old sorted buffer becomes the next gp
   c->aperm = c->permbuf;
   std::swap(c->A, c->B);
+  std::swap(c->MA, c->MB);
   set_cols(c);
   c->b_valid = false;
   c->b_aligned = false;
   c->exch_pending = false;
   c->h_cols_ok[c->ca] = false;
   return 0;
+}
+
+int mixed_only_step3() {
+  return set_error(WPIC_EINVAL,
+                   "the mixed-precision form runs the fused loop (wpic_step3d); single routines need an FP64 mode");
 }
 
 int cols_from_np2(wpic_ctx3d* c, const int* np2, std::vector<int>& hcols, std::vector<long long>& aosb,
@@ -315,7 +320,10 @@ int upload_cols(wpic_ctx3d* c, int which, const double* up, const int* np2, int 
   HIPCHK(hipMemcpyAsync(c->staging, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->cols[which], hcols.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->aosbase, aosb.data(), sizeof(long long) * nr, hipMemcpyHostToDevice, c->st));
-  HIPCHK(launch_up_to_soa3d(c, c->staging, total, kind));
+  if (kind == 3)
+    HIPCHK(launch_up_to_mixed3d(c, c->staging));
+  else
+    HIPCHK(launch_up_to_soa3d(c, c->staging, total, kind));
   HIPCHK(hipStreamSynchronize(c->st));
   c->h_cols[which] = hcols;
   c->h_cols_ok[which] = true;
@@ -337,7 +345,10 @@ int download_cols(wpic_ctx3d* c, int which, int kind, double* up, int* np2, int 
   rc = ensure_staging3(c, (size_t)total * 7);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(c->aosbase, aosb.data(), sizeof(long long) * nr, hipMemcpyHostToDevice, c->st));
-  HIPCHK(launch_soa_to_aos3d(c, c->staging, kind));
+  if (kind == 3)
+    HIPCHK(launch_mixed_to_aos3d(c, c->staging));
+  else
+    HIPCHK(launch_soa_to_aos3d(c, c->staging, kind));
   std::vector<double> h((size_t)total * 7);
   HIPCHK(hipMemcpyAsync(h.data(), c->staging, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -389,11 +400,14 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   if (p->capacity <= 0 || p->capacity * p->nsp >= (1ll << 31))
     return set_error(WPIC_EINVAL, "capacity must be positive and nsp*capacity below 2^31");
   if ((long long)ny * nz >= (1ll << 27)) return set_error(WPIC_EINVAL, "too many (j, k) columns");
+  if (p->mode != WPIC_MODE_FAST && p->mode != WPIC_MODE_PARITY && p->mode != WPIC_MODE_MIXED)
+    return set_error(WPIC_EINVAL, "bad mode");
 
   wpic_ctx3d* c = new wpic_ctx3d();
   std::memset((void*)&c->g, 0, sizeof(c->g));
   c->P = *p;
   c->mode = p->mode;
+  c->mixed = p->mode == WPIC_MODE_MIXED;
   Grid3D& g = c->g;
   g.nx = nx;
   g.ny = ny;
@@ -446,10 +460,18 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   const size_t N = (size_t)g.ncap * g.nsp;
   const int nr = g.nsp * g.ncol;
   const size_t pl = plane3(g), nc = (size_t)nx * ny * nz;
-  double** dp[] = {&c->A.x, &c->A.y, &c->A.z, &c->A.ux, &c->A.uy, &c->A.uz,
-                   &c->B.x, &c->B.y, &c->B.z, &c->B.ux, &c->B.uy, &c->B.uz};
-  for (auto q : dp)
-    if (e == hipSuccess) e = dalloc(q, N);
+  if (c->mixed) {
+    for (PartsM3* m : {&c->MA, &c->MB}) {
+      if (e == hipSuccess) e = dalloc(&m->ix, N);
+      for (float** q : {&m->ox, &m->oy, &m->oz, &m->ux, &m->uy, &m->uz})
+        if (e == hipSuccess) e = dalloc(q, N);
+    }
+  } else {
+    double** dp[] = {&c->A.x, &c->A.y, &c->A.z, &c->A.ux, &c->A.uy, &c->A.uz,
+                     &c->B.x, &c->B.y, &c->B.z, &c->B.ux, &c->B.uy, &c->B.uz};
+    for (auto q : dp)
+      if (e == hipSuccess) e = dalloc(q, N);
+  }
   for (int k = 0; k < 2 && e == hipSuccess; k++) e = dalloc(&c->id[k], N);
   int** ip[] = {&c->code, &c->R, &c->holes, &c->mlist, &c->permbuf};
   for (auto q : ip)
@@ -495,7 +517,8 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   c->cb = 1;
   c->cn = 2;
   set_cols(c);
-  c->ida = c->idb = 0;
+  c->A.id = c->MA.id = c->id[0];
+  c->B.id = c->MB.id = c->id[1];
   for (int k = 0; k < 3; k++) c->h_cols_ok[k] = false;
   *out = c;
   return 0;
@@ -513,7 +536,9 @@ int wpic_destroy3d(wpic_ctx3d* c) {
                 c->holes,  c->mlist, c->cols[0],  c->cols[1], c->cols[2], c->cumcnt, c->dcnt,   c->dstart,
                 c->colkm,  c->seg,   c->nseg,     c->aosbase, c->uf,     c->df,      c->uj,     c->pjbuf,
                 c->phi,    c->pa,    c->pb,       c->rr,      c->bb,     c->ap,      c->part,   c->scal,
-                c->err,    c->staging, c->gbuf,   c->red,     c->permbuf, c->nov};
+                c->err,    c->staging, c->gbuf,   c->red,     c->permbuf, c->nov,
+                c->MA.ix,  c->MA.ox, c->MA.oy,    c->MA.oz,   c->MA.ux,  c->MA.uy,  c->MA.uz,
+                c->MB.ix,  c->MB.ox, c->MB.oy,    c->MB.oz,   c->MB.ux,  c->MB.uy,  c->MB.uz};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -522,7 +547,10 @@ int wpic_destroy3d(wpic_ctx3d* c) {
 }
 
 int wpic_set_mode3d(wpic_ctx3d* c, int mode) {
-  if (!c || (mode != WPIC_MODE_FAST && mode != WPIC_MODE_PARITY)) return set_error(WPIC_EINVAL, "bad mode");
+  if (!c || (mode != WPIC_MODE_FAST && mode != WPIC_MODE_PARITY && mode != WPIC_MODE_MIXED))
+    return set_error(WPIC_EINVAL, "bad mode");
+  if (c->mixed != (mode == WPIC_MODE_MIXED))
+    return set_error(WPIC_EINVAL, "the mixed-precision form is fixed when the context is created");
   c->mode = mode;
   return 0;
 }
@@ -544,7 +572,7 @@ int wpic_upload_up3d(wpic_ctx3d* c, const double* up, const int* np2, const int*
       if (cr[i + 1] < cr[i]) return set_error(WPIC_EINVAL, "cumcnt must be non-decreasing");
   }
   c->aperm = nullptr;
-  int rc = upload_cols(c, c->ca, up, np2, np, 0);
+  int rc = upload_cols(c, c->ca, up, np2, np, c->mixed ? 3 : 0);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(c->cumcnt, cumcnt, sizeof(int) * (size_t)nr * (nx + 1), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -556,7 +584,7 @@ int wpic_upload_up3d(wpic_ctx3d* c, const double* up, const int* np2, const int*
 
 int wpic_download_up3d(wpic_ctx3d* c, double* up, int* np2, int* cumcnt, int np) {
   if (!c || !up) return set_error(WPIC_EINVAL, "null argument");
-  int rc = download_cols(c, c->ca, 0, up, np2, np);
+  int rc = download_cols(c, c->ca, c->mixed ? 3 : 0, up, np2, np);
   if (rc) return rc;
   if (cumcnt) {
     const size_t ncc = (size_t)c->g.nsp * c->g.ncol * (c->g.nx + 1);
@@ -567,9 +595,9 @@ int wpic_download_up3d(wpic_ctx3d* c, double* up, int* np2, int* cumcnt, int np)
 
 int wpic_upload_gp3d(wpic_ctx3d* c, const double* gp, const int* np2, int np) {
   if (!c || !gp || !np2) return set_error(WPIC_EINVAL, "null argument");
+  if (c->mixed) return mixed_only_step3();
   if (c->cb == c->ca) c->cb = pick_free3(c->ca, c->cn);
   set_cols(c);
-  if (c->idb == c->ida) c->idb = 1 - c->ida;
   int rc = upload_cols(c, c->cb, gp, np2, np, 1);
   if (rc) return rc;
   c->b_valid = true;
@@ -582,6 +610,7 @@ int wpic_upload_gp3d(wpic_ctx3d* c, const double* gp, const int* np2, int np) {
 
 int wpic_download_gp3d(wpic_ctx3d* c, double* gp, int* np2, int np) {
   if (!c || !gp) return set_error(WPIC_EINVAL, "null argument");
+  if (c->mixed) return mixed_only_step3();
   if (!c->b_valid) return set_error(WPIC_EINVAL, "no gp on the device (call particle__solv first)");
   if (c->exch_pending) return download_cols(c, c->cn, 1, gp, np2, np);
   return download_cols(c, c->cb, 2, gp, np2, np);
@@ -614,6 +643,7 @@ int wpic_upload_dfield3d(wpic_ctx3d* c, const double* df) {
 
 int wpic_particle_solv3d(wpic_ctx3d* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step3();
   int rc = push3(c);
   if (rc) return rc;
   return check_err3(c);
@@ -621,6 +651,7 @@ int wpic_particle_solv3d(wpic_ctx3d* c) {
 
 int wpic_field_fdtd_i3d(wpic_ctx3d* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step3();
   if (!c->b_valid || !c->b_aligned || c->exch_pending)
     return set_error(WPIC_EINVAL, "field__fdtd_i needs gp slot-aligned with up (call particle__solv first)");
   int rc = deposit3(c);
@@ -632,6 +663,7 @@ int wpic_field_fdtd_i3d(wpic_ctx3d* c) {
 
 int wpic_bc_particle_x3d(wpic_ctx3d* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step3();
   if (!c->b_valid || c->exch_pending) return set_error(WPIC_EINVAL, "bc__particle_x needs gp before the exchange");
   HIPCHK(launch_wrapx3d(c));
   return check_err3(c);
@@ -639,6 +671,7 @@ int wpic_bc_particle_x3d(wpic_ctx3d* c) {
 
 int wpic_bc_particle_yz3d(wpic_ctx3d* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step3();
   if (!c->b_valid || c->exch_pending) return set_error(WPIC_EINVAL, "bc__particle_yz needs gp before the exchange");
   if (c->cn == c->cb || c->cn == c->ca) c->cn = pick_free3(c->ca, c->cb);
   set_cols(c);
@@ -649,6 +682,7 @@ int wpic_bc_particle_yz3d(wpic_ctx3d* c) {
 
 int wpic_sort_bucket3d(wpic_ctx3d* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step3();
   if (!c->b_valid) return set_error(WPIC_EINVAL, "sort__bucket needs gp on the device");
   if (!c->exch_pending) {
     if (c->cn == c->cb || c->cn == c->ca) c->cn = pick_free3(c->ca, c->cb);
@@ -674,7 +708,7 @@ int wpic_step3d(wpic_ctx3d* c, int nsteps) {
   for (int it = 0; it < nsteps; it++) {
     c->cn = (c->ca + 1) % 3;
     int rc;
-    if (c->mode == WPIC_MODE_FAST) {
+    if (c->mode == WPIC_MODE_FAST || c->mixed) {
       rc = push_deposit3(c);
       if (rc) return rc;
     } else {
@@ -687,7 +721,7 @@ int wpic_step3d(wpic_ctx3d* c, int nsteps) {
     }
     rc = field_after_deposit3(c);
     if (rc) return rc;
-    {
+    if (!c->mixed) {
       Timed3 t(c, "exchange");
       HIPCHK(launch_wrapx3d(c));
     }

@@ -39,8 +39,11 @@ __host__ __device__ inline int cidx(const Grid2D& g, int i, int j) {
   return (i - g.nxgs) + g.nx * (j - g.nys);
 }
 
+// a particle buffer: the record and the reference's id (raw 64 bits) travel
+// together
 struct Parts {
   double *x, *y, *ux, *uy, *uz;
+  uint64_t* id;
 };
 
 // mixed-precision particles: x cell index, FP32 in-cell offsets (x - ix and
@@ -48,6 +51,7 @@ struct Parts {
 struct PartsM {
   int* ix;
   float *ox, *oy, *ux, *uy, *uz;
+  uint64_t* id;
 };
 
 // an in-cell offset in [0, 1) as FP32: rounding up to 1 is pulled back below
@@ -104,6 +108,13 @@ __host__ __device__ inline double rd_div(double a, double b) {
   return q;
 }
 
+// int(x / delx) of the particle boundaries, under round-down
+// (boundary_periodic.f90:74) or round-to-nearest: with delx = 1, the only
+// cell width a context accepts (sort__bucket bins by int(x)), the quotient is
+// x itself in every rounding mode
+__host__ __device__ inline int cell_rd(double x, double delx) { return (int)(delx == 1.0 ? x : rd_div(x, delx)); }
+__host__ __device__ inline int cell_rn(double x, double delx) { return (int)(delx == 1.0 ? x : x / delx); }
+
 // quadratic spline with nodes at cell centres (particle.f90:99-107)
 __host__ __device__ inline void shape3(double dh, double& s0, double& s1, double& s2) {
   s0 = 0.5 * (0.5 - dh) * (0.5 - dh);
@@ -129,6 +140,18 @@ struct CellList {
     p = b + o;
   }
 };
+
+// Slots in an LDS queue for the active lanes of a wave with one atomic on the
+// counter *n: the active lanes get consecutive slots in lane order.
+__device__ inline int wave_enqueue(int* n) {
+  const uint64_t m = __ballot(1);
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(n, __popcll(m));
+  base = __shfl(base, leader);
+  return base + __popcll(m & ((1ull << lane) - 1));
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous band of logical tiles so neighbouring tiles (which share halo
@@ -164,15 +187,15 @@ struct wpic_ctx {
   // particles: species s occupies [s*ncap, (s+1)*ncap); rows compact within
   // A holds the sorted particles (the reference's up) as aperm[k] -> slot of
   // A's buffer (null: identity); B holds gp, slot-aligned with A after the
-  // push.  The sort writes only the permutation and the ids, then A and B
-  // swap buffers: the records are read through aperm by the next push.
+  // push, which moves each record and its id.  The sort writes only the
+  // permutation, then A and B swap buffers (ids included): the records are
+  // read through aperm by the next push.
   wpic::Parts A, B;
   bool mixed;            // WPIC_MODE_MIXED context: MA/MB hold the particles
   wpic::PartsM MA, MB;
   int* aperm;            // A's slot -> record in A's buffer, or null
   int* permbuf;          // [N] the sort's permutation
-  uint64_t* id[2];
-  int ida, idb;
+  uint64_t* id[2];  // the two id buffers (A's and B's)
   int *code, *R, *holes, *ulist, *dlist;
   int* rows[3];                   // row structures: rowbase [nsp*ny] then np2 [nsp*ny]
   int ra, rb, rn;                 // which row structure belongs to A, B, the exchange

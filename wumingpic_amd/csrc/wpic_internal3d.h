@@ -29,8 +29,18 @@ __host__ __device__ inline size_t cidx3(const Grid3D& g, int i, int j, int k) {
 }
 __host__ __device__ inline size_t plane3(const Grid3D& g) { return (size_t)g.fw * g.fh * g.fd; }
 
+// a particle buffer: the record and its id travel together
 struct Parts3 {
   double *x, *y, *z, *ux, *uy, *uz;
+  uint64_t* id;
+};
+
+// mixed-precision particles: x cell index, FP32 in-cell offsets (x - ix,
+// y - j, z - k, all in [0, 1)) and FP32 momenta; 36 B per particle with the id
+struct PartsM3 {
+  int* ix;
+  float *ox, *oy, *oz, *ux, *uy, *uz;
+  uint64_t* id;
 };
 
 // Exchange direction of a particle leaving column (j, k) for (j+dj, k+dk):
@@ -53,14 +63,16 @@ struct wpic_ctx3d {
 
   // particles: species s occupies [s*ncap, (s+1)*ncap); columns compact within.
   // A holds the sorted particles (the reference's up) as aperm[k] -> slot of
-  // A's buffer (null: identity); B (gp) shares A's slots after the push.  The
-  // sort writes the permutation and swaps the buffers.
+  // A's buffer (null: identity); B (gp) shares A's slots after the push, which
+  // moves each record and its id.  The sort writes the permutation and swaps
+  // the buffers (ids included).
   wpic::Parts3 A, B;
+  bool mixed;  // WPIC_MODE_MIXED context: MA/MB hold the particles
+  wpic::PartsM3 MA, MB;
   int* aperm;    // A's slot -> record in A's buffer, or null
   int* permbuf;  // [N] the sort's permutation
   int* nov;      // [1] overflow count of the fused push+deposit
-  uint64_t* id[2];
-  int ida, idb;
+  uint64_t* id[2];  // the two id buffers (A's and B's)
   int *code, *R, *holes, *mlist;  // [N]
   int* cols[3];                   // column structures: base [nsp*ncol] then np2 [nsp*ncol]
   int ca, cb, cn;                 // which column structure belongs to A, B, the exchange
@@ -105,6 +117,7 @@ int hip_fail(hipError_t e, const char* what);
 hipError_t launch_push3d(wpic_ctx3d* c);
 hipError_t launch_deposit3d(wpic_ctx3d* c);
 hipError_t launch_push_deposit3d(wpic_ctx3d* c);
+hipError_t launch_push_deposit_mixed3d(wpic_ctx3d* c);
 hipError_t launch_gather_pj3d(wpic_ctx3d* c);
 // fields
 hipError_t launch_fill3d(wpic_ctx3d* c, double* p, size_t n, double v);
@@ -119,7 +132,7 @@ hipError_t launch_energy3d(wpic_ctx3d* c, double* dout);
 hipError_t launch_gauss3d(wpic_ctx3d* c, double* G, double* rho);
 // particles
 hipError_t launch_wrapx3d(wpic_ctx3d* c);
-hipError_t launch_colscan3d(wpic_ctx3d* c);
+hipError_t launch_colscan3d(wpic_ctx3d* c, int wrap = 1);
 hipError_t launch_colcount3d(wpic_ctx3d* c);
 hipError_t launch_colbuild3d(wpic_ctx3d* c);
 hipError_t launch_colsort3d(wpic_ctx3d* c);
@@ -131,5 +144,7 @@ hipError_t launch_planes_from_aos3d(wpic_ctx3d* c, const double* aos, double* pl
 hipError_t launch_planes_to_aos3d(wpic_ctx3d* c, const double* planes, double* aos, int ncomp);
 hipError_t launch_ic_beams3d(wpic_ctx3d* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
                              double uth_i);
+hipError_t launch_up_to_mixed3d(wpic_ctx3d* c, const double* aos);
+hipError_t launch_mixed_to_aos3d(wpic_ctx3d* c, double* aos);
 
 }  // namespace wpic
