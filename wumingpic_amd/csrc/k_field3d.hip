@@ -176,10 +176,13 @@ struct Cg3Args {
   int parity;
 };
 
+// in 32-bit arithmetic: a context holds fewer than 2^31 cells
 __device__ inline void decode3(const Grid3D& g, size_t q, int& i, int& j, int& k) {
-  i = g.nxgs + (int)(q % g.nx);
-  j = g.nys + (int)((q / g.nx) % g.ny);
-  k = g.nzs + (int)(q / ((size_t)g.nx * g.ny));
+  const unsigned qq = (unsigned)q, nx = (unsigned)g.nx, ny = (unsigned)g.ny;
+  const unsigned row = qq / nx;
+  i = g.nxgs + (int)(qq - row * nx);
+  j = g.nys + (int)(row % ny);
+  k = g.nzs + (int)(row / ny);
 }
 
 // interior index of the periodic image of (i, j, k), one coordinate at most

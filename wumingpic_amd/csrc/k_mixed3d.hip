@@ -10,8 +10,8 @@
 // the FP64 uj with FP64 atomics and the field solve is the FP64 one.  The work mapping is the FP64
 // fused kernel's (k_push_deposit3d_fast): 8x4x4-cell tiles, 8 lanes per cell,
 // the 54 current values of a particle that stays in its cell in registers, a
-// particle that changes cell queued and added with its 3 x 4 x 4 window per
-// component after the tile's cells.  The push writes the exchange code
+// particle that changes cell queued with its offsets and added with its
+// 3 x 4 x 4 window per component after each round of cells.  The push writes the exchange code
 // ((new x cell - nxgs) << 4 | y-z direction) itself: the x wrap is a cell
 // wrap and the y-z offsets are already relative to the destination column.
 #include "wpic_internal3d.h"
@@ -25,7 +25,7 @@ namespace m3 {
 constexpr int TX = 8, TY = 4, TZ = 4, NT = 256, G = 8;
 constexpr int SW = TX + 2, SH = TY + 2, SD = TZ + 2, SN = SW * SH * SD;  // staged tmpf
 constexpr int AW = TX + 4, AH = TY + 4, AD = TZ + 4, AN = AW * AH * AD;  // current tile
-constexpr int NXQ = 2048;  // queued cell-changing particles per tile (beyond: added at once)
+constexpr int NXQ = 512;   // queued cell-changing particles per round (beyond: added at once)
 }  // namespace m3
 
 struct Mix3Args {
@@ -194,14 +194,17 @@ __device__ inline void tile_origin_m(const Grid3D& g, int tile, int ntx, int nty
   k0 = g.nzs + (rest / nty) * m3::TZ;
 }
 
-__global__ __launch_bounds__(m3::NT, 2) void k_push_deposit_mixed3d(Mix3Args a) {
+__global__ __launch_bounds__(m3::NT, 3) void k_push_deposit_mixed3d(Mix3Args a) {
 #pragma clang fp contract(fast)
   using namespace m3;
   __shared__ float sf[6][SN];
   __shared__ double acc[3][AN];
   __shared__ float spc[WPIC_MAXSP][4];  // fac1, txxx, fac2, q delx / delt
-  __shared__ int2 xq[NXQ];              // slot, cell in the tile | species << 8
-  __shared__ int nxq;
+  // this round's cell-changing particles: old and new in-cell offsets, and
+  // cell in the tile | species << 7 | (inc + 1) per axis << 9, 11, 13
+  __shared__ float xqo[NXQ][6];
+  __shared__ int xqc[NXQ];
+  __shared__ int nxq[2];
   const Grid3D& g = a.g;
   const int ntiles = a.ntx * a.nty * a.ntz;
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(m3::NT, 2) void k_push_deposit_mixed3d(Mix3Args a) 
     if (i <= g.nxge + 1 && j <= g.nye + 1 && k <= g.nze + 1) sf[c][rem] = (float)tmpf3m(g, a.uf, c, i, j, k);
   }
   for (int q = tid; q < 3 * AN; q += NT) (&acc[0][0])[q] = 0.0f;
-  if (tid == 0) nxq = 0;
+  if (tid == 0) nxq[0] = nxq[1] = 0;
 #pragma unroll
   for (int s = 0; s < WPIC_MAXSP; s++) {
     if (tid == s && s < g.nsp) {
@@ -234,10 +237,13 @@ __global__ __launch_bounds__(m3::NT, 2) void k_push_deposit_mixed3d(Mix3Args a) 
   mc.delt = (float)g.delt;
   mc.ic2 = (float)(1.0 / (g.c * g.c));
   mc.cc2 = (float)(g.c * g.c);
-  for (int cc = grp; cc < TX * TY * TZ; cc += NGRP) {
+  // rounds of NGRP cells (one per group), each followed by the queue's drain
+  for (int rnd = 0; rnd < (TX * TY * TZ) / NGRP; rnd++) {
+    int* const nq_r = &nxq[rnd & 1];
+    const int cc = rnd * NGRP + grp;
     const int li = cc % TX, lj = (cc / TX) % TY, lk = cc / (TX * TY);
     const int i = i0 + li, j = j0 + lj, k = k0 + lk;
-    if (i > g.nxge || j > g.nye || k > g.nze) continue;  // uniform within a group
+    if (i <= g.nxge && j <= g.nye && k <= g.nze) {  // uniform within a group
     const int col = (j - g.nys) + g.ny * (k - g.nzs);
     const int so = (lk + 1) * (SW * SH) + (lj + 1) * SW + (li + 1);
     const int bx = li + 2, by = lj + 2, bz = lk + 2;
@@ -255,8 +261,8 @@ __global__ __launch_bounds__(m3::NT, 2) void k_push_deposit_mixed3d(Mix3Args a) 
       const float qd = kc[3];
       for (int pi = pbeg + lg; pi < pend; pi += G) {
         const int ia = a.ap ? a.ap[pi] : pi;
-        const Mix3Push r = mixed_push3(mc, sf, so, kc, a.A.ox[ia], a.A.oy[ia], a.A.oz[ia], a.A.ux[ia], a.A.uy[ia],
-                                       a.A.uz[ia]);
+        const float o[3] = {a.A.ox[ia], a.A.oy[ia], a.A.oz[ia]};
+        const Mix3Push r = mixed_push3(mc, sf, so, kc, o[0], o[1], o[2], a.A.ux[ia], a.A.uy[ia], a.A.uz[ia]);
         a.B.ux[pi] = r.g4;
         a.B.uy[pi] = r.g5;
         a.B.uz[pi] = r.g6;
@@ -299,9 +305,14 @@ __global__ __launch_bounds__(m3::NT, 2) void k_push_deposit_mixed3d(Mix3Args a) 
               }
             }
         } else {
-          const int slot = wave_enqueue(&nxq);
+          const int slot = wave_enqueue(nq_r);
           if (slot < NXQ) {
-            xq[slot] = make_int2(pi, cc | (s << 8));
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+              xqo[slot][d] = o[d];
+              xqo[slot][3 + d] = r.f[d];
+            }
+            xqc[slot] = cc | (s << 7) | ((r.inc[0] + 1) << 9) | ((r.inc[1] + 1) << 11) | ((r.inc[2] + 1) << 13);
           } else {
             deposit3_window_m(r, qd, [&](int comp, int xo, int yo, int zo, float v) {
               tile_add(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
@@ -342,29 +353,28 @@ __global__ __launch_bounds__(m3::NT, 2) void k_push_deposit_mixed3d(Mix3Args a) 
         tile_add(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
       }
     }
-  }
+    }  // the cell
 
-  __syncthreads();
-  // the queued particles that changed cell: the push again, then the window
-  const int nq = nxq < NXQ ? nxq : NXQ;
-  // thread t drains entries [t per, (t + 1) per): the lanes of a wave work on
-  // entries far apart, mostly of different cells, so their atomics rarely meet
-  const int per = (nq + NT - 1) / NT;
-  for (int it = 0; it < per; it++) {
-    const int q = tid * per + it;
-    if (q >= nq) break;
-    const int2 e = xq[q];
-    const int pi = e.x, cc = e.y & 255, s = e.y >> 8;
-    const int li = cc % TX, lj = (cc / TX) % TY, lk = cc / (TX * TY);
-    const int so = (lk + 1) * (SW * SH) + (lj + 1) * SW + (li + 1);
-    const int bx = li + 2, by = lj + 2, bz = lk + 2;
-    const int ia = a.ap ? a.ap[pi] : pi;
-    const float* kc = spc[s];
-    const Mix3Push r =
-        mixed_push3(mc, sf, so, kc, a.A.ox[ia], a.A.oy[ia], a.A.oz[ia], a.A.ux[ia], a.A.uy[ia], a.A.uz[ia]);
-    deposit3_window_m(r, kc[3], [&](int comp, int xo, int yo, int zo, float v) {
-      tile_add(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
-    });
+    __syncthreads();
+    // this round's queued particles that changed cell, from their offsets
+    const int nq = *nq_r < NXQ ? *nq_r : NXQ;
+    for (int q = tid; q < nq; q += NT) {
+      const int e = xqc[q];
+      const int qcc = e & 127, s = (e >> 7) & 3;
+      const int qbx = qcc % TX + 2, qby = (qcc / TX) % TY + 2, qbz = qcc / (TX * TY) + 2;
+      Mix3Push r;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        shape3m(xqo[q][d] - 0.5f, r.sh[3 * d], r.sh[3 * d + 1], r.sh[3 * d + 2]);
+        r.f[d] = xqo[q][3 + d];
+        r.inc[d] = ((e >> (9 + 2 * d)) & 3) - 1;
+      }
+      deposit3_window_m(r, spc[s][3], [&](int comp, int xo, int yo, int zo, float v) {
+        tile_add(&acc[comp][((qbz + zo) * AH + (qby + yo)) * AW + (qbx + xo)], v);
+      });
+    }
+    if (tid == 0) nxq[(rnd + 1) & 1] = 0;
+    __syncthreads();
   }
 
   __syncthreads();

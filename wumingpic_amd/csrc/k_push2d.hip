@@ -466,20 +466,21 @@ __device__ inline void push_reflect(const Grid2D& g, const double (*sf)[SFW * SF
 // last up to rounding), so its current lives on 6 + 6 + 9 nodes: each lane
 // accumulates those 21 values in registers over the particles of the cell it
 // handles and the group reduces them once per cell.  A particle that changes
-// cell is queued in LDS and, after the tile's cells, pushed again (the same
-// bits) and added with its full block by all lanes (deposit2_full); beyond the
-// queue's capacity it is added at once.  With FUSED the wraps and the exchange
-// code follow in the same pass; otherwise gp keeps the unwrapped positions as
-// particle__solv leaves them.
-constexpr int NXQ2 = 3072;  // queued cell-changing particles per tile
-
+// cell (about one in eight here) is queued in LDS with what its current needs
+// (old and unwrapped new position, q vz); after each round of cells (one per
+// group) all lanes add the queued particles' full blocks (deposit2_full), and
+// beyond the queue's capacity a particle adds its block at once.  With FUSED
+// the wraps and the exchange code follow in the same pass; otherwise gp keeps
+// the unwrapped positions as particle__solv leaves them.
 template <int G, bool FUSED, int BC>
 __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
   __shared__ double sf[6][SFW * SFH];
   __shared__ double acc[3][ACW * ACH];
   __shared__ double spc[WPIC_MAXSP][5];  // fac1, txxx, fac2, qk, q
-  __shared__ int2 xq[NXQ2];              // slot, cell in the tile | species << 8
-  __shared__ int nxq;
+  constexpr int NXQR = 512;        // queued cell-changing particles per round
+  __shared__ double xqd[NXQR][5];  // x, y, unwrapped new x, y, q vz
+  __shared__ int xqc[NXQR];        // cell in the tile | species << 8
+  __shared__ int nxq[2];           // this round's and the next round's counts
 
   const Grid2D& g = a.g;
   const int ntiles = a.ntx * a.nty;
@@ -495,7 +496,7 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
     if (i <= g.nxge + 1 && j <= g.nye + 1) sf[c][rem] = cell_field(g, a.uf, c, i, j);
   }
   for (int k = tid; k < 3 * ACW * ACH; k += NT) (&acc[0][0])[k] = 0.0;
-  if (tid == 0) nxq = 0;
+  if (tid == 0) nxq[0] = nxq[1] = 0;
   // per-species constants (particle.f90:92-94, field.f90:274)
 #pragma unroll
   for (int s = 0; s < WPIC_MAXSP; s++) {
@@ -515,10 +516,13 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
   constexpr int NGRP = NT / G;
   const bool unit_c = g.c * g.c == 1.0;
 
-  for (int cc = grp; cc < TX * TY; cc += NGRP) {
+  // rounds of NGRP cells (one per group), each followed by the queue's drain
+  for (int rnd = 0; rnd < (TX * TY) / NGRP; rnd++) {
+    int* const nq_r = &nxq[rnd & 1];
+    const int cc = rnd * NGRP + grp;
     const int li = cc % TX, lj = cc / TX;
     const int i = i0 + li, j = j0 + lj;
-    if (i > g.nxge || j > g.nye) continue;  // uniform within a group
+    if (i <= g.nxge && j <= g.nye) {  // uniform within a group
     // stayers: sx[m][u] Jx at x face m, y node u - 1; sy[m][u] Jy at y face
     // m, x node u - 1; sz[v][u] Jz at x node u - 1, y node v - 1
     double sx[2][3], sy[2][3], sz[3][3];
@@ -570,11 +574,17 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
             for (int u = 0; u < 3; u++) sz[v][u] = fma(s0x[u + 1], A, fma(dsx[u + 1], B, sz[v][u]));
           }
         } else {
-          const int slot = wave_enqueue(&nxq);
-          if (slot < NXQ2)
-            xq[slot] = make_int2(pi, cc | (s << 8));
-          else
+          const int slot = wave_enqueue(nq_r);
+          if (slot < NXQR) {
+            xqd[slot][0] = x;
+            xqd[slot][1] = y;
+            xqd[slot][2] = nxp;
+            xqd[slot][3] = nyp;
+            xqd[slot][4] = qvz;
+            xqc[slot] = cc | (s << 8);
+          } else {
             deposit2_full(acc, li, lj, s0x, s0y, dsx, dsy, qk, qvz);
+          }
         }
 
         if (FUSED) {
@@ -603,34 +613,22 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
         atomicAdd(&acc[comp][(lj + 2 + yo) * ACW + (li + 2 + xo)], v);
       }
     }
+    }  // the cell
+    __syncthreads();
+    // this round's queued particles that changed cell
+    const int nq = *nq_r < NXQR ? *nq_r : NXQR;
+    for (int q = tid; q < nq; q += NT) {
+      const int e = xqc[q];
+      const int qcc = e & 255, s = e >> 8;
+      const int qli = qcc % TX, qlj = qcc / TX;
+      double s0x[5], s0y[5], dsx[5], dsy[5];
+      esirkepov_shapes(g, xqd[q][0], xqd[q][1], xqd[q][2], xqd[q][3], i0 + qli, j0 + qlj, s0x, s0y, dsx, dsy);
+      deposit2_full(acc, qli, qlj, s0x, s0y, dsx, dsy, spc[s][3], xqd[q][4]);
+    }
+    if (tid == 0) nxq[(rnd + 1) & 1] = 0;
+    __syncthreads();
   }
 
-  __syncthreads();
-  // the queued particles that changed cell: the push again (the same bits, its
-  // stores already made) and the full block
-  const int nq = nxq < NXQ2 ? nxq : NXQ2;
-  // thread t drains entries [t per, (t + 1) per): the lanes of a wave work on
-  // entries far apart, mostly of different cells, so their atomics rarely meet
-  const int per = (nq + NT - 1) / NT;
-  for (int it = 0; it < per; it++) {
-    const int k = tid * per + it;
-    if (k >= nq) break;
-    const int2 e = xq[k];
-    const int pi = e.x, cc = e.y & 255, s = e.y >> 8;
-    const int li = cc % TX, lj = cc / TX;
-    const int i = i0 + li, j = j0 + lj;
-    const double* kc = spc[s];
-    const int ia = a.ap ? a.ap[pi] : pi;
-    const double x = a.A.x[ia], y = a.A.y[ia];
-    double nxp, nyp, g3, g4, g5, gmv;
-    push_reflect<FUSED ? BC : WPIC_BC_PERIODIC>(g, sf, li, lj, i, j, unit_c, kc, x, y, a.A.ux[ia], a.A.uy[ia],
-                                                a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
-    double s0x[5], s0y[5], dsx[5], dsy[5];
-    esirkepov_shapes(g, x, y, nxp, nyp, i, j, s0x, s0y, dsx, dsy);
-    deposit2_full(acc, li, lj, s0x, s0y, dsx, dsy, kc[3], kc[4] * (g5 * gmv));
-  }
-
-  __syncthreads();
   const size_t pl = (size_t)g.fw * g.fh;
   for (int k = tid; k < 3 * ACW * ACH; k += NT) {
     const int comp = k / (ACW * ACH), rem = k - comp * (ACW * ACH);

@@ -469,6 +469,64 @@ __device__ inline void deposit3_window(const Grid3D& g, const Parts3& A, const P
   }
 }
 
+// The same window from the particle's old and new in-cell shape arguments
+// (dh = x/dx - 0.5 - cell per axis, field.f90:253-266) and its cell increments.
+template <class Add>
+__device__ inline void deposit3_window_dh(const double dh0[3], const double dh1[3], const int inc[3], double qd,
+                                          Add add) {
+  double S0[3][4], DS[3][4], cq[3][3];
+  int lo[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double s0m, s00, s0p, s11, s12, s13;
+    shape3(dh0[d], s0m, s00, s0p);
+    shape3(dh1[d], s11, s12, s13);
+    const int in = inc[d];
+    const double smo1 = in < 0 ? (double)(-in) : 0.0;
+    const double smo2 = (double)(-abs(in) + 1);
+    const double smo3 = in > 0 ? (double)in : 0.0;
+    const double f0 = s11 * smo1 - 0e0;
+    const double f1 = s11 * smo2 + s12 * smo1 - s0m;
+    const double f2 = s12 * smo2 + s13 * smo1 + s11 * smo3 - s00;
+    const double f3 = s13 * smo2 + s12 * smo3 - s0p;
+    const double f4 = s13 * smo3 - 0e0;
+    const bool left = in < 0;
+    lo[d] = left ? 0 : 1;
+    S0[d][0] = left ? 0.0 : s0m;
+    S0[d][1] = left ? s0m : s00;
+    S0[d][2] = left ? s00 : s0p;
+    S0[d][3] = left ? s0p : 0.0;
+    DS[d][0] = left ? f0 : f1;
+    DS[d][1] = left ? f1 : f2;
+    DS[d][2] = left ? f2 : f3;
+    DS[d][3] = left ? f3 : f4;
+    double run = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      run = run - DS[d][m];
+      cq[d][m] = run * qd;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const double ax = fma(0.5, DS[0][u], S0[0][u]), bxu = fma(1.0 / 3.0, DS[0][u], 0.5 * S0[0][u]);
+    const double ay = fma(0.5, DS[1][u], S0[1][u]), byu = fma(1.0 / 3.0, DS[1][u], 0.5 * S0[1][u]);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const double wx = fma(ay, S0[2][v], byu * DS[2][v]);  // y node u, z node v
+      const double wy = fma(ax, S0[2][v], bxu * DS[2][v]);  // x node u, z node v
+      const double wz = fma(ax, S0[1][v], bxu * DS[1][v]);  // x node u, y node v
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        const double vx = cq[0][m] * wx, vy = cq[1][m] * wy, vz = cq[2][m] * wz;
+        if (vx != 0.0) add(0, lo[0] + m - 1, lo[1] + u - 2, lo[2] + v - 2, vx);
+        if (vy != 0.0) add(1, lo[0] + u - 2, lo[1] + m - 1, lo[2] + v - 2, vy);
+        if (vz != 0.0) add(2, lo[0] + u - 2, lo[1] + v - 2, lo[2] + m - 1, vz);
+      }
+    }
+  }
+}
+
 // Fast mode, push and deposit in one pass over each particle.  The push is the
 // shared bit-exact boris_push3; the deposit uses the factored weights of
 // k_deposit3d's fast path,
@@ -485,7 +543,7 @@ __device__ inline void deposit3_window(const Grid3D& g, const Parts3& A, const P
 // tile is added to uj with one HBM atomic per node.
 namespace p3f {
 constexpr int G = 8;       // lanes per cell
-constexpr int NXQ = 2048;  // queued cell-changing particles per tile (beyond: a global list)
+constexpr int NXQ = 512;   // queued cell-changing particles per round (beyond: a global list)
 }
 
 __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) {
@@ -494,8 +552,11 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
   __shared__ double sf[6][SN];
   __shared__ double acc[3][AN];
   __shared__ double spc[WPIC_MAXSP][4];  // fac1, txxx, fac2, q delx / delt
-  __shared__ int2 xq[p3f::NXQ];  // slot, cell in the tile | species << 8
-  __shared__ int nxq;
+  // this round's cell-changing particles: old and new shape arguments per
+  // axis, and cell in the tile | species << 7 | (inc + 1) per axis << 9, 11, 13
+  __shared__ double xqd[p3f::NXQ][6];
+  __shared__ int xqc[p3f::NXQ];
+  __shared__ int nxq[2];
   using p3f::NXQ;
   const Grid3D& g = a.g;
   const int ntiles = a.ntx * a.nty * a.ntz;
@@ -510,7 +571,7 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
     if (i <= g.nxge + 1 && j <= g.nye + 1 && k <= g.nze + 1) sf[c][rem] = tmpf3(g, a.uf, c, i, j, k);
   }
   for (int q = tid; q < 3 * AN; q += NT) (&acc[0][0])[q] = 0.0;
-  if (tid == 0) nxq = 0;
+  if (tid == 0) nxq[0] = nxq[1] = 0;
 #pragma unroll
   for (int s = 0; s < WPIC_MAXSP; s++) {
     if (tid == s && s < g.nsp) {
@@ -527,10 +588,13 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
   const int grp = tid / GF, lg = tid % GF;
   constexpr int NGRP = NT / GF;
   const double d_delx = g.d_delx, delt = g.delt, c = g.c;
-  for (int cc = grp; cc < TX * TY * TZ; cc += NGRP) {
+  // rounds of NGRP cells (one per group), each followed by the queue's drain
+  for (int rnd = 0; rnd < (TX * TY * TZ) / NGRP; rnd++) {
+    int* const nq_r = &nxq[rnd & 1];
+    const int cc = rnd * NGRP + grp;
     const int li = cc % TX, lj = (cc / TX) % TY, lk = cc / (TX * TY);
     const int i = i0 + li, j = j0 + lj, k = k0 + lk;
-    if (i > g.nxge || j > g.nye || k > g.nze) continue;  // uniform within a group
+    if (i <= g.nxge && j <= g.nye && k <= g.nze) {  // uniform within a group
     const int col = (j - g.nys) + g.ny * (k - g.nzs);
     CellList cl;
     int ntot = 0;
@@ -578,14 +642,14 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
       // new shape and cell increment per axis (field.f90:253-329)
       const double qd = kc[3];
       const int cell[3] = {i, j, k};
-      double s1[3][3];
+      double s1[3][3], dh1[3];
       int inc[3];
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         const int i2 = (int)(nw[d] * d_delx);
-        const double dh = nw[d] * d_delx - 5e-1 - i2;
+        dh1[d] = nw[d] * d_delx - 5e-1 - i2;
         inc[d] = i2 - cell[d];
-        shape3(dh, s1[d][0], s1[d][1], s1[d][2]);
+        shape3(dh1[d], s1[d][0], s1[d][1], s1[d][2]);
       }
       if ((inc[0] | inc[1] | inc[2]) == 0) {
         // nodes -1..1 of each axis: s0, ds = s1 - s0, a = s0 + ds/2, b = s0/2 + ds/3
@@ -619,9 +683,15 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
           }
       } else {
         // a particle that changes cell: its window is added after the cell loop
-        const int slot = wave_enqueue(&nxq);
+        const int slot = wave_enqueue(nq_r);
         if (slot < NXQ) {
-          xq[slot] = make_int2(pc, cc | (sc << 8));
+          const double o[3] = {x, y, z};
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            xqd[slot][d] = o[d] * d_delx - 5e-1 - cell[d];
+            xqd[slot][3 + d] = dh1[d];
+          }
+          xqc[slot] = cc | (sc << 7) | ((inc[0] + 1) << 9) | ((inc[1] + 1) << 11) | ((inc[2] + 1) << 13);
         } else {
           // beyond the LDS queue: listed for k_deposit3d_overflow
           const int o = atomicAdd(a.nov, 1);
@@ -650,27 +720,25 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
         atomicAdd(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
       }
     }
-  }
+    }  // the cell
 
-  __syncthreads();
-  // the queued particles that changed cell
-  const int nq = nxq < NXQ ? nxq : NXQ;
-  // thread t drains entries [t per, (t + 1) per): the lanes of a wave work on
-  // entries far apart, mostly of different cells, so their atomics rarely meet
-  const int per = (nq + NT - 1) / NT;
-  for (int it = 0; it < per; it++) {
-    const int q = tid * per + it;
-    if (q >= nq) break;
-    const int2 e = xq[q];
-    const int cc = e.y & 255, s = e.y >> 8;
-    const int li = cc % TX, lj = (cc / TX) % TY, lk = cc / (TX * TY);
-    const int cell[3] = {i0 + li, j0 + lj, k0 + lk};
-    const int bx = li + 2, by = lj + 2, bz = lk + 2;
-    deposit3_window(g, a.A, a.B, a.ap, e.x, cell, spc[s][3], [&](int comp, int xo, int yo, int zo, double v) {
-      atomicAdd(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
-    });
+    __syncthreads();
+    // this round's queued particles that changed cell; thread t drains entries
+    // t, t + NT, ...
+    const int nq = *nq_r < NXQ ? *nq_r : NXQ;
+    for (int q = tid; q < nq; q += NT) {
+      const int e = xqc[q];
+      const int qcc = e & 127, s = (e >> 7) & 3;
+      const int inc[3] = {((e >> 9) & 3) - 1, ((e >> 11) & 3) - 1, ((e >> 13) & 3) - 1};
+      const int qbx = qcc % TX + 2, qby = (qcc / TX) % TY + 2, qbz = qcc / (TX * TY) + 2;
+      const double dh0[3] = {xqd[q][0], xqd[q][1], xqd[q][2]}, dh1[3] = {xqd[q][3], xqd[q][4], xqd[q][5]};
+      deposit3_window_dh(dh0, dh1, inc, spc[s][3], [&](int comp, int xo, int yo, int zo, double v) {
+        atomicAdd(&acc[comp][((qbz + zo) * AH + (qby + yo)) * AW + (qbx + xo)], v);
+      });
+    }
+    if (tid == 0) nxq[(rnd + 1) & 1] = 0;
+    __syncthreads();
   }
-  __syncthreads();
   const size_t pl = plane3(g);
   for (int q = tid; q < 3 * AN; q += NT) {
     const int comp = q / AN, rem = q - comp * AN;

@@ -400,6 +400,7 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   if (p->capacity <= 0 || p->capacity * p->nsp >= (1ll << 31))
     return set_error(WPIC_EINVAL, "capacity must be positive and nsp*capacity below 2^31");
   if ((long long)ny * nz >= (1ll << 27)) return set_error(WPIC_EINVAL, "too many (j, k) columns");
+  if ((long long)nx * ny * nz >= (1ll << 31)) return set_error(WPIC_EINVAL, "too many cells (2^31 at most)");
   if (p->mode != WPIC_MODE_FAST && p->mode != WPIC_MODE_PARITY && p->mode != WPIC_MODE_MIXED)
     return set_error(WPIC_EINVAL, "bad mode");
 
