@@ -909,3 +909,135 @@ void or2d_gauss(const or2d_params *p, double *G, const double *uf, const double 
 #undef RHO
   free(rho);
 }
+
+/* ---------------------------------------------------------------------------
+ * Moments.  mom(7, nxgs-1:nxge+1, nys-1:nye+1, nsp): N, Vx, Vy, Vz, Txx, Tyy,
+ * Tzz at the CIC nodes (i+1/2, j+1/2).
+ * ------------------------------------------------------------------------- */
+static inline size_t iMOM(const or2d_params *p, int k, int i, int j, int isp) {
+  return (size_t)(k - 1) +
+         7u * ((size_t)(i - (p->nxgs - 1)) + (size_t)(NX + 2) * ((size_t)(j - (p->nys - 1)) + (size_t)(NY + 2) * (size_t)(isp - 1)));
+}
+
+/* mom_calc__accl: 2d/common/mom_calc.f90:48-164.  The half-step Boris kick of
+ * particle__solv with mom_calc's own delt = delt_in*0.5 (mom_calc.f90:34) and
+ * no move: gp(1:2) = up(1:2), gp(3:5) = the kicked momenta.  gp(6) is not
+ * written (the reference's gp is intent(out) there). */
+void or2d_mom_accl(const or2d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt) {
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int tw = NX + 2;
+  const double d_delx = 1. / p->delx;
+  const double delt = p->delt * 0.5, c = p->c;
+  double *tmp = (double *)malloc(sizeof(double) * 6u * (size_t)tw * (size_t)(NY + 2));
+#define TMP(cc, i, j) tmp[(size_t)((cc)-1) + 6u * ((size_t)((i) - (nxs - 1)) + (size_t)tw * (size_t)((j) - (nys - 1)))]
+#define UF(cc, i, j) uf[iF6(p, cc, i, j)]
+  /* mom_calc.f90:68-81 */
+  for (int j = nys - 1; j <= nye + 1; j++)
+    for (int i = nxs - 1; i <= nxe + 1; i++) {
+      TMP(1, i, j) = 0.5 * (+UF(1, i, j) + UF(1, i, j + 1));
+      TMP(2, i, j) = 0.5 * (+UF(2, i, j) + UF(2, i + 1, j));
+      TMP(3, i, j) = 0.25 * (+UF(3, i, j) + UF(3, i + 1, j) + UF(3, i, j + 1) + UF(3, i + 1, j + 1));
+      TMP(4, i, j) = 0.5 * (+UF(4, i, j) + UF(4, i + 1, j));
+      TMP(5, i, j) = 0.5 * (+UF(5, i, j) + UF(5, i, j + 1));
+      TMP(6, i, j) = UF(6, i, j);
+    }
+  /* mom_calc.f90:83-162 */
+  for (int j = nys; j <= nye; j++)
+    for (int i = nxs; i <= nxe; i++)
+      for (int isp = 1; isp <= p->nsp; isp++) {
+        const double q = p->q[isp - 1], r = p->r[isp - 1];
+        const double fac1 = q / r * 0.5 * delt;
+        const double txxx = fac1 * fac1;
+        const double fac2 = q * delt / r;
+        for (int ii = cumcnt[iCC(p, i, j, isp)] + 1; ii <= cumcnt[iCC(p, i + 1, j, isp)]; ii++) {
+          const double *u = &up[iUP(p, 1, ii, j, isp)];
+          double *g = &gp[iUP(p, 1, ii, j, isp)];
+          double sx[3], sy[3];
+          shape3(u[0] * d_delx - 0.5 - i, sx);
+          shape3(u[1] * d_delx - 0.5 - j, sy);
+#define GATH(cc)                                                                                   \
+  (+(+TMP(cc, i - 1, j - 1) * sx[0] + TMP(cc, i, j - 1) * sx[1] + TMP(cc, i + 1, j - 1) * sx[2]) * sy[0] + \
+   (+TMP(cc, i - 1, j) * sx[0] + TMP(cc, i, j) * sx[1] + TMP(cc, i + 1, j) * sx[2]) * sy[1] +             \
+   (+TMP(cc, i - 1, j + 1) * sx[0] + TMP(cc, i, j + 1) * sx[1] + TMP(cc, i + 1, j + 1) * sx[2]) * sy[2])
+          const double bpx = GATH(1), bpy = GATH(2), bpz = GATH(3);
+          const double epx = GATH(4), epy = GATH(5), epz = GATH(6);
+#undef GATH
+          double uvm1 = u[2] + fac1 * epx;
+          double uvm2 = u[3] + fac1 * epy;
+          double uvm3 = u[4] + fac1 * epz;
+          const double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+          const double igam = 1. / gam;
+          const double fac1r = fac1 * igam;
+          const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+          const double uvm4 = uvm1 + fac1r * (+uvm2 * bpz - uvm3 * bpy);
+          const double uvm5 = uvm2 + fac1r * (+uvm3 * bpx - uvm1 * bpz);
+          const double uvm6 = uvm3 + fac1r * (+uvm1 * bpy - uvm2 * bpx);
+          uvm1 = uvm1 + fac2r * (+uvm5 * bpz - uvm6 * bpy);
+          uvm2 = uvm2 + fac2r * (+uvm6 * bpx - uvm4 * bpz);
+          uvm3 = uvm3 + fac2r * (+uvm4 * bpy - uvm5 * bpx);
+          g[0] = u[0];
+          g[1] = u[1];
+          g[2] = uvm1 + fac1 * epx;
+          g[3] = uvm2 + fac1 * epy;
+          g[4] = uvm3 + fac1 * epz;
+        }
+      }
+#undef TMP
+#undef UF
+  free(tmp);
+}
+
+/* mom_calc__nvt: 2d/common/mom_calc.f90:167-249, serial in the reference's
+ * (isp, j, ii) order (its OMP REDUCTION at one thread: each entry starts at
+ * +0 and takes the particles' terms in that order).  ih = int(), which
+ * truncates toward zero; the grid keeps x > 1/2 so that it is floor(). */
+void or2d_mom_nvt(const or2d_params *p, double *mom, const double *up, const int *np2) {
+  const double d_delx = 1. / p->delx, c = p->c;
+  memset(mom, 0, sizeof(double) * 7u * (size_t)(NX + 2) * (size_t)(NY + 2) * (size_t)p->nsp);
+  for (int isp = 1; isp <= p->nsp; isp++)
+    for (int j = p->nys; j <= p->nye; j++)
+      for (int ii = 1; ii <= np2[iNP(p, j, isp)]; ii++) {
+        const double *u = &up[iUP(p, 1, ii, j, isp)];
+        const int ih = (int)(u[0] * d_delx - 0.5);
+        const int jh = (int)(u[1] * d_delx - 0.5);
+        const double dx = u[0] * d_delx - 0.5 - ih;
+        const double dxm = 1. - dx;
+        const double dy = u[1] * d_delx - 0.5 - jh;
+        const double dym = 1. - dy;
+        const double gam = 1. / sqrt(1.0 + (+u[2] * u[2] + u[3] * u[3] + u[4] * u[4]) / (c * c));
+        const double w[4] = {dxm * dym, dx * dym, dxm * dy, dx * dy};
+        const int ni[4] = {ih, ih + 1, ih, ih + 1}, nj[4] = {jh, jh, jh + 1, jh + 1};
+        const double wx[4] = {dxm, dx, dxm, dx}, wy[4] = {dym, dym, dy, dy};
+        for (int n = 0; n < 4; n++) {
+          double *m = &mom[iMOM(p, 1, ni[n], nj[n], isp)];
+          m[0] = m[0] + w[n];
+          for (int d = 0; d < 3; d++) m[1 + d] = m[1 + d] + u[2 + d] * gam * wx[n] * wy[n];
+          for (int d = 0; d < 3; d++) m[4 + d] = m[4 + d] + u[2 + d] * u[2 + d] * gam * wx[n] * wy[n];
+        }
+      }
+}
+
+/* boundary_periodic__mom (2d/common/boundary_periodic.f90:571-636) and
+ * boundary_reconnection__mom (2d/proj/reconnection/boundary_reconnection.f90:
+ * 582-650) at one rank: the x fold over all rows (periodic: the ghost node
+ * nxge+1 onto nxgs and nxgs-1 onto nxge; walls: each ghost onto its own
+ * edge), then per species the y fold, nys-1 onto nye and then nye+1 onto nys,
+ * over all x nodes. */
+void or2d_bc_mom(const or2d_params *p, double *mom) {
+  const int xa = p->bc == OR_BC_WALLX ? p->nxgs - 1 : p->nxge + 1;
+  const int xb = p->bc == OR_BC_WALLX ? p->nxge + 1 : p->nxgs - 1;
+  for (int isp = 1; isp <= p->nsp; isp++)
+    for (int j = p->nys - 1; j <= p->nye + 1; j++)
+      for (int k = 1; k <= 7; k++) {
+        mom[iMOM(p, k, p->nxgs, j, isp)] = mom[iMOM(p, k, p->nxgs, j, isp)] + mom[iMOM(p, k, xa, j, isp)];
+        mom[iMOM(p, k, p->nxge, j, isp)] = mom[iMOM(p, k, p->nxge, j, isp)] + mom[iMOM(p, k, xb, j, isp)];
+      }
+  for (int isp = 1; isp <= p->nsp; isp++) {
+    for (int i = p->nxgs - 1; i <= p->nxge + 1; i++)
+      for (int k = 1; k <= 7; k++)
+        mom[iMOM(p, k, i, p->nye, isp)] = mom[iMOM(p, k, i, p->nye, isp)] + mom[iMOM(p, k, i, p->nys - 1, isp)];
+    for (int i = p->nxgs - 1; i <= p->nxge + 1; i++)
+      for (int k = 1; k <= 7; k++)
+        mom[iMOM(p, k, i, p->nys, isp)] = mom[iMOM(p, k, i, p->nys, isp)] + mom[iMOM(p, k, i, p->nye + 1, isp)];
+  }
+}

@@ -791,3 +791,138 @@ void or3d_gauss(const or3d_params *p, double *G, const double *uf, const double 
       }
   free(rho);
 }
+
+/* ---------------------------------------------------------------------------
+ * Moments.  mom(7, nxgs-1:nxge+1, nys-1:nye+1, nzs-1:nze+1, nsp).
+ * ------------------------------------------------------------------------- */
+static inline size_t iMOM(const or3d_params *p, int l, int i, int j, int k, int isp) {
+  return (size_t)(l - 1) +
+         7u * ((size_t)(i - (p->nxgs - 1)) +
+               (size_t)(NX + 2) * ((size_t)(j - (p->nys - 1)) +
+                                   (size_t)(NY + 2) * ((size_t)(k - (p->nzs - 1)) + (size_t)(NZ + 2) * (size_t)(isp - 1))));
+}
+
+/* mom_calc__accl: 3d/common/mom_calc.f90:49-216, delt = delt_in*5d-1
+ * (mom_calc.f90:37); gp(1:3) = up(1:3), gp(4:6) kicked, gp(7) not written */
+void or3d_mom_accl(const or3d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt) {
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye, nzs = p->nzs, nze = p->nze;
+  const double d_delx = 1e0 / p->delx, delt = p->delt * 5e-1, c = p->c;
+  const int tx = NX + 2, ty = NY + 2;
+  double *tmpf = (double *)malloc(sizeof(double) * 6u * (size_t)tx * ty * (NZ + 2));
+#define TF(cc, i, j, k) tmpf[(size_t)((cc)-1) + 6u * ((size_t)((i) - (nxs - 1)) + (size_t)tx * ((size_t)((j) - (nys - 1)) + (size_t)ty * (size_t)((k) - (nzs - 1))))]
+  for (int k = nzs - 1; k <= nze + 1; k++)
+    for (int j = nys - 1; j <= nye + 1; j++)
+      for (int i = nxs - 1; i <= nxe + 1; i++) {
+        TF(1, i, j, k) = 2.5e-1 * (+UF(1, i, j, k) + UF(1, i, j + 1, k) + UF(1, i, j, k + 1) + UF(1, i, j + 1, k + 1));
+        TF(2, i, j, k) = 2.5e-1 * (+UF(2, i, j, k) + UF(2, i + 1, j, k) + UF(2, i, j, k + 1) + UF(2, i + 1, j, k + 1));
+        TF(3, i, j, k) = 2.5e-1 * (+UF(3, i, j, k) + UF(3, i + 1, j, k) + UF(3, i, j + 1, k) + UF(3, i + 1, j + 1, k));
+        TF(4, i, j, k) = 5e-1 * (+UF(4, i, j, k) + UF(4, i + 1, j, k));
+        TF(5, i, j, k) = 5e-1 * (+UF(5, i, j, k) + UF(5, i, j + 1, k));
+        TF(6, i, j, k) = 5e-1 * (+UF(6, i, j, k) + UF(6, i, j, k + 1));
+      }
+  for (int k = nzs; k <= nze; k++)
+    for (int j = nys; j <= nye; j++)
+      for (int i = nxs; i <= nxe; i++)
+        for (int isp = 1; isp <= p->nsp; isp++) {
+          const double q = p->q[isp - 1], r = p->r[isp - 1];
+          const double fac1 = q / r * 5e-1 * delt;
+          const double txxx = fac1 * fac1;
+          const double fac2 = q * delt / r;
+          for (int ii = cumcnt[iCC(p, i, j, k, isp)] + 1; ii <= cumcnt[iCC(p, i + 1, j, k, isp)]; ii++) {
+            const double *u = &up[iUP(p, 1, ii, j, k, isp)];
+            double *g = &gp[iUP(p, 1, ii, j, k, isp)];
+            double shxm, shx, shxp, shym, shy, shyp, shzm, shz, shzp;
+            shape3(u[0] * d_delx - 5e-1 - i, &shxm, &shx, &shxp);
+            shape3(u[1] * d_delx - 5e-1 - j, &shym, &shy, &shyp);
+            shape3(u[2] * d_delx - 5e-1 - k, &shzm, &shz, &shzp);
+#define RX(cc, jj, kk) (+TF(cc, i - 1, jj, kk) * shxm + TF(cc, i, jj, kk) * shx + TF(cc, i + 1, jj, kk) * shxp)
+#define RY(cc, kk) (+RX(cc, j - 1, kk) * shym + RX(cc, j, kk) * shy + RX(cc, j + 1, kk) * shyp)
+#define G3(cc) (RY(cc, k - 1) * shzm + RY(cc, k) * shz + RY(cc, k + 1) * shzp)
+            const double bpx = G3(1), bpy = G3(2), bpz = G3(3), epx = G3(4), epy = G3(5), epz = G3(6);
+#undef G3
+#undef RY
+#undef RX
+            double uvm1 = u[3] + fac1 * epx;
+            double uvm2 = u[4] + fac1 * epy;
+            double uvm3 = u[5] + fac1 * epz;
+            const double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+            const double igam = 1e0 / gam;
+            const double fac1r = fac1 * igam;
+            const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+            const double uvm4 = uvm1 + fac1r * (+uvm2 * bpz - uvm3 * bpy);
+            const double uvm5 = uvm2 + fac1r * (+uvm3 * bpx - uvm1 * bpz);
+            const double uvm6 = uvm3 + fac1r * (+uvm1 * bpy - uvm2 * bpx);
+            uvm1 = uvm1 + fac2r * (+uvm5 * bpz - uvm6 * bpy);
+            uvm2 = uvm2 + fac2r * (+uvm6 * bpx - uvm4 * bpz);
+            uvm3 = uvm3 + fac2r * (+uvm4 * bpy - uvm5 * bpx);
+            g[0] = u[0];
+            g[1] = u[1];
+            g[2] = u[2];
+            g[3] = uvm1 + fac1 * epx;
+            g[4] = uvm2 + fac1 * epy;
+            g[5] = uvm3 + fac1 * epz;
+          }
+        }
+#undef TF
+  free(tmpf);
+}
+
+/* mom_calc__nvt: 3d/common/mom_calc.f90:219-333 (serial in the reference:
+ * isp, k, j, ii).  floor() cell of the lower node, and the offset is taken
+ * from the position itself, up(1)-5d-1-ih, without d_delx (:243-251). */
+void or3d_mom_nvt(const or3d_params *p, double *mom, const double *up, const int *np2) {
+  const double d_delx = 1e0 / p->delx, c = p->c;
+  memset(mom, 0, sizeof(double) * 7u * (size_t)(NX + 2) * (NY + 2) * (NZ + 2) * p->nsp);
+  for (int isp = 1; isp <= p->nsp; isp++)
+    for (int k = p->nzs; k <= p->nze; k++)
+      for (int j = p->nys; j <= p->nye; j++)
+        for (int ii = 1; ii <= np2[iNP(p, j, k, isp)]; ii++) {
+          const double *u = &up[iUP(p, 1, ii, j, k, isp)];
+          const int ih = (int)floor(u[0] * d_delx - 5e-1);
+          const int jh = (int)floor(u[1] * d_delx - 5e-1);
+          const int kh = (int)floor(u[2] * d_delx - 5e-1);
+          const double dx = u[0] - 5e-1 - ih, dxm = 1e0 - dx;
+          const double dy = u[1] - 5e-1 - jh, dym = 1e0 - dy;
+          const double dz = u[2] - 5e-1 - kh, dzm = 1e0 - dz;
+          const double gam = 1e0 / sqrt(1e0 + (+u[3] * u[3] + u[4] * u[4] + u[5] * u[5]) / (c * c));
+          for (int n = 0; n < 8; n++) {
+            const int ox = n & 1, oy = (n >> 1) & 1, oz = n >> 2;
+            const double wx = ox ? dx : dxm, wy = oy ? dy : dym, wz = oz ? dz : dzm;
+            double *m = &mom[iMOM(p, 1, ih + ox, jh + oy, kh + oz, isp)];
+            m[0] = m[0] + wx * wy * wz;
+            for (int d = 0; d < 3; d++) m[1 + d] = m[1 + d] + u[3 + d] * gam * wx * wy * wz;
+            for (int d = 0; d < 3; d++) m[4 + d] = m[4 + d] + u[3 + d] * u[3 + d] * gam * wx * wy * wz;
+          }
+        }
+}
+
+/* boundary_periodic__mom: 3d/common/boundary_periodic.f90:1102-1235 at one
+ * rank.  x fold over everything, then per species the y fold (nys-1 onto
+ * nye, then nye+1 onto nys) over all x nodes and z planes, then the z fold
+ * (nzs-1 onto nze, then nze+1 onto nzs) over all x nodes and interior y. */
+void or3d_bc_mom(const or3d_params *p, double *mom) {
+  const int xs = p->nxgs, xe = p->nxge, ys = p->nys, ye = p->nye, zs = p->nzs, ze = p->nze;
+#define M(l, i, j, k, s) mom[iMOM(p, l, i, j, k, s)]
+  for (int s = 1; s <= p->nsp; s++)
+    for (int k = zs - 1; k <= ze + 1; k++)
+      for (int j = ys - 1; j <= ye + 1; j++)
+        for (int l = 1; l <= 7; l++) {
+          M(l, xs, j, k, s) = M(l, xs, j, k, s) + M(l, xe + 1, j, k, s);
+          M(l, xe, j, k, s) = M(l, xe, j, k, s) + M(l, xs - 1, j, k, s);
+        }
+  for (int s = 1; s <= p->nsp; s++) {
+    for (int k = zs - 1; k <= ze + 1; k++)
+      for (int i = xs - 1; i <= xe + 1; i++)
+        for (int l = 1; l <= 7; l++) M(l, i, ye, k, s) = M(l, i, ye, k, s) + M(l, i, ys - 1, k, s);
+    for (int k = zs - 1; k <= ze + 1; k++)
+      for (int i = xs - 1; i <= xe + 1; i++)
+        for (int l = 1; l <= 7; l++) M(l, i, ys, k, s) = M(l, i, ys, k, s) + M(l, i, ye + 1, k, s);
+    for (int j = ys; j <= ye; j++)
+      for (int i = xs - 1; i <= xe + 1; i++)
+        for (int l = 1; l <= 7; l++) M(l, i, j, ze, s) = M(l, i, j, ze, s) + M(l, i, j, zs - 1, s);
+    for (int j = ys; j <= ye; j++)
+      for (int i = xs - 1; i <= xe + 1; i++)
+        for (int l = 1; l <= 7; l++) M(l, i, j, zs, s) = M(l, i, j, zs, s) + M(l, i, j, ze + 1, s);
+  }
+#undef M
+}

@@ -112,6 +112,9 @@ def lib():
         L.or2d_step.argtypes = [P, S, dp, dp, dp, ip, ip, C.c_int]
         L.or2d_energy.argtypes = [P, dp, dp, ip, np.ctypeslib.ndpointer(np.float64)]
         L.or2d_gauss.argtypes = [P, dp, dp, dp, ip]
+        L.or2d_mom_accl.argtypes = [P, dp, dp, dp, ip]
+        L.or2d_mom_nvt.argtypes = [P, dp, dp, ip]
+        L.or2d_bc_mom.argtypes = [P, dp]
         P3 = C.POINTER(Params3D)
         L.or3d_fstate_init.argtypes = [P3, S]
         L.or3d_fstate_free.argtypes = [S]
@@ -129,6 +132,9 @@ def lib():
         L.or3d_step.argtypes = [P3, S, dp, dp, dp, ip, ip, C.c_int]
         L.or3d_energy.argtypes = [P3, dp, dp, ip, np.ctypeslib.ndpointer(np.float64)]
         L.or3d_gauss.argtypes = [P3, dp, dp, dp, ip]
+        L.or3d_mom_accl.argtypes = [P3, dp, dp, dp, ip]
+        L.or3d_mom_nvt.argtypes = [P3, dp, dp, ip]
+        L.or3d_bc_mom.argtypes = [P3, dp]
         for f in ("or_add_rd", "or_mul_rd", "or_div_rd"):
             getattr(L, f).argtypes = [C.c_double, C.c_double]
             getattr(L, f).restype = C.c_double
@@ -236,10 +242,23 @@ class Oracle2D:
         self.L.or2d_gauss(C.byref(self.p), G, uf, up, np2)
         return G
 
+    def mom_accl(self, gp, up, uf, cumcnt):
+        self.L.or2d_mom_accl(C.byref(self.p), gp, up, uf, cumcnt)
+
+    def mom_nvt(self, up, np2):
+        """mom(7, nxgs-1:nxge+1, nys-1:nye+1, nsp) of up's first np2 slots."""
+        mom = np.zeros((7, self.p.nx + 2, self.p.ny + 2, self.p.nsp), order="F")
+        self.L.or2d_mom_nvt(C.byref(self.p), mom, up, np2)
+        return mom
+
+    def bc_mom(self, mom):
+        self.L.or2d_bc_mom(C.byref(self.p), mom)
+
 
 def run_ref2d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
-    """Run the reference's own particle__solv (mode 1) or sort__bucket (mode 2)
-    through oracle/_ref/ref_golden2d on raw files in case_dir."""
+    """Run the reference's own particle__solv (mode 1), sort__bucket (mode 2) or
+    mom_calc__accl + mom_calc__nvt (mode 3: returns gp, mom) through
+    oracle/_ref/ref_golden2d on raw files in case_dir."""
     os.makedirs(case_dir, exist_ok=True)
     with open(os.path.join(case_dir, "params.txt"), "w") as f:
         f.write(f"{mode} {p.ndim} {p.np} {p.nsp} {p.nxgs} {p.nxge} {p.nygs} {p.nyge} {p.nys} {p.nye}\n")
@@ -247,16 +266,20 @@ def run_ref2d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
         f.write(" ".join(repr(float(p.q[k])) for k in range(p.nsp)) + "\n")
         f.write(" ".join(repr(float(p.r[k])) for k in range(p.nsp)) + "\n")
     up.ravel(order="F").tofile(os.path.join(case_dir, "up.bin"))
-    if mode == 1:
+    if mode in (1, 3):
         uf.ravel(order="F").tofile(os.path.join(case_dir, "uf.bin"))
         cumcnt.ravel(order="F").tofile(os.path.join(case_dir, "cumcnt.bin"))
-    else:
+    if mode in (2, 3):
         np2.ravel(order="F").tofile(os.path.join(case_dir, "np2.bin"))
     env = dict(os.environ, OMP_NUM_THREADS="1")
     subprocess.run([REF_DRIVER2D, case_dir], check=True, env=env)
     gp = np.fromfile(os.path.join(case_dir, "gp.bin"), dtype=np.float64).reshape(up.shape, order="F")
     if mode == 1:
         return gp
+    if mode == 3:
+        mom = np.fromfile(os.path.join(case_dir, "mom.bin"), dtype=np.float64).reshape(
+            (7, p.nx + 2, p.ny + 2, p.nsp), order="F")
+        return gp, mom
     cc = np.fromfile(os.path.join(case_dir, "cumcnt.bin"), dtype=np.int32).reshape(
         (p.nx + 1, p.ny, p.nsp), order="F")
     return gp, cc
@@ -366,14 +389,28 @@ class Oracle3D:
         self.L.or3d_gauss(C.byref(self.p), G, uf, up, np2)
         return G
 
+    def mom_accl(self, gp, up, uf, cumcnt):
+        self.L.or3d_mom_accl(C.byref(self.p), gp, up, uf, cumcnt)
+
+    def mom_nvt(self, up, np2):
+        """mom(7, nxgs-1:nxge+1, nys-1:nye+1, nzs-1:nze+1, nsp)."""
+        p = self.p
+        mom = np.zeros((7, p.nx + 2, p.ny + 2, p.nz + 2, p.nsp), order="F")
+        self.L.or3d_mom_nvt(C.byref(p), mom, up, np2)
+        return mom
+
+    def bc_mom(self, mom):
+        self.L.or3d_bc_mom(C.byref(self.p), mom)
+
 
 def have_ref3d():
     return os.path.exists(REF_DRIVER3D)
 
 
 def run_ref3d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
-    """Run the reference's own 3D particle__solv (mode 1) or sort__bucket
-    (mode 2) through oracle/_ref/ref_golden3d on raw files in case_dir."""
+    """Run the reference's own 3D particle__solv (mode 1), sort__bucket (mode 2)
+    or mom_calc__accl + mom_calc__nvt (mode 3: returns gp, mom) through
+    oracle/_ref/ref_golden3d on raw files in case_dir."""
     os.makedirs(case_dir, exist_ok=True)
     with open(os.path.join(case_dir, "params.txt"), "w") as f:
         f.write(f"{mode} {p.ndim} {p.np} {p.nsp} {p.nxgs} {p.nxge} {p.nygs} {p.nyge} {p.nzgs} {p.nzge} "
@@ -382,16 +419,20 @@ def run_ref3d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
         f.write(" ".join(repr(float(p.q[k])) for k in range(p.nsp)) + "\n")
         f.write(" ".join(repr(float(p.r[k])) for k in range(p.nsp)) + "\n")
     up.ravel(order="F").tofile(os.path.join(case_dir, "up.bin"))
-    if mode == 1:
+    if mode in (1, 3):
         uf.ravel(order="F").tofile(os.path.join(case_dir, "uf.bin"))
         cumcnt.ravel(order="F").tofile(os.path.join(case_dir, "cumcnt.bin"))
-    else:
+    if mode in (2, 3):
         np2.ravel(order="F").tofile(os.path.join(case_dir, "np2.bin"))
     env = dict(os.environ, OMP_NUM_THREADS="1")
     subprocess.run([REF_DRIVER3D, case_dir], check=True, env=env)
     gp = np.fromfile(os.path.join(case_dir, "gp.bin"), dtype=np.float64).reshape(up.shape, order="F")
     if mode == 1:
         return gp
+    if mode == 3:
+        mom = np.fromfile(os.path.join(case_dir, "mom.bin"), dtype=np.float64).reshape(
+            (7, p.nx + 2, p.ny + 2, p.nz + 2, p.nsp), order="F")
+        return gp, mom
     cc = np.fromfile(os.path.join(case_dir, "cumcnt.bin"), dtype=np.int32).reshape(
         (p.nx + 1, p.ny, p.nz, p.nsp), order="F")
     return gp, cc

@@ -1,5 +1,5 @@
 ! ref_golden3d.f90 -- golden-vector driver for the reference's own 3D
-! `particle` and `sort` modules (compiled from the reference sources by
+! `particle`, `sort` and `mom_calc` modules (compiled from the reference sources by
 ! oracle/Makefile into oracle/_ref/).  TEST INFRASTRUCTURE ONLY.
 !
 ! Usage: ref_golden3d <case-dir>
@@ -7,15 +7,19 @@
 !                           nys nye nzs nze / delx delt c / q(1:nsp) / r(1:nsp)
 !   mode 1 (push):  in  up.bin uf.bin cumcnt.bin      out gp.bin
 !   mode 2 (sort):  in  up.bin np2.bin                out gp.bin cumcnt.bin
+!   mode 3 (mom):   in  up.bin uf.bin cumcnt.bin np2.bin
+!                   out gp.bin (mom_calc__accl) mom.bin (mom_calc__nvt of that gp)
 ! Arrays are raw little-endian stream files in the reference's own layout.
 program ref_golden3d
   use particle, only: particle__init, particle__solv
   use sort, only: sort__init, sort__bucket
+  use mom_calc, only: mom_calc__init, mom_calc__accl, mom_calc__nvt
   implicit none
   character(len=512) :: dir
   integer :: mode, ndim, np, nsp, nxgs, nxge, nygs, nyge, nzgs, nzge, nys, nye, nzs, nze, u
   real(8) :: delx, delt, c
   real(8), allocatable :: q(:), r(:), up(:,:,:,:,:), gp(:,:,:,:,:), uf(:,:,:,:)
+  real(8), allocatable :: mom(:,:,:,:,:)
   integer, allocatable :: cumcnt(:,:,:,:), np2(:,:,:)
 
   call get_command_argument(1, dir)
@@ -39,6 +43,17 @@ program ref_golden3d
      call particle__init(ndim,np,nsp,nxgs,nxge,nygs,nyge,nzgs,nzge,nys,nye,nzs,nze,delx,delt,c,q,r)
      call particle__solv(gp,up,uf,cumcnt,nxgs,nxge)
      call write_r8(trim(dir)//'/gp.bin', gp, size(gp))
+  else if (mode == 3) then
+     allocate(uf(6,nxgs-2:nxge+2,nys-2:nye+2,nzs-2:nze+2))
+     allocate(mom(7,nxgs-1:nxge+1,nys-1:nye+1,nzs-1:nze+1,nsp))
+     call read_r8(trim(dir)//'/uf.bin', uf, size(uf))
+     call read_i4(trim(dir)//'/cumcnt.bin', cumcnt, size(cumcnt))
+     call read_i4(trim(dir)//'/np2.bin', np2, size(np2))
+     call mom_calc__init(ndim,np,nsp,nxgs,nxge,nygs,nyge,nzgs,nzge,nys,nye,nzs,nze,delx,delt,c,q,r)
+     call mom_calc__accl(gp,up,uf,cumcnt,nxgs,nxge)
+     call mom_calc__nvt(mom,gp,np2)
+     call write_r8(trim(dir)//'/gp.bin', gp, size(gp))
+     call write_r8(trim(dir)//'/mom.bin', mom, size(mom))
   else
      call read_i4(trim(dir)//'/np2.bin', np2, size(np2))
      call sort__init(ndim,np,nsp,nxgs,nxge,nygs,nyge,nzgs,nzge,nys,nye,nzs,nze)

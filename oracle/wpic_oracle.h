@@ -27,6 +27,14 @@
  *                         operator; vacuum wave energy; div B / Gauss law.
  *   or2d_bc_*             brute-force periodic indexing on tiny grids, the
  *                         round-down wrap edge case, wall sign tables.
+ *   or2d_mom_accl, or3d_mom_accl, or2d_mom_nvt, or3d_mom_nvt
+ *                         bit-exact vs the reference mom_calc.f90 (2D and 3D)
+ *                         built with amdflang (oracle/_ref); nvt also against
+ *                         a brute-force periodic CIC deposit and sum rules.
+ *   or2d_bc_mom, or3d_bc_mom
+ *                         the folded moments equal a deposit with periodic
+ *                         (or mirrored-wall) node indices computed directly;
+ *                         interior totals conserve the ghost sums.
  *   or2d_bc_particle_y    particle multiset / id conservation, row membership;
  *                         the exact in-row order after hole filling is
  *                         "parity unpinned" (only a reference build with MPI
@@ -107,6 +115,13 @@ void or2d_energy(const or2d_params *p, const double *up, const double *uf,
  * same quadratic shape as the deposit (s0 x s0).  G(nxgs:nxge, nys:nye). */
 void or2d_gauss(const or2d_params *p, double *G, const double *uf,
                 const double *up, const int *np2);
+/* moments: mom(7, nxgs-1:nxge+1, nys-1:nye+1, nsp) = N, Vx, Vy, Vz, Txx,
+ * Tyy, Tzz.  mom_calc__accl (2d/common/mom_calc.f90:48-164), mom_calc__nvt
+ * (:167-249), boundary_periodic__mom (boundary_periodic.f90:571-636) or
+ * boundary_reconnection__mom (boundary_reconnection.f90:582-650) by p->bc. */
+void or2d_mom_accl(const or2d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt);
+void or2d_mom_nvt(const or2d_params *p, double *mom, const double *up, const int *np2);
+void or2d_bc_mom(const or2d_params *p, double *mom);
 
 /* ---------------------------------------------------------------------------
  * 3D (the 3d/common modules), periodic in x, y and z at one rank.
@@ -160,6 +175,12 @@ int or3d_step(const or3d_params *p, or3d_fstate *st, double *up, double *gp, dou
 void or3d_energy(const or3d_params *p, const double *up, const double *uf, const int *np2, double out[3]);
 /* G(nx, ny, nz) = div E - 4 pi rho at cell centres */
 void or3d_gauss(const or3d_params *p, double *G, const double *uf, const double *up, const int *np2);
+/* moments: mom(7, nxgs-1:nxge+1, nys-1:nye+1, nzs-1:nze+1, nsp).
+ * 3d/common/mom_calc.f90:49-216 (accl), :219-333 (nvt),
+ * 3d/common/boundary_periodic.f90:1102-1235 (bc) */
+void or3d_mom_accl(const or3d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt);
+void or3d_mom_nvt(const or3d_params *p, double *mom, const double *up, const int *np2);
+void or3d_bc_mom(const or3d_params *p, double *mom);
 
 /* round-toward-minus-infinity arithmetic used by the periodic wraps
  * (ieee_set_rounding_mode(ieee_down), boundary_periodic.f90:74,124) */

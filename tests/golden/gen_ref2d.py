@@ -1,11 +1,12 @@
-"""Write golden vectors from the reference's OWN 2D particle__solv and
-sort__bucket (2d/common/particle.f90:48-179, 2d/common/sort.f90:36-82),
+"""Write golden vectors from the reference's OWN 2D particle__solv,
+sort__bucket and mom_calc (2d/common/particle.f90:48-179,
+2d/common/sort.f90:36-82, 2d/common/mom_calc.f90:48-249),
 compiled from the reference sources by `make -C oracle ref` and run at one
 OpenMP thread.
 
     python tests/golden/gen_ref2d.py        (needs oracle/_ref/ref_golden2d)
 
-Outputs tests/golden/ref2d_push.npz and tests/golden/ref2d_sort.npz: the
+Outputs tests/golden/ref2d_push.npz, ref2d_sort.npz and ref2d_mom.npz: the
 inputs, the parameters and the reference's outputs.  The fixtures are data
 only; tests/test_oracle_golden.py checks the oracle against them.
 """
@@ -76,10 +77,32 @@ def sort_case(tmp):
                         **params_dict(p))
 
 
+def mom_case(tmp):
+    """mom_calc__accl then mom_calc__nvt on a sorted state (2d/proj/weibel/app.f90:121-122)."""
+    rng = np.random.default_rng(20261204)
+    nx, ny, n_ppc = 12, 10, 8
+    q, r = S.weibel_consts(n_ppc)
+    p = po.make_params(nx, ny, np_cap=160, delt=0.5, q=q, r=r)
+    counts = [rng.integers(4, 13, size=(nx, ny)) for _ in range(2)]
+    up, cumcnt, np2 = S.sorted_particles(p, n_ppc, rng, u_scale=(0.05, 0.4), drift=(0.0, 0.5),
+                                         colocate=False, counts=counts)
+    # a particle on a cell's left face, one on a cell centre (dx = 0 in the
+    # CIC weights), one just below the domain's top face, a relativistic one
+    up[0, 0, 0, 0] = float(p.nxgs)
+    up[0:2, 1, 2, 0] = [float(p.nxgs) + 0.5 + float(int(up[0, 1, 2, 0] - p.nxgs)), float(p.nys + 2) + 0.5]
+    up[1, 2, ny - 1, 1] = np.nextafter(float(p.nye + 1), 0.0)
+    up[2:5, 3, 4, 1] = [40.0, -25.0, 7.5]
+    uf = S.random_fields(p, rng, scale=0.3, b0=0.2)
+    gp, mom = po.run_ref2d(os.path.join(tmp, "mom"), 3, p, up, np2=np2, uf=uf, cumcnt=cumcnt)
+    np.savez_compressed(os.path.join(HERE, "ref2d_mom.npz"), up=up, uf=uf, cumcnt=cumcnt, np2=np2,
+                        gp=gp, mom=mom, **params_dict(p))
+
+
 if __name__ == "__main__":
     if not po.have_ref():
         sys.exit("oracle/_ref/ref_golden2d is missing: run `make -C oracle ref` first")
     with tempfile.TemporaryDirectory() as tmp:
         push_case(tmp)
         sort_case(tmp)
-    print("wrote", os.path.join(HERE, "ref2d_push.npz"), os.path.join(HERE, "ref2d_sort.npz"))
+        mom_case(tmp)
+    print("wrote", *(os.path.join(HERE, f"ref2d_{k}.npz") for k in ("push", "sort", "mom")))

@@ -1,11 +1,12 @@
-"""Write golden vectors from the reference's OWN 3D particle__solv and
-sort__bucket (3d/common/particle.f90:52-233, 3d/common/sort.f90:40-88),
+"""Write golden vectors from the reference's OWN 3D particle__solv,
+sort__bucket and mom_calc (3d/common/particle.f90:52-233,
+3d/common/sort.f90:40-88, 3d/common/mom_calc.f90:49-333),
 compiled from the reference sources by `make -C oracle ref` and run at one
 OpenMP thread.
 
     python tests/golden/gen_ref3d.py        (needs oracle/_ref/ref_golden3d)
 
-Outputs tests/golden/ref3d_push.npz and tests/golden/ref3d_sort.npz: the
+Outputs tests/golden/ref3d_push.npz, ref3d_sort.npz and ref3d_mom.npz: the
 inputs, the parameters and the reference's outputs.  The fixtures are data
 only; tests/test_oracle3d.py checks the oracle against them.
 """
@@ -78,10 +79,29 @@ def sort_case(tmp):
                         **params_dict(p))
 
 
+def mom_case(tmp):
+    """mom_calc__accl then mom_calc__nvt on a sorted 3D state."""
+    rng = np.random.default_rng(20261205)
+    nx, ny, nz, n_ppc = 8, 6, 5, 4
+    q, r = S.weibel_consts(n_ppc)
+    p = po.make_params3d(nx, ny, nz, np_cap=64, delt=0.5, q=q, r=r)
+    counts = [rng.integers(1, 7, size=(nx, ny, nz)) for _ in range(2)]
+    up, cumcnt, np2 = S.sorted_particles3d(p, n_ppc, rng, u_scale=(0.05, 0.4), drift=(0.0, 0.5),
+                                           colocate=False, counts=counts)
+    up[0:3, 0, 0, 0, 0] = [float(p.nxgs), float(p.nys) + 0.999999999999, float(p.nzs) + 0.5]
+    up[3:6, 0, 4, 2, 0] = [-3.0, 2.0, -1.0]
+    up[3:6, 2, 3, 1, 1] = [40.0, -25.0, 7.5]
+    uf = S.random_fields3d(p, rng, scale=0.3, b0=0.2)
+    gp, mom = po.run_ref3d(os.path.join(tmp, "mom"), 3, p, up, np2=np2, uf=uf, cumcnt=cumcnt)
+    np.savez_compressed(os.path.join(HERE, "ref3d_mom.npz"), up=up, uf=uf, cumcnt=cumcnt, np2=np2,
+                        gp=gp, mom=mom, **params_dict(p))
+
+
 if __name__ == "__main__":
     if not po.have_ref3d():
         sys.exit("oracle/_ref/ref_golden3d is missing: run `make -C oracle ref` first")
     with tempfile.TemporaryDirectory() as tmp:
         push_case(tmp)
         sort_case(tmp)
-    print("wrote", os.path.join(HERE, "ref3d_push.npz"), os.path.join(HERE, "ref3d_sort.npz"))
+        mom_case(tmp)
+    print("wrote", *(os.path.join(HERE, f"ref3d_{k}.npz") for k in ("push", "sort", "mom")))
