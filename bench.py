@@ -55,6 +55,11 @@ PUSH3D_BYTES = 112
 DEPOSIT3D_BYTES = 48
 # 3D mixed form: the 36-byte record (cell, 3 offsets, 3 momenta, id) each way
 PUSH3D_BYTES_MIXED = 72
+# moments (fused kick + CIC deposit): the 40-byte record without its id read
+# once through the 4-byte permutation entry; nothing per particle is written
+MOM_BYTES = 44
+# 3D: the 48-byte record and the permutation entry
+MOM3D_BYTES = 52
 
 
 def parse():
@@ -141,6 +146,20 @@ def timed_run(sim, warmup, steps):
     return time.perf_counter() - t0
 
 
+def moments_run(sim, npart, nbytes, reps=5):
+    """the output-cadence moments (mom_calc__accl + __nvt + bc__mom fused) on
+    the state the timed steps left, outside the timed region"""
+    import torch
+    for _ in range(reps):
+        sim.moments()
+    torch.cuda.synchronize()
+    ms, n = sim.timing("moments")
+    avg = ms / max(n, 1)
+    return {"ms": avg, "launches": n, "bytes_per_particle": nbytes,
+            "achieved_GBs": nbytes * npart / (avg / 1e3) / 1e9,
+            "what": "mom_calc__accl, mom_calc__nvt and bc__mom in one pass (kick fused into the deposit)"}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -207,6 +226,7 @@ def main3d(a, w):
     ph = {name: sim.timing(name) for name in names}
     ph = {k: v for k, v in ph.items() if v[1] > 0}
     cg = sim.cg_iters()
+    moments = moments_run(sim, npart, MOM3D_BYTES)
     sim.set_timing(False)
     energy64 = sim.energy()
     uf64 = sim.download_fields() if not a.no_mixed else None
@@ -226,6 +246,7 @@ def main3d(a, w):
         "phase_ms_per_step": {k: v[0] / a.steps for k, v in ph.items()},
         **rates,
         "cg_iters_last_step": list(cg),
+        "moments": moments,
     }
     if not a.no_mixed:
         out["mixed"] = mixed_run(w, a, npart, elapsed, uf64, energy64)
@@ -262,6 +283,7 @@ def main():
     fld_ms, fld_n = sim.timing("field")
     phases = {name: sim.timing(name)[0] / a.steps for name in ("push_deposit", "field", "cg", "exchange", "sort")}
     cg_iters = sim.cg_iters()
+    moments = moments_run(sim, npart, MOM_BYTES)
     sim.set_timing(False)
     energy = sim.energy()
     uf64 = sim.download_fields()
@@ -297,6 +319,7 @@ def main():
         "field_ms_per_step": fld_ms / max(fld_n, 1),
         "phase_ms_per_step": phases,
         "cg_iters_last_step": list(cg_iters),
+        "moments": moments,
         "energy_final": {"kinetic": energy[0], "E": energy[1], "B": energy[2]},
         "cpu_baseline": None,
         "mixed": mixed,

@@ -122,6 +122,32 @@ int wpic_gauss2d(wpic_ctx *ctx, double *G);
 /* particle counts per species on the device */
 int wpic_counts2d(wpic_ctx *ctx, long long *counts);
 
+/* Moments mom(7, nxgs-1:nxge+1, nys-1:nye+1, nsp) real(8): N, Vx, Vy, Vz,
+ * Txx, Tyy, Tzz at the nodes (i+1/2, j+1/2), held on the device.
+ * mom_calc__init (2d/common/mom_calc.f90:18-45) takes particle__init's
+ * arguments, so the context carries its state.
+ * wpic_mom_accl2d:  mom_calc__accl (mom_calc.f90:48-164): gp := up with the
+ *                   momenta kicked over delt/2, positions kept, slot-aligned.
+ * wpic_mom_nvt2d:   mom_calc__nvt (mom_calc.f90:167-249) of gp.  Bit-identical
+ *                   to the reference at one thread in WPIC_MODE_PARITY; LDS
+ *                   and HBM atomics (summation order free) in WPIC_MODE_FAST.
+ * wpic_bc_mom2d:    boundary_periodic__mom (boundary_periodic.f90:571-636) or
+ *                   boundary_reconnection__mom
+ *                   (2d/proj/reconnection/boundary_reconnection.f90:582-650),
+ *                   by the context's boundary kind.
+ * wpic_moments2d:   the app's output sequence accl, nvt, bc__mom
+ *                   (2d/proj/weibel/app.f90:121-123) in one pass over up
+ *                   (FAST: the kick fused into the deposit); gp is left
+ *                   undefined, as mom_calc__accl leaves it overwritten.
+ * Particles whose lower node falls outside their row or the grid give
+ * WPIC_EDOMAIN.  FP64 modes only. */
+int wpic_mom_accl2d(wpic_ctx *ctx);
+int wpic_mom_nvt2d(wpic_ctx *ctx);
+int wpic_bc_mom2d(wpic_ctx *ctx);
+int wpic_moments2d(wpic_ctx *ctx);
+int wpic_download_mom2d(wpic_ctx *ctx, double *mom);
+int wpic_upload_mom2d(wpic_ctx *ctx, const double *mom);
+
 /* seeded initial state ----------------------------------------------------- */
 /* Counter-based seeded two-species beam state: n_ppc particles per cell per
  * species at uniformly random in-cell positions, ions and electrons

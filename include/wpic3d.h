@@ -84,6 +84,18 @@ int wpic_energy3d(wpic_ctx3d *ctx, double out[3]);
 int wpic_gauss3d(wpic_ctx3d *ctx, double *G);
 int wpic_counts3d(wpic_ctx3d *ctx, long long *counts);
 
+/* Moments mom(7, nxgs-1:nxge+1, nys-1:nye+1, nzs-1:nze+1, nsp) real(8), as
+ * the 2D calls of wpic.h: mom_calc__accl (3d/common/mom_calc.f90:49-216),
+ * mom_calc__nvt (:219-333), boundary_periodic__mom
+ * (3d/common/boundary_periodic.f90:1102-1235) and the three in one pass
+ * (the output sequence of 3d/proj/weibel/app.f90:122-124). */
+int wpic_mom_accl3d(wpic_ctx3d *ctx);
+int wpic_mom_nvt3d(wpic_ctx3d *ctx);
+int wpic_bc_mom3d(wpic_ctx3d *ctx);
+int wpic_moments3d(wpic_ctx3d *ctx);
+int wpic_download_mom3d(wpic_ctx3d *ctx, double *mom);
+int wpic_upload_mom3d(wpic_ctx3d *ctx, const double *mom);
+
 /* seeded two-species beam state, as wpic_ic_beams2d in 3D (positions uniform
  * in each cell in x, y and z; ions and electrons co-located) */
 int wpic_ic_beams3d(wpic_ctx3d *ctx, unsigned long long seed, int n_ppc, double udrift,

@@ -1,6 +1,7 @@
 """The reference app's call sequence, written in Fortran against the shim
-modules (particle, field, sort, boundary_periodic, boundary_reconnection),
-run on the GPU: the result equals the oracle's bit for bit in parity mode,
+modules (particle, field, sort, mom_calc, boundary_periodic,
+boundary_reconnection), run on the GPU: the state after the loop and the
+moments of the output sequence equal the oracle's bit for bit in parity mode,
 with host mirroring after every call and in resident mode."""
 import os
 import subprocess
@@ -63,6 +64,7 @@ def test_fortran_app_loop_on_shim_matches_oracle(bc, resident):
         f_dev = np.fromfile(os.path.join(tmp, "uf.out")).reshape(uf.shape, order="F")
         c_dev = np.fromfile(os.path.join(tmp, "cumcnt.out"), dtype=np.int32).reshape(cc.shape, order="F")
         n_dev = np.fromfile(os.path.join(tmp, "np2.out"), dtype=np.int32).reshape(np2.shape, order="F")
+        m_dev = np.fromfile(os.path.join(tmp, "mom.out")).reshape((7, p.nx + 2, p.ny + 2, p.nsp), order="F")
     o = po.Oracle2D(p)
     gp = np.zeros_like(up, order="F")
     o.step(up, gp, uf, cc, np2, nsteps)
@@ -73,3 +75,8 @@ def test_fortran_app_loop_on_shim_matches_oracle(bc, resident):
             n = np2[j, s]
             assert np.array_equal(_bits(u_dev[:, :n, j, s]), _bits(up[:, :n, j, s])), (s, j)
     assert np.array_equal(_bits(f_dev), _bits(uf))
+    # the moment output sequence after the loop (app.f90:121-123)
+    o.mom_accl(gp, up, uf, cc)
+    mom = o.mom_nvt(gp, np2)
+    o.bc_mom(mom)
+    assert np.array_equal(_bits(m_dev), _bits(mom))

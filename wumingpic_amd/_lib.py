@@ -27,6 +27,8 @@ EXPORTS = (
     "wpic_sort_bucket2d", "wpic_cg_iters2d", "wpic_step2d",
     "wpic_energy2d", "wpic_gauss2d", "wpic_counts2d", "wpic_ic_beams2d",
     "wpic_set_timing", "wpic_timing",
+    "wpic_mom_accl2d", "wpic_mom_nvt2d", "wpic_bc_mom2d", "wpic_moments2d",
+    "wpic_download_mom2d", "wpic_upload_mom2d",
 )
 
 # every symbol include/wpic3d.h declares
@@ -39,6 +41,8 @@ EXPORTS3D = (
     "wpic_sort_bucket3d", "wpic_cg_iters3d", "wpic_step3d",
     "wpic_energy3d", "wpic_gauss3d", "wpic_counts3d", "wpic_ic_beams3d",
     "wpic_set_timing3d", "wpic_timing3d",
+    "wpic_mom_accl3d", "wpic_mom_nvt3d", "wpic_bc_mom3d", "wpic_moments3d",
+    "wpic_download_mom3d", "wpic_upload_mom3d",
 )
 
 
@@ -118,6 +122,12 @@ def lib():
         "wpic_ic_beams2d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
         "wpic_set_timing": [vp, C.c_int],
         "wpic_timing": [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
+        "wpic_mom_accl2d": [vp],
+        "wpic_mom_nvt2d": [vp],
+        "wpic_bc_mom2d": [vp],
+        "wpic_moments2d": [vp],
+        "wpic_download_mom2d": [vp, d],
+        "wpic_upload_mom2d": [vp, d],
         "wpic_set_cg_tol2d": [vp, C.c_double],
         "wpic_perturb_bz2d": [vp, C.c_double, C.c_int],
         "wpic_adj_forward2d": [vp, C.c_int, C.c_int, C.POINTER(C.c_double)],
@@ -150,6 +160,12 @@ def lib():
         "wpic_ic_beams3d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
         "wpic_set_timing3d": [vp, C.c_int],
         "wpic_timing3d": [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
+        "wpic_mom_accl3d": [vp],
+        "wpic_mom_nvt3d": [vp],
+        "wpic_bc_mom3d": [vp],
+        "wpic_moments3d": [vp],
+        "wpic_download_mom3d": [vp, d],
+        "wpic_upload_mom3d": [vp, d],
     }
     for name, args in sig.items():
         f = getattr(L, name)

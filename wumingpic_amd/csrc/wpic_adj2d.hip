@@ -109,6 +109,7 @@ int restore_state(wpic_ctx* c, int s) {
   if (copy_d(c, c->df, a.ck_df[s], sizeof(double) * 6 * a.pl)) return WPIC_EHIP;
   c->h_rows_ok[c->ra] = false;
   c->b_valid = false;
+  c->b_accl = false;
   c->b_aligned = false;
   c->exch_pending = false;
   return 0;

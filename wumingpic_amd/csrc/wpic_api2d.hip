@@ -249,6 +249,7 @@ int sort_into_a(wpic_ctx* c) {
   std::swap(c->MA, c->MB);
   set_rows(c);
   c->b_valid = false;
+  c->b_accl = false;
   c->b_aligned = false;
   c->exch_pending = false;
   return 0;
@@ -266,6 +267,7 @@ int step_once(wpic_ctx* c) {
   int rc = deposit(c, true, true);
   if (rc) return rc;
   c->b_valid = true;
+  c->b_accl = false;
   c->b_aligned = true;
   rc = field_after_deposit(c);
   if (rc) return rc;
@@ -421,7 +423,7 @@ int wpic_destroy(wpic_ctx* c) {
   void* ps[] = {c->A.x, c->A.y, c->A.ux, c->A.uy, c->A.uz, c->B.x, c->B.y, c->B.ux, c->B.uy, c->B.uz,
                 c->id[0], c->id[1], c->code, c->R, c->holes, c->ulist, c->dlist, c->rows[0], c->rows[1],
                 c->rows[2], c->cumcnt, c->rowcnt, c->aosbase, c->uf, c->df, c->uj, c->pjbuf, c->phi, c->pa,
-                c->pb, c->rr, c->bb, c->ap, c->part, c->scal, c->err, c->staging, c->gbuf, c->red,
+                c->pb, c->rr, c->bb, c->ap, c->part, c->scal, c->err, c->staging, c->gbuf, c->red, c->mom,
                 c->MA.ix, c->MA.ox, c->MA.oy, c->MA.ux, c->MA.uy, c->MA.uz,
                 c->MB.ix, c->MB.ox, c->MB.oy, c->MB.ux, c->MB.uy, c->MB.uz, c->permbuf};
   for (void* q : ps)
@@ -547,6 +549,7 @@ int wpic_upload_up2d(wpic_ctx* c, const double* up, const int* np2, const int* c
   HIPCHK(hipMemcpyAsync(c->cumcnt, cumcnt, sizeof(int) * ncc, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->b_valid = false;
+  c->b_accl = false;
   c->b_aligned = false;
   c->exch_pending = false;
   return 0;
@@ -571,6 +574,7 @@ int wpic_upload_gp2d(wpic_ctx* c, const double* gp, const int* np2, int np) {
   int rc = upload_rows(c, c->rb, gp, np2, np, 1);
   if (rc) return rc;
   c->b_valid = true;
+  c->b_accl = false;
   c->exch_pending = false;
   // gp sits slot for slot on up when the rows agree (a field__fdtd_i input);
   // both use compact row bases, so equal counts mean equal bases
@@ -649,6 +653,7 @@ int wpic_particle_solv2d(wpic_ctx* c) {
     HIPCHK(launch_push_deposit(c, true, false, false));
   }
   c->b_valid = true;
+  c->b_accl = false;
   c->b_aligned = true;
   c->exch_pending = false;
   return check_err(c);
@@ -670,6 +675,7 @@ int wpic_bc_particle_x2d(wpic_ctx* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
   if (c->mixed) return mixed_only_step();
   if (!c->b_valid || c->exch_pending) return set_error(WPIC_EINVAL, "bc__particle_x needs gp before the y exchange");
+  c->b_accl = false;
   HIPCHK(launch_wrapx(c));
   return check_err(c);
 }
@@ -798,6 +804,7 @@ int wpic_ic_beams2d(wpic_ctx* c, unsigned long long seed, int n_ppc, double udri
   c->h_rows[c->ra] = hrows;
   c->h_rows_ok[c->ra] = true;
   c->b_valid = false;
+  c->b_accl = false;
   c->b_aligned = false;
   c->exch_pending = false;
   return 0;
@@ -826,3 +833,110 @@ int wpic_timing(wpic_ctx* c, const char* name, double* ms, long long* count) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// moments: mom_calc__accl / mom_calc__nvt (2d/common/mom_calc.f90:48-249) and
+// boundary_periodic__mom / boundary_reconnection__mom
+// ---------------------------------------------------------------------------
+static int ensure_mom(wpic_ctx* c) {
+  if (c->mom) return 0;
+  HIPCHK(dalloc(&c->mom, (size_t)7 * (c->g.nx + 2) * (c->g.ny + 2) * c->g.nsp));
+  HIPCHK(launch_zero(c, c->mom, (size_t)7 * (c->g.nx + 2) * (c->g.ny + 2) * c->g.nsp));
+  return 0;
+}
+
+int wpic_mom_accl2d(wpic_ctx* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
+  c->rb = c->ra;
+  if (c->rn == c->ra) c->rn = pick_free(c->ra, c->rb);
+  set_rows(c);
+  {
+    Timed t(c, "mom_accl");
+    HIPCHK(launch_mom_accl(c));
+  }
+  c->b_valid = true;
+  c->b_accl = true;  // gp holds up's positions: sorted by cell like up
+  c->b_aligned = true;
+  c->exch_pending = false;
+  return check_err(c);
+}
+
+int wpic_mom_nvt2d(wpic_ctx* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
+  if (!c->b_valid || c->exch_pending)
+    return set_error(WPIC_EINVAL, "mom_calc__nvt needs gp on the device (call mom_calc__accl first)");
+  int rc = ensure_mom(c);
+  if (rc) return rc;
+  set_rows(c);
+  const bool parity = c->mode == WPIC_MODE_PARITY;
+  // the tile deposit needs gp in up's rows (particles whose node lies off
+  // their tile go to HBM directly); the parity gather's cell-range shortcut
+  // needs gp sorted by cell like up, as mom_calc__accl leaves it
+  int kind;
+  if (parity)
+    kind = c->b_accl && c->b_aligned ? MOM_PARITY_ALIGNED : MOM_PARITY_ANY;
+  else
+    kind = c->b_aligned ? MOM_TILE_GP : MOM_ANY;
+  {
+    Timed t(c, "mom_nvt");
+    HIPCHK(launch_mom_deposit(c, kind));
+  }
+  return check_err(c);
+}
+
+int wpic_bc_mom2d(wpic_ctx* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  int rc = ensure_mom(c);
+  if (rc) return rc;
+  HIPCHK(launch_mom_bc(c));
+  return check_err(c);
+}
+
+int wpic_moments2d(wpic_ctx* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
+  int rc = ensure_mom(c);
+  if (rc) return rc;
+  {
+    Timed t(c, "moments");
+    if (c->mode == WPIC_MODE_PARITY) {
+      c->rb = c->ra;
+      if (c->rn == c->ra) c->rn = pick_free(c->ra, c->rb);
+      set_rows(c);
+      HIPCHK(launch_mom_accl(c));
+      HIPCHK(launch_mom_deposit(c, MOM_PARITY_ALIGNED));
+    } else {
+      set_rows(c);
+      HIPCHK(launch_mom_deposit(c, MOM_TILE_KICK));
+    }
+    HIPCHK(launch_mom_bc(c));
+  }
+  // the reference's sequence leaves gp overwritten by mom_calc__accl
+  c->b_valid = false;
+  c->b_accl = false;
+  c->b_aligned = false;
+  c->exch_pending = false;
+  return check_err(c);
+}
+
+int wpic_download_mom2d(wpic_ctx* c, double* mom) {
+  if (!c || !mom) return set_error(WPIC_EINVAL, "null argument");
+  int rc = ensure_mom(c);
+  if (rc) return rc;
+  const size_t n = (size_t)7 * (c->g.nx + 2) * (c->g.ny + 2) * c->g.nsp;
+  HIPCHK(hipMemcpyAsync(mom, c->mom, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int wpic_upload_mom2d(wpic_ctx* c, const double* mom) {
+  if (!c || !mom) return set_error(WPIC_EINVAL, "null argument");
+  int rc = ensure_mom(c);
+  if (rc) return rc;
+  const size_t n = (size_t)7 * (c->g.nx + 2) * (c->g.ny + 2) * c->g.nsp;
+  HIPCHK(hipMemcpyAsync(c->mom, mom, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}

@@ -219,6 +219,7 @@ int push3(wpic_ctx3d* c) {
     HIPCHK(launch_push3d(c));
   }
   c->b_valid = true;
+  c->b_accl = false;
   c->b_aligned = true;
   c->exch_pending = false;
   return 0;
@@ -239,6 +240,7 @@ int push_deposit3(wpic_ctx3d* c) {
       HIPCHK(launch_push_deposit3d(c));
   }
   c->b_valid = true;
+  c->b_accl = false;
   c->b_aligned = true;
   c->exch_pending = false;
   return 0;
@@ -270,6 +272,7 @@ int sort_into_a3(wpic_ctx3d* c) {
   std::swap(c->MA, c->MB);
   set_cols(c);
   c->b_valid = false;
+  c->b_accl = false;
   c->b_aligned = false;
   c->exch_pending = false;
   c->h_cols_ok[c->ca] = false;
@@ -537,7 +540,7 @@ int wpic_destroy3d(wpic_ctx3d* c) {
                 c->holes,  c->mlist, c->cols[0],  c->cols[1], c->cols[2], c->cumcnt, c->dcnt,   c->dstart,
                 c->colkm,  c->seg,   c->nseg,     c->aosbase, c->uf,     c->df,      c->uj,     c->pjbuf,
                 c->phi,    c->pa,    c->pb,       c->rr,      c->bb,     c->ap,      c->part,   c->scal,
-                c->err,    c->staging, c->gbuf,   c->red,     c->permbuf, c->nov,
+                c->err,    c->staging, c->gbuf,   c->red,     c->permbuf, c->nov,    c->mom,
                 c->MA.ix,  c->MA.ox, c->MA.oy,    c->MA.oz,   c->MA.ux,  c->MA.uy,  c->MA.uz,
                 c->MB.ix,  c->MB.ox, c->MB.oy,    c->MB.oz,   c->MB.ux,  c->MB.uy,  c->MB.uz};
   for (void* q : ps)
@@ -578,6 +581,7 @@ int wpic_upload_up3d(wpic_ctx3d* c, const double* up, const int* np2, const int*
   HIPCHK(hipMemcpyAsync(c->cumcnt, cumcnt, sizeof(int) * (size_t)nr * (nx + 1), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->b_valid = false;
+  c->b_accl = false;
   c->b_aligned = false;
   c->exch_pending = false;
   return 0;
@@ -602,6 +606,7 @@ int wpic_upload_gp3d(wpic_ctx3d* c, const double* gp, const int* np2, int np) {
   int rc = upload_cols(c, c->cb, gp, np2, np, 1);
   if (rc) return rc;
   c->b_valid = true;
+  c->b_accl = false;
   c->exch_pending = false;
   rc = fetch_cols(c, c->ca);
   if (rc) return rc;
@@ -666,6 +671,7 @@ int wpic_bc_particle_x3d(wpic_ctx3d* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
   if (c->mixed) return mixed_only_step3();
   if (!c->b_valid || c->exch_pending) return set_error(WPIC_EINVAL, "bc__particle_x needs gp before the exchange");
+  c->b_accl = false;
   HIPCHK(launch_wrapx3d(c));
   return check_err3(c);
 }
@@ -804,6 +810,7 @@ int wpic_ic_beams3d(wpic_ctx3d* c, unsigned long long seed, int n_ppc, double ud
   c->h_cols[c->ca] = hcols;
   c->h_cols_ok[c->ca] = true;
   c->b_valid = false;
+  c->b_accl = false;
   c->b_aligned = false;
   c->exch_pending = false;
   return 0;
@@ -832,3 +839,104 @@ int wpic_timing3d(wpic_ctx3d* c, const char* name, double* ms, long long* count)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// moments: mom_calc__accl / mom_calc__nvt (3d/common/mom_calc.f90:49-333) and
+// boundary_periodic__mom (3d/common/boundary_periodic.f90:1102-1235)
+// ---------------------------------------------------------------------------
+static int ensure_mom3(wpic_ctx3d* c) {
+  if (c->mom) return 0;
+  HIPCHK(dalloc(&c->mom, mom3_size(c->g)));
+  HIPCHK(launch_fill3d(c, c->mom, mom3_size(c->g), 0.0));
+  return 0;
+}
+
+int wpic_mom_accl3d(wpic_ctx3d* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step3();
+  c->cb = c->ca;
+  if (c->cn == c->ca) c->cn = pick_free3(c->ca, c->cb);
+  set_cols(c);
+  {
+    Timed3 t(c, "mom_accl");
+    HIPCHK(launch_mom_accl3d(c));
+  }
+  c->b_valid = true;
+  c->b_accl = true;  // gp holds up's positions: sorted by cell like up
+  c->b_aligned = true;
+  c->exch_pending = false;
+  return check_err3(c);
+}
+
+int wpic_mom_nvt3d(wpic_ctx3d* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step3();
+  if (!c->b_valid || c->exch_pending)
+    return set_error(WPIC_EINVAL, "mom_calc__nvt needs gp on the device (call mom_calc__accl first)");
+  int rc = ensure_mom3(c);
+  if (rc) return rc;
+  set_cols(c);
+  int kind;
+  if (c->mode == WPIC_MODE_PARITY)
+    kind = c->b_accl && c->b_aligned ? MOM_PARITY_ALIGNED : MOM_PARITY_ANY;
+  else
+    kind = c->b_aligned ? MOM_TILE_GP : MOM_ANY;
+  {
+    Timed3 t(c, "mom_nvt");
+    HIPCHK(launch_mom_deposit3d(c, kind));
+  }
+  return check_err3(c);
+}
+
+int wpic_bc_mom3d(wpic_ctx3d* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  int rc = ensure_mom3(c);
+  if (rc) return rc;
+  HIPCHK(launch_mom_bc3d(c));
+  return check_err3(c);
+}
+
+int wpic_moments3d(wpic_ctx3d* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step3();
+  int rc = ensure_mom3(c);
+  if (rc) return rc;
+  {
+    Timed3 t(c, "moments");
+    if (c->mode == WPIC_MODE_PARITY) {
+      c->cb = c->ca;
+      if (c->cn == c->ca) c->cn = pick_free3(c->ca, c->cb);
+      set_cols(c);
+      HIPCHK(launch_mom_accl3d(c));
+      HIPCHK(launch_mom_deposit3d(c, MOM_PARITY_ALIGNED));
+    } else {
+      set_cols(c);
+      HIPCHK(launch_mom_deposit3d(c, MOM_TILE_KICK));
+    }
+    HIPCHK(launch_mom_bc3d(c));
+  }
+  // the reference's sequence leaves gp overwritten by mom_calc__accl
+  c->b_valid = false;
+  c->b_accl = false;
+  c->b_aligned = false;
+  c->exch_pending = false;
+  return check_err3(c);
+}
+
+int wpic_download_mom3d(wpic_ctx3d* c, double* mom) {
+  if (!c || !mom) return set_error(WPIC_EINVAL, "null argument");
+  int rc = ensure_mom3(c);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(mom, c->mom, sizeof(double) * mom3_size(c->g), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int wpic_upload_mom3d(wpic_ctx3d* c, const double* mom) {
+  if (!c || !mom) return set_error(WPIC_EINVAL, "null argument");
+  int rc = ensure_mom3(c);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(c->mom, mom, sizeof(double) * mom3_size(c->g), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}

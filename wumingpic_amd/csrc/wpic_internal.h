@@ -205,6 +205,7 @@ struct wpic_ctx {
   int* rowcnt;                    // [nsp*ny][4] leavers, up, down, incoming
   bool b_valid;       // B holds a gp array
   bool b_aligned;     // B's slots are A's slots (after particle__solv)
+  bool b_accl;        // B is mom_calc__accl's gp: A's positions, A's cell order
   bool exch_pending;  // R / N rows describe B after bc__particle_y
   std::vector<int> h_rows[3];  // host mirrors of the row structures
   bool h_rows_ok[3];
@@ -225,6 +226,7 @@ struct wpic_ctx {
   size_t staging_n;
   long long* aosbase;  // [nsp*ny] row starts in a compact AoS transfer buffer
   double* gbuf;        // Gauss residual / rho scratch (lazy)
+  double* mom;         // moments mom(7, nxgs-1:nxge+1, nys-1:nye+1, nsp) (lazy)
   double* red;      // diagnostics reduction scratch
   size_t red_n;
 
@@ -274,6 +276,11 @@ hipError_t launch_ic_beams(wpic_ctx* c, unsigned long long seed, int n_ppc, doub
                            double uth_i);
 hipError_t launch_fill(wpic_ctx* c, double* p, size_t n, double v);
 hipError_t launch_gather_rows(wpic_ctx* c, const Parts& src, const int* ap, const Parts& dst, const int* rows);
+// moments (mom_calc.f90, boundary_*__mom): deposit kinds
+enum { MOM_TILE_KICK = 0, MOM_TILE_GP = 1, MOM_ANY = 2, MOM_PARITY_ALIGNED = 3, MOM_PARITY_ANY = 4 };
+hipError_t launch_mom_accl(wpic_ctx* c);
+hipError_t launch_mom_deposit(wpic_ctx* c, int kind);
+hipError_t launch_mom_bc(wpic_ctx* c);
 // mixed-precision form
 hipError_t launch_push_deposit_mixed(wpic_ctx* c);
 hipError_t launch_up_to_mixed(wpic_ctx* c, const double* aos);

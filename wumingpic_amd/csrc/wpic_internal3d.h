@@ -85,6 +85,7 @@ struct wpic_ctx3d {
   int* seg;     // [ncol][8] arrival segments (source column << 4 | dir), in order
   int* nseg;    // [ncol]
   bool b_valid, b_aligned, exch_pending;
+  bool b_accl;  // B is mom_calc__accl's gp: A's positions, A's cell order
   std::vector<int> h_cols[3];
   bool h_cols_ok[3];
 
@@ -102,6 +103,7 @@ struct wpic_ctx3d {
   long long* aosbase;  // [nsp*ncol]
   double* gbuf;
   double* red;
+  double* mom;  // mom(7, nxgs-1:nxge+1, nys-1:nye+1, nzs-1:nze+1, nsp) (lazy)
 
   bool timing;
   std::vector<wpic::TimerRec> timers;
@@ -113,6 +115,11 @@ namespace wpic {
 int set_error(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
+// moments (MOM_* deposit kinds of wpic_internal.h)
+size_t mom3_size(const Grid3D& g);
+hipError_t launch_mom_accl3d(wpic_ctx3d* c);
+hipError_t launch_mom_deposit3d(wpic_ctx3d* c, int kind);
+hipError_t launch_mom_bc3d(wpic_ctx3d* c);
 // push / deposit
 hipError_t launch_push3d(wpic_ctx3d* c);
 hipError_t launch_deposit3d(wpic_ctx3d* c);

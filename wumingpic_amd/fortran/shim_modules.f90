@@ -3,10 +3,12 @@
 !   particle:  particle__init, particle__solv      (2d/common/particle.f90:18,48)
 !   field:     field__init, field__fdtd_i          (2d/common/field.f90:22,66)
 !   sort:      sort__init, sort__bucket            (2d/common/sort.f90:16,36)
+!   mom_calc:  mom_calc__init, mom_calc__accl, mom_calc__nvt
+!                                                  (2d/common/mom_calc.f90:18,48,167)
 !   boundary_periodic / boundary_reconnection:
-!              *__init, *__particle_x, *__particle_y and the boundary
-!              procedures handed to field__fdtd_i  (boundary_periodic.f90:26-568,
-!              2d/proj/reconnection/boundary_reconnection.f90:26-579)
+!              *__init, *__particle_x, *__particle_y, *__mom and the boundary
+!              procedures handed to field__fdtd_i  (boundary_periodic.f90:26-636,
+!              2d/proj/reconnection/boundary_reconnection.f90:26-650)
 ! An app built against these modules instead of the reference's runs its
 ! per-step loop on one MI355X.  The boundary procedures are accepted by
 ! field__fdtd_i for source compatibility and applied on the device according
@@ -164,6 +166,72 @@ contains
 end module sort
 
 
+module mom_calc
+  use iso_c_binding
+  use wpic_capi
+  use wpic_shim_state
+  implicit none
+  private
+  public :: mom_calc__init, mom_calc__accl, mom_calc__nvt
+
+contains
+
+  ! mom_calc__init takes particle__init's arguments; its delt*0.5 is applied
+  ! on the device from the context's delt
+  subroutine mom_calc__init(ndim_in, np_in, nsp_in, nxgs_in, nxge_in, nygs_in, nyge_in, nys_in, nye_in, &
+                            delx_in, delt_in, c_in, q_in, r_in)
+    integer, intent(in) :: ndim_in, np_in, nsp_in
+    integer, intent(in) :: nxgs_in, nxge_in, nygs_in, nyge_in, nys_in, nye_in
+    real(8), intent(in) :: delx_in, delt_in, c_in, q_in(nsp_in), r_in(nsp_in)
+    call shim_set_grid(ndim_in, np_in, nsp_in, nxgs_in, nxge_in, nygs_in, nyge_in, nys_in, nye_in)
+    if (have_particle) then
+       if (prm%delt /= delt_in .or. prm%c /= c_in .or. prm%delx /= delx_in) then
+          write(6,*) 'wpic shim: mom_calc__init must use particle__init''s delx, delt and c'
+          stop
+       end if
+    end if
+    have_mom = .true.
+  end subroutine mom_calc__init
+
+  subroutine mom_calc__accl(gp, up, uf, cumcnt, nxs, nxe)
+    integer, intent(in)  :: nxs, nxe
+    integer, intent(in)  :: cumcnt(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp)
+    real(8), intent(in)  :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    real(8), intent(in)  :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
+    real(8), intent(out) :: gp(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    integer(c_int) :: np2(s_nys:s_nye, s_nsp)
+    if (.not. have_mom) then
+       write(6,*) 'Initialize first by calling mom_calc__init()'
+       stop
+    end if
+    np2 = cumcnt(nxe+1, :, :)
+    call shim_ctx(np2)
+    if (.not. resident) then
+       call shim_check(wpic_upload_up2d(ctx, up, np2, cumcnt, s_np))
+       call shim_check(wpic_upload_fields2d(ctx, uf))
+    end if
+    call shim_check(wpic_mom_accl2d(ctx))
+    if (.not. resident) call shim_check(wpic_download_gp2d(ctx, gp, np2, s_np))
+  end subroutine mom_calc__accl
+
+  subroutine mom_calc__nvt(mom, up, np2)
+    integer, intent(in)  :: np2(s_nys:s_nye, s_nsp)
+    real(8), intent(in)  :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    real(8), intent(out) :: mom(7, s_nxgs-1:s_nxge+1, s_nys-1:s_nye+1, s_nsp)
+    if (.not. have_mom) then
+       write(6,*) 'Initialize first by calling mom_calc__init()'
+       stop
+    end if
+    call shim_ctx(np2)
+    ! resident: the device's gp (from mom_calc__accl) is the argument
+    if (.not. resident) call shim_check(wpic_upload_gp2d(ctx, up, np2, s_np))
+    call shim_check(wpic_mom_nvt2d(ctx))
+    call shim_check(wpic_download_mom2d(ctx, mom))
+  end subroutine mom_calc__nvt
+
+end module mom_calc
+
+
 module boundary_periodic
   use iso_c_binding
   use wpic_capi
@@ -171,7 +239,7 @@ module boundary_periodic
   implicit none
   private
   public :: boundary_periodic__init, boundary_periodic__dfield, boundary_periodic__particle_x
-  public :: boundary_periodic__particle_y, boundary_periodic__curre, boundary_periodic__phi
+  public :: boundary_periodic__particle_y, boundary_periodic__curre, boundary_periodic__phi, boundary_periodic__mom
 
 contains
 
@@ -233,6 +301,19 @@ contains
     stop
   end subroutine boundary_periodic__phi
 
+  ! the moment folds on the device's moments; mirror mode sends the host
+  ! array up and brings the folded one back
+  subroutine boundary_periodic__mom(mom)
+    real(8), intent(inout) :: mom(7, s_nxgs-1:s_nxge+1, s_nys-1:s_nye+1, s_nsp)
+    if (.not. c_associated(ctx)) then
+       write(6,*) 'boundary_periodic__mom: no device state (call mom_calc__accl first)'
+       stop
+    end if
+    if (.not. resident) call shim_check(wpic_upload_mom2d(ctx, mom))
+    call shim_check(wpic_bc_mom2d(ctx))
+    call shim_check(wpic_download_mom2d(ctx, mom))
+  end subroutine boundary_periodic__mom
+
 end module boundary_periodic
 
 
@@ -243,7 +324,7 @@ module boundary_reconnection
   implicit none
   private
   public :: boundary_reconnection__init, boundary_reconnection__dfield, boundary_reconnection__particle_x
-  public :: boundary_reconnection__particle_y, boundary_reconnection__curre, boundary_reconnection__phi
+  public :: boundary_reconnection__particle_y, boundary_reconnection__curre, boundary_reconnection__phi, boundary_reconnection__mom
 
 contains
 
@@ -303,6 +384,19 @@ contains
     write(6,*) 'boundary_reconnection__phi: applied on the device inside field__fdtd_i'
     stop
   end subroutine boundary_reconnection__phi
+
+  ! the moment folds on the device's moments; mirror mode sends the host
+  ! array up and brings the folded one back
+  subroutine boundary_reconnection__mom(mom)
+    real(8), intent(inout) :: mom(7, s_nxgs-1:s_nxge+1, s_nys-1:s_nye+1, s_nsp)
+    if (.not. c_associated(ctx)) then
+       write(6,*) 'boundary_reconnection__mom: no device state (call mom_calc__accl first)'
+       stop
+    end if
+    if (.not. resident) call shim_check(wpic_upload_mom2d(ctx, mom))
+    call shim_check(wpic_bc_mom2d(ctx))
+    call shim_check(wpic_download_mom2d(ctx, mom))
+  end subroutine boundary_reconnection__mom
 
 end module boundary_reconnection
 

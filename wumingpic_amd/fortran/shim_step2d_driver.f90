@@ -9,6 +9,8 @@
 !               q(1:nsp)
 !               r(1:nsp)
 !   in:  up.bin uf.bin cumcnt.bin np2.bin     out: up.out uf.out cumcnt.out np2.out
+!   and, after the loop, the app's moment output sequence
+!   (2d/proj/weibel/app.f90:121-123): mom.out
 program shim_step2d_driver
   use iso_c_binding, only: c_int
   use wpic_capi, only: WPIC_MODE_FAST, WPIC_MODE_PARITY
@@ -16,19 +18,20 @@ program shim_step2d_driver
   use particle
   use field
   use sort
+  use mom_calc
   use boundary_periodic, only: bcp_init => boundary_periodic__init, bcp_x => boundary_periodic__particle_x, &
        bcp_y => boundary_periodic__particle_y, bcp_dfield => boundary_periodic__dfield, &
-       bcp_curre => boundary_periodic__curre, bcp_phi => boundary_periodic__phi
+       bcp_curre => boundary_periodic__curre, bcp_phi => boundary_periodic__phi, bcp_mom => boundary_periodic__mom
   use boundary_reconnection, only: bcr_init => boundary_reconnection__init, &
        bcr_x => boundary_reconnection__particle_x, bcr_y => boundary_reconnection__particle_y, &
        bcr_dfield => boundary_reconnection__dfield, bcr_curre => boundary_reconnection__curre, &
-       bcr_phi => boundary_reconnection__phi
+       bcr_phi => boundary_reconnection__phi, bcr_mom => boundary_reconnection__mom
   implicit none
   character(len=512) :: dir
   integer :: nsteps, bc, mode, res, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, nxs, nxe, u, it
   integer :: nstat(1)
   real(8) :: delx, delt, c, gfac
-  real(8), allocatable :: q(:), r(:), up(:,:,:,:), gp(:,:,:,:), uf(:,:,:)
+  real(8), allocatable :: q(:), r(:), up(:,:,:,:), gp(:,:,:,:), uf(:,:,:), mom(:,:,:,:)
   integer, allocatable :: cumcnt(:,:,:), np2(:,:)
 
   call get_command_argument(1, dir)
@@ -48,6 +51,7 @@ program shim_step2d_driver
   allocate(up(ndim,np,nys:nye,nsp), gp(ndim,np,nys:nye,nsp))
   allocate(uf(6,nxgs-2:nxge+2,nys-2:nye+2))
   allocate(cumcnt(nxgs:nxge+1,nys:nye,nsp), np2(nys:nye,nsp))
+  allocate(mom(7,nxgs-1:nxge+1,nys-1:nye+1,nsp))
   gp = 0.0d0
   call read_r8(trim(dir)//'/up.bin', up, size(up))
   call read_r8(trim(dir)//'/uf.bin', uf, size(uf))
@@ -63,6 +67,7 @@ program shim_step2d_driver
   call particle__init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c, q, r)
   call field__init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, 0, 0, 0, 0, delx, delt, c, q, r, gfac)
   call sort__init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye)
+  call mom_calc__init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c, q, r)
   if (res == 1) then
      call wpic_shim_resident(.true.)
      call wpic_shim_push(up, uf, np2, cumcnt)
@@ -81,6 +86,13 @@ program shim_step2d_driver
      end if
      call sort__bucket(up, gp, cumcnt, np2, nxs, nxe)
   end do
+  call mom_calc__accl(gp, up, uf, cumcnt, nxs, nxe)
+  call mom_calc__nvt(mom, gp, np2)
+  if (bc == 0) then
+     call bcp_mom(mom)
+  else
+     call bcr_mom(mom)
+  end if
   if (res == 1) call wpic_shim_pull(up, uf, np2, cumcnt)
   call wpic_shim_finalize()
 
@@ -88,6 +100,7 @@ program shim_step2d_driver
   call write_r8(trim(dir)//'/uf.out', uf, size(uf))
   call write_i4(trim(dir)//'/cumcnt.out', cumcnt, size(cumcnt))
   call write_i4(trim(dir)//'/np2.out', np2, size(np2))
+  call write_r8(trim(dir)//'/mom.out', mom, size(mom))
 
 contains
 

@@ -76,6 +76,38 @@ module wpic_capi
        real(c_double), intent(out) :: uf(*)
      end function wpic_download_fields2d
 
+     integer(c_int) function wpic_mom_accl2d(ctx) bind(C, name='wpic_mom_accl2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_mom_accl2d
+
+     integer(c_int) function wpic_mom_nvt2d(ctx) bind(C, name='wpic_mom_nvt2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_mom_nvt2d
+
+     integer(c_int) function wpic_bc_mom2d(ctx) bind(C, name='wpic_bc_mom2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_bc_mom2d
+
+     integer(c_int) function wpic_moments2d(ctx) bind(C, name='wpic_moments2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_moments2d
+
+     integer(c_int) function wpic_download_mom2d(ctx, mom) bind(C, name='wpic_download_mom2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: mom(*)
+     end function wpic_download_mom2d
+
+     integer(c_int) function wpic_upload_mom2d(ctx, mom) bind(C, name='wpic_upload_mom2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: mom(*)
+     end function wpic_upload_mom2d
+
      integer(c_int) function wpic_particle_solv2d(ctx) bind(C, name='wpic_particle_solv2d')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -18,7 +18,7 @@ module wpic_shim_state
 
   type(c_ptr), save :: ctx = c_null_ptr
   type(wpic_params2d), save :: prm
-  logical, save :: have_particle = .false., have_field = .false., have_bc = .false.
+  logical, save :: have_particle = .false., have_field = .false., have_bc = .false., have_mom = .false.
   logical, save :: resident = .false.
   integer, save :: s_ndim, s_np, s_nsp, s_nxgs, s_nxge, s_nygs, s_nyge, s_nys, s_nye
   integer(c_int), save :: s_mode = WPIC_MODE_FAST

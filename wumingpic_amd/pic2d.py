@@ -154,6 +154,32 @@ class Pic2D:
     def step(self, nsteps=1):
         check(self._L.wpic_step2d(self._h, int(nsteps)))
 
+    # -- moments (2d/common/mom_calc.f90, boundary_*__mom) ---------------------
+    def mom_accl(self):
+        """mom_calc__accl(gp, up, uf, cumcnt, nxs, nxe): mom_calc.f90:48"""
+        check(self._L.wpic_mom_accl2d(self._h))
+
+    def mom_nvt(self):
+        """mom_calc__nvt(mom, gp, np2): mom_calc.f90:167"""
+        check(self._L.wpic_mom_nvt2d(self._h))
+
+    def bc_mom(self):
+        """boundary_periodic__mom / boundary_reconnection__mom by the boundary kind"""
+        check(self._L.wpic_bc_mom2d(self._h))
+
+    def moments(self):
+        """accl, nvt and bc__mom in one pass (2d/proj/weibel/app.f90:121-123)"""
+        check(self._L.wpic_moments2d(self._h))
+
+    def download_mom(self):
+        """mom(7, nxgs-1:nxge+1, nys-1:nye+1, nsp)"""
+        mom = np.zeros((7, self.nx + 2, self.ny + 2, self.nsp), order="F")
+        check(self._L.wpic_download_mom2d(self._h, mom))
+        return mom
+
+    def upload_mom(self, mom):
+        check(self._L.wpic_upload_mom2d(self._h, np.asfortranarray(mom, dtype=np.float64)))
+
     # -- diagnostics and initial state ------------------------------------------
     def energy(self):
         out = np.zeros(3)

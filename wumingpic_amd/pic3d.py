@@ -141,6 +141,32 @@ class Pic3D:
         """nsteps of the loop body of 3d/proj/weibel/app.f90:103-108"""
         check(self._L.wpic_step3d(self._h, int(nsteps)))
 
+    # -- moments (3d/common/mom_calc.f90, boundary_periodic__mom) ----------------------
+    def mom_accl(self):
+        """mom_calc__accl(gp, up, uf, cumcnt, nxs, nxe): 3d/common/mom_calc.f90:49"""
+        check(self._L.wpic_mom_accl3d(self._h))
+
+    def mom_nvt(self):
+        """mom_calc__nvt(mom, gp, np2): 3d/common/mom_calc.f90:219"""
+        check(self._L.wpic_mom_nvt3d(self._h))
+
+    def bc_mom(self):
+        """boundary_periodic__mom(mom): 3d/common/boundary_periodic.f90:1102"""
+        check(self._L.wpic_bc_mom3d(self._h))
+
+    def moments(self):
+        """accl, nvt and bc__mom in one pass (3d/proj/weibel/app.f90:122-124)"""
+        check(self._L.wpic_moments3d(self._h))
+
+    def download_mom(self):
+        """mom(7, nxgs-1:nxge+1, nys-1:nye+1, nzs-1:nze+1, nsp)"""
+        mom = np.zeros((7, self.nx + 2, self.ny + 2, self.nz + 2, self.nsp), order="F")
+        check(self._L.wpic_download_mom3d(self._h, mom))
+        return mom
+
+    def upload_mom(self, mom):
+        check(self._L.wpic_upload_mom3d(self._h, np.asfortranarray(mom, dtype=np.float64)))
+
     # -- diagnostics and initial state ------------------------------------------------
     def energy(self):
         out = np.zeros(3)
