@@ -1,5 +1,5 @@
 """The C-ABI library loads on a CPU-only host and exports every entry point
-include/wpic.h declares (no compute calls here: there is no GPU)."""
+include/*.h declares (no compute calls here: there is no GPU)."""
 import ctypes as C
 import os
 import re
@@ -11,6 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "wpic.h")
 HEADER3D = os.path.join(ROOT, "include", "wpic3d.h")
 HEADER_ADJ = os.path.join(ROOT, "include", "wpic_adj.h")
+HEADER_IO = os.path.join(ROOT, "include", "wpic_io.h")
 LIB = os.path.join(ROOT, "wumingpic_amd", "libwpic.so")
 
 
@@ -24,7 +25,7 @@ def test_library_built():
     assert os.path.exists(LIB), "build with `python -m wumingpic_amd.build` (or __graft_entry__.build())"
 
 
-@pytest.mark.parametrize("header", [HEADER, HEADER3D, HEADER_ADJ])
+@pytest.mark.parametrize("header", [HEADER, HEADER3D, HEADER_ADJ, HEADER_IO])
 def test_every_declared_symbol_is_exported(header):
     names = declared(header)
     assert len(names) >= 6
@@ -41,6 +42,7 @@ def test_python_binding_covers_the_header():
     assert sorted(_lib.EXPORTS) == declared()
     assert sorted(_lib.EXPORTS3D) == declared(HEADER3D)
     assert sorted(_lib.EXPORTS_ADJ) == declared(HEADER_ADJ)
+    assert sorted(_lib.EXPORTS_IO) == declared(HEADER_IO)
 
 
 def test_missing_library_fails_loudly(monkeypatch):

@@ -1,8 +1,10 @@
 """The reference app's call sequence, written in Fortran against the shim
-modules (particle, field, sort, mom_calc, boundary_periodic,
+modules (particle, field, sort, mom_calc, paraio, boundary_periodic,
 boundary_reconnection), run on the GPU: the state after the loop and the
 moments of the output sequence equal the oracle's bit for bit in parity mode,
-with host mirroring after every call and in resident mode."""
+with host mirroring after every call and in resident mode, and the files
+io__mom / io__output write are the expected bytes."""
+import json
 import os
 import subprocess
 import tempfile
@@ -65,6 +67,11 @@ def test_fortran_app_loop_on_shim_matches_oracle(bc, resident):
         c_dev = np.fromfile(os.path.join(tmp, "cumcnt.out"), dtype=np.int32).reshape(cc.shape, order="F")
         n_dev = np.fromfile(os.path.join(tmp, "np2.out"), dtype=np.int32).reshape(np2.shape, order="F")
         m_dev = np.fromfile(os.path.join(tmp, "mom.out")).reshape((7, p.nx + 2, p.ny + 2, p.nsp), order="F")
+        files = {}
+        for nm in ("%07d_mom" % nsteps, "restart"):
+            for ext in (".json", ".raw"):
+                with open(os.path.join(tmp, nm + ext), "rb") as f:
+                    files[nm + ext] = f.read()
     o = po.Oracle2D(p)
     gp = np.zeros_like(up, order="F")
     o.step(up, gp, uf, cc, np2, nsteps)
@@ -80,3 +87,33 @@ def test_fortran_app_loop_on_shim_matches_oracle(bc, resident):
     mom = o.mom_nvt(gp, np2)
     o.bc_mom(mom)
     assert np.array_equal(_bits(m_dev), _bits(mom))
+    # io__mom and io__output through the shim's paraio module: the same bytes
+    # as the host writer (pinned against the reference's jsonio separately)
+    # gives for the oracle's state
+    from wumingpic_amd.paraio import ParaIO
+    with tempfile.TemporaryDirectory() as t:
+        io = ParaIO(ndim=p.ndim, np_cap=p.np, nsp=p.nsp, nxgs=p.nxgs, nxge=p.nxge, nygs=p.nygs, nyge=p.nyge,
+                    delx=p.delx, delt=p.delt, c=p.c, q=[p.q[k] for k in range(p.nsp)],
+                    r=[p.r[k] for k in range(p.nsp)], dir=t + "/")
+        io.mom(mom, uf, nsteps)
+        io.output(up, uf, np2, p.nxgs, p.nxge, nsteps, "restart")
+        io.close()
+        for name, data in files.items():
+            with open(os.path.join(t, name), "rb") as f:
+                mine = f.read()
+            if name == "restart.raw":
+                # the padded particle array: slots past np2 hold whatever each
+                # run left there; the active records, np2 and uf must agree
+                assert len(mine) == len(data)
+                meta = json.loads(files["restart.json"])["dataset"]
+                for key in ("np2", "uf"):
+                    o, n = meta[key]["offset"], meta[key]["size"]
+                    assert mine[o:o + n] == data[o:o + n], key
+                o = meta["up"]["offset"]
+                a = np.frombuffer(data, dtype="<u8", count=up.size, offset=o).reshape(up.shape, order="F")
+                b = np.frombuffer(mine, dtype="<u8", count=up.size, offset=o).reshape(up.shape, order="F")
+                for s in range(p.nsp):
+                    for j in range(p.ny):
+                        assert np.array_equal(a[:, :np2[j, s], j, s], b[:, :np2[j, s], j, s]), (s, j)
+                continue
+            assert mine == data, name

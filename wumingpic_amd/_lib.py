@@ -46,6 +46,13 @@ EXPORTS3D = (
 )
 
 
+# every symbol include/wpic_io.h declares
+EXPORTS_IO = (
+    "wpic_paraio_init2d", "wpic_paraio_init3d", "wpic_paraio_finalize", "wpic_paraio_output",
+    "wpic_paraio_input", "wpic_paraio_param", "wpic_paraio_mom", "wpic_paraio_ptcl", "wpic_paraio_orb",
+)
+
+
 # every symbol include/wpic_adj.h declares
 EXPORTS_ADJ = (
     "wpic_set_cg_tol2d", "wpic_perturb_bz2d", "wpic_adj_forward2d", "wpic_adj_backward2d",
@@ -167,6 +174,23 @@ def lib():
         "wpic_download_mom3d": [vp, d],
         "wpic_upload_mom3d": [vp, d],
     }
+    ci = C.c_int
+    cd = C.c_double
+    dv = np.ctypeslib.ndpointer(dtype=np.float64)
+    ip = C.POINTER(C.c_int)
+    sig.update({
+        "wpic_paraio_init2d": [C.POINTER(vp), ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, cd, cd, cd, dv, dv,
+                               C.c_char_p],
+        "wpic_paraio_init3d": [C.POINTER(vp), ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, cd, cd,
+                               cd, dv, dv, C.c_char_p],
+        "wpic_paraio_finalize": [vp],
+        "wpic_paraio_output": [vp, d, d, i, ci, ci, ci, C.c_char_p],
+        "wpic_paraio_input": [vp, d, d, i, ip, ip, ip, C.c_char_p],
+        "wpic_paraio_param": [vp, ci, cd, cd, cd, cd, cd, cd, C.c_char_p],
+        "wpic_paraio_mom": [vp, d, d, ci],
+        "wpic_paraio_ptcl": [vp, d, d, i, ci],
+        "wpic_paraio_orb": [vp, d, d, i, ci],
+    })
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args

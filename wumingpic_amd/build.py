@@ -1,4 +1,5 @@
-"""Build the gfx950 shared library wumingpic_amd/libwpic.so from csrc/*.hip.
+"""Build the gfx950 shared library wumingpic_amd/libwpic.so from csrc/*.hip
+(device code, hipcc) and csrc/*.cpp (host-only file formats, g++).
 
     python -m wumingpic_amd.build          (or build() from Python)
 
@@ -27,13 +28,21 @@ CFLAGS = [
 ]
 
 
+CXX = os.environ.get("CXX", "g++")
+# host-only sources (file formats): plain C++
+CXXFLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = [CXX, *CXXFLAGS, "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

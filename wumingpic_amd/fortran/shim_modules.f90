@@ -5,6 +5,8 @@
 !   sort:      sort__init, sort__bucket            (2d/common/sort.f90:16,36)
 !   mom_calc:  mom_calc__init, mom_calc__accl, mom_calc__nvt
 !                                                  (2d/common/mom_calc.f90:18,48,167)
+!   paraio:    paraio__init, __finalize, __output, __input, __param, __mom,
+!              __ptcl, __orb                       (2d/common/paraio.f90:37-741)
 !   boundary_periodic / boundary_reconnection:
 !              *__init, *__particle_x, *__particle_y, *__mom and the boundary
 !              procedures handed to field__fdtd_i  (boundary_periodic.f90:26-636,
@@ -453,3 +455,102 @@ contains
   end subroutine wpic_shim_finalize
 
 end module wpic_shim
+
+
+! the raw + JSON files of 2d/common/paraio.f90 at one rank, written by the
+! host C++ of include/wpic_io.h (byte-identical files); host arrays as the
+! reference passes them
+module paraio
+  use iso_c_binding
+  use wpic_capi
+  implicit none
+  private
+  public :: paraio__init, paraio__finalize, paraio__output, paraio__input
+  public :: paraio__param, paraio__mom, paraio__ptcl, paraio__orb
+
+  type(c_ptr), save :: io = c_null_ptr
+  integer, save :: s_ndim, s_np, s_nsp, s_nxgs, s_nxge, s_nys, s_nye
+
+contains
+
+  subroutine io_check(rc)
+    integer(c_int), intent(in) :: rc
+    if (rc /= 0) then
+       write(0,*) wpic_error_text()
+       stop
+    end if
+  end subroutine io_check
+
+  function cstr(s) result(o)
+    character(len=*), intent(in) :: s
+    character(kind=c_char, len=len_trim(s)+1) :: o
+    o = trim(s) // c_null_char
+  end function cstr
+
+  subroutine paraio__init(ndim_in, np_in, nsp_in, nxgs_in, nxge_in, nygs_in, nyge_in, nys_in, nye_in, &
+       nproc_in, nrank_in, delx_in, delt_in, c_in, q_in, r_in, dir_in)
+    integer, intent(in) :: ndim_in, np_in, nsp_in
+    integer, intent(in) :: nxgs_in, nxge_in, nygs_in, nyge_in, nys_in, nye_in
+    integer, intent(in) :: nproc_in, nrank_in
+    real(8), intent(in) :: delx_in, delt_in, c_in, q_in(nsp_in), r_in(nsp_in)
+    character(len=*), intent(in) :: dir_in
+    s_ndim = ndim_in; s_np = np_in; s_nsp = nsp_in
+    s_nxgs = nxgs_in; s_nxge = nxge_in; s_nys = nys_in; s_nye = nye_in
+    call io_check(wpic_paraio_init2d(io, ndim_in, np_in, nsp_in, nxgs_in, nxge_in, nygs_in, nyge_in, &
+         nys_in, nye_in, nproc_in, nrank_in, delx_in, delt_in, c_in, q_in, r_in, cstr(dir_in)))
+  end subroutine paraio__init
+
+  subroutine paraio__finalize()
+    integer(c_int) :: rc
+    if (c_associated(io)) rc = wpic_paraio_finalize(io)
+    io = c_null_ptr
+  end subroutine paraio__finalize
+
+  subroutine paraio__output(up, uf, np2, nxs, nxe, it, filename)
+    integer, intent(in) :: np2(s_nys:s_nye, s_nsp), nxs, nxe, it
+    real(8), intent(in) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    real(8), intent(in) :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
+    character(len=*), intent(in) :: filename
+    call io_check(wpic_paraio_output(io, up, uf, np2, nxs, nxe, it, cstr(filename)))
+  end subroutine paraio__output
+
+  subroutine paraio__input(up, uf, np2, nxs, nxe, it, filename)
+    real(8), intent(out) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    real(8), intent(out) :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
+    integer, intent(out) :: np2(s_nys:s_nye, s_nsp)
+    integer, intent(out) :: nxs, nxe, it
+    character(len=*), intent(in) :: filename
+    call io_check(wpic_paraio_input(io, up, uf, np2, nxs, nxe, it, cstr(filename)))
+  end subroutine paraio__input
+
+  subroutine paraio__param(n0, wpe, wpi, wge, wgi, vti, vte, filename)
+    integer, intent(in) :: n0
+    real(8), intent(in) :: wpe, wpi, wge, wgi, vti, vte
+    character(len=*), intent(in) :: filename
+    call io_check(wpic_paraio_param(io, n0, wpe, wpi, wge, wgi, vti, vte, cstr(filename)))
+  end subroutine paraio__param
+
+  subroutine paraio__mom(mom, uf, it)
+    integer, intent(in) :: it
+    real(8), intent(in) :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
+    real(8), intent(inout) :: mom(7, s_nxgs-1:s_nxge+1, s_nys-1:s_nye+1, s_nsp)
+    call io_check(wpic_paraio_mom(io, mom, uf, it))
+  end subroutine paraio__mom
+
+  subroutine paraio__ptcl(up, uf, np2, it)
+    integer, intent(in) :: it
+    integer, intent(in) :: np2(s_nys:s_nye, s_nsp)
+    real(8), intent(in) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    real(8), intent(in) :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
+    call io_check(wpic_paraio_ptcl(io, up, uf, np2, it))
+  end subroutine paraio__ptcl
+
+  subroutine paraio__orb(up, uf, np2, it)
+    integer, intent(in) :: it
+    integer, intent(in) :: np2(s_nys:s_nye, s_nsp)
+    real(8), intent(in) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    real(8), intent(in) :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
+    call io_check(wpic_paraio_orb(io, up, uf, np2, it))
+  end subroutine paraio__orb
+
+end module paraio

@@ -10,7 +10,9 @@
 !               r(1:nsp)
 !   in:  up.bin uf.bin cumcnt.bin np2.bin     out: up.out uf.out cumcnt.out np2.out
 !   and, after the loop, the app's moment output sequence
-!   (2d/proj/weibel/app.f90:121-123): mom.out
+!   (2d/proj/weibel/app.f90:121-124): mom.out, then io__mom and save_restart's
+!   io__output through the paraio module: <nsteps>_mom.{raw,json},
+!   restart.{raw,json}
 program shim_step2d_driver
   use iso_c_binding, only: c_int
   use wpic_capi, only: WPIC_MODE_FAST, WPIC_MODE_PARITY
@@ -19,6 +21,7 @@ program shim_step2d_driver
   use field
   use sort
   use mom_calc
+  use paraio
   use boundary_periodic, only: bcp_init => boundary_periodic__init, bcp_x => boundary_periodic__particle_x, &
        bcp_y => boundary_periodic__particle_y, bcp_dfield => boundary_periodic__dfield, &
        bcp_curre => boundary_periodic__curre, bcp_phi => boundary_periodic__phi, bcp_mom => boundary_periodic__mom
@@ -101,6 +104,11 @@ program shim_step2d_driver
   call write_i4(trim(dir)//'/cumcnt.out', cumcnt, size(cumcnt))
   call write_i4(trim(dir)//'/np2.out', np2, size(np2))
   call write_r8(trim(dir)//'/mom.out', mom, size(mom))
+
+  call paraio__init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, 1, 0, delx, delt, c, q, r, trim(dir)//'/')
+  call paraio__mom(mom, uf, nsteps)
+  call paraio__output(up, uf, np2, nxs, nxe, nsteps, 'restart')
+  call paraio__finalize()
 
 contains
 

@@ -1,5 +1,5 @@
-! wpic_capi.f90 -- ISO_C_BINDING interfaces to the C-ABI of include/wpic.h
-! and include/wpic_adj.h.
+! wpic_capi.f90 -- ISO_C_BINDING interfaces to the C-ABI of include/wpic.h,
+! include/wpic_adj.h and include/wpic_io.h.
 module wpic_capi
   use iso_c_binding
   implicit none
@@ -107,6 +107,74 @@ module wpic_capi
        type(c_ptr), value :: ctx
        real(c_double), intent(in) :: mom(*)
      end function wpic_upload_mom2d
+
+     integer(c_int) function wpic_paraio_init2d(io, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, nproc, &
+          nrank, delx, delt, c, q, r, dir) bind(C, name='wpic_paraio_init2d')
+       import :: c_int, c_ptr, c_double, c_char
+       type(c_ptr), intent(out) :: io
+       integer(c_int), value :: ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, nproc, nrank
+       real(c_double), value :: delx, delt, c
+       real(c_double), intent(in) :: q(*), r(*)
+       character(kind=c_char), intent(in) :: dir(*)
+     end function wpic_paraio_init2d
+
+     integer(c_int) function wpic_paraio_finalize(io) bind(C, name='wpic_paraio_finalize')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: io
+     end function wpic_paraio_finalize
+
+     integer(c_int) function wpic_paraio_output(io, up, uf, np2, nxs, nxe, it, filename) &
+          bind(C, name='wpic_paraio_output')
+       import :: c_int, c_ptr, c_double, c_char
+       type(c_ptr), value :: io
+       real(c_double), intent(in) :: up(*), uf(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: nxs, nxe, it
+       character(kind=c_char), intent(in) :: filename(*)
+     end function wpic_paraio_output
+
+     integer(c_int) function wpic_paraio_input(io, up, uf, np2, nxs, nxe, it, filename) &
+          bind(C, name='wpic_paraio_input')
+       import :: c_int, c_ptr, c_double, c_char
+       type(c_ptr), value :: io
+       real(c_double), intent(out) :: up(*), uf(*)
+       integer(c_int), intent(out) :: np2(*)
+       integer(c_int), intent(out) :: nxs, nxe, it
+       character(kind=c_char), intent(in) :: filename(*)
+     end function wpic_paraio_input
+
+     integer(c_int) function wpic_paraio_param(io, n0, wpe, wpi, wge, wgi, vti, vte, filename) &
+          bind(C, name='wpic_paraio_param')
+       import :: c_int, c_ptr, c_double, c_char
+       type(c_ptr), value :: io
+       integer(c_int), value :: n0
+       real(c_double), value :: wpe, wpi, wge, wgi, vti, vte
+       character(kind=c_char), intent(in) :: filename(*)
+     end function wpic_paraio_param
+
+     integer(c_int) function wpic_paraio_mom(io, mom, uf, it) bind(C, name='wpic_paraio_mom')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: io
+       real(c_double), intent(inout) :: mom(*)
+       real(c_double), intent(in) :: uf(*)
+       integer(c_int), value :: it
+     end function wpic_paraio_mom
+
+     integer(c_int) function wpic_paraio_ptcl(io, up, uf, np2, it) bind(C, name='wpic_paraio_ptcl')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: io
+       real(c_double), intent(in) :: up(*), uf(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: it
+     end function wpic_paraio_ptcl
+
+     integer(c_int) function wpic_paraio_orb(io, up, uf, np2, it) bind(C, name='wpic_paraio_orb')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: io
+       real(c_double), intent(in) :: up(*), uf(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: it
+     end function wpic_paraio_orb
 
      integer(c_int) function wpic_particle_solv2d(ctx) bind(C, name='wpic_particle_solv2d')
        import :: c_int, c_ptr
