@@ -157,6 +157,24 @@ int wpic_upload_mom2d(wpic_ctx *ctx, const double *mom);
 int wpic_ic_beams2d(wpic_ctx *ctx, unsigned long long seed, int n_ppc, double udrift,
                     double uth_e, double uth_i, double b0);
 
+/* The reference apps' own initial states (FP64 modes), with its OS-seeded
+ * random draws replaced by counter-based draws from `seed`:
+ * wpic_ic_weibel2d: 2d/proj/weibel/app.f90:366-438 -- Bz = b0 everywhere,
+ *   n0 particles per cell per species on a regular x lattice along each row,
+ *   y uniform in the row, shared by both species; Maxwellian momenta with
+ *   spread v_thi (ions) / v_the (electrons), t_ani times that along z.
+ * wpic_ic_harris2d: 2d/proj/reconnection/app.f90:260-452 (walls in x) --
+ *   the Harris sheet b0 tanh((x-x0)/lcs) along y with the X-point
+ *   perturbation e1 (the reference hard-codes 0.12), nbg background and
+ *   ncs sheet particles per cell, drift carrying jz, temperatures vti, vte,
+ *   Te/Ti = rtemp; lcs in cells (the reference's lcs*c/wpi); the loaded
+ *   particles are sorted as the reference sorts them.
+ * Ids are -1, -2, ... row by row per species (app.f90:455-470). */
+int wpic_ic_weibel2d(wpic_ctx *ctx, unsigned long long seed, int n0, double v_thi, double v_the, double t_ani,
+                     double b0);
+int wpic_ic_harris2d(wpic_ctx *ctx, unsigned long long seed, int nbg, int ncs, double lcs, double x0, double y0,
+                     double b0, double e1, double rtemp, double vti, double vte);
+
 /* timing of named kernels with HIP events on the context's stream ---------- */
 int wpic_set_timing(wpic_ctx *ctx, int on);
 /* name: "push_deposit", "sort", "exchange", "field", "cg"; returns the summed

@@ -12,6 +12,8 @@ HEADER = os.path.join(ROOT, "include", "wpic.h")
 HEADER3D = os.path.join(ROOT, "include", "wpic3d.h")
 HEADER_ADJ = os.path.join(ROOT, "include", "wpic_adj.h")
 HEADER_IO = os.path.join(ROOT, "include", "wpic_io.h")
+HEADER_APP = os.path.join(ROOT, "include", "wpic_app.h")
+APP = os.path.join(ROOT, "wumingpic_amd", "app", "wpic_app2d")
 LIB = os.path.join(ROOT, "wumingpic_amd", "libwpic.so")
 
 
@@ -25,10 +27,10 @@ def test_library_built():
     assert os.path.exists(LIB), "build with `python -m wumingpic_amd.build` (or __graft_entry__.build())"
 
 
-@pytest.mark.parametrize("header", [HEADER, HEADER3D, HEADER_ADJ, HEADER_IO])
+@pytest.mark.parametrize("header", [HEADER, HEADER3D, HEADER_ADJ, HEADER_IO, HEADER_APP])
 def test_every_declared_symbol_is_exported(header):
     names = declared(header)
-    assert len(names) >= 6
+    assert len(names) >= (1 if header == HEADER_APP else 6)
     lib = C.CDLL(LIB)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
@@ -43,6 +45,7 @@ def test_python_binding_covers_the_header():
     assert sorted(_lib.EXPORTS3D) == declared(HEADER3D)
     assert sorted(_lib.EXPORTS_ADJ) == declared(HEADER_ADJ)
     assert sorted(_lib.EXPORTS_IO) == declared(HEADER_IO)
+    assert sorted(_lib.EXPORTS_APP) == declared(HEADER_APP)
 
 
 def test_missing_library_fails_loudly(monkeypatch):
@@ -88,3 +91,20 @@ def test_invalid_3d_parameters_rejected_without_gpu_work():
     p.delx = 0.5
     assert L.wpic_create3d(C.byref(h), C.byref(p)) == _lib.E_INVAL
     assert b"delx" in L.wpic_last_error()
+
+
+def test_app_driver_rejects_bad_configs_before_gpu_work(tmp_path):
+    """load_config's errors (2d/proj/weibel/app.f90:160-250): a missing file, a file that is not
+    JSON, a parameter section that names no single problem"""
+    assert os.path.exists(APP)
+    r = subprocess.run([APP, str(tmp_path / "none.json")], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "does not exists" in r.stderr
+    bad = tmp_path / "bad.json"
+    bad.write_text("{ not json")
+    r = subprocess.run([APP, str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "failed to load" in r.stderr
+    two = tmp_path / "two.json"
+    two.write_text('{"config": {"max_it": 1}, "parameter": {"n_x": 8, "n_y": 8, "mass_ratio": 25.0, '
+                   '"t_ani": 5.0, "nbg": 3, "ncs": 9}}')
+    r = subprocess.run([APP, str(two)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "one problem" in r.stderr

@@ -28,7 +28,7 @@ EXPORTS = (
     "wpic_energy2d", "wpic_gauss2d", "wpic_counts2d", "wpic_ic_beams2d",
     "wpic_set_timing", "wpic_timing",
     "wpic_mom_accl2d", "wpic_mom_nvt2d", "wpic_bc_mom2d", "wpic_moments2d",
-    "wpic_download_mom2d", "wpic_upload_mom2d",
+    "wpic_download_mom2d", "wpic_upload_mom2d", "wpic_ic_weibel2d", "wpic_ic_harris2d",
 )
 
 # every symbol include/wpic3d.h declares
@@ -51,6 +51,10 @@ EXPORTS_IO = (
     "wpic_paraio_init2d", "wpic_paraio_init3d", "wpic_paraio_finalize", "wpic_paraio_output",
     "wpic_paraio_input", "wpic_paraio_param", "wpic_paraio_mom", "wpic_paraio_ptcl", "wpic_paraio_orb",
 )
+
+
+# every symbol include/wpic_app.h declares
+EXPORTS_APP = ("wpic_app2d_main",)
 
 
 # every symbol include/wpic_adj.h declares
@@ -129,6 +133,8 @@ def lib():
         "wpic_ic_beams2d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
         "wpic_set_timing": [vp, C.c_int],
         "wpic_timing": [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
+        "wpic_ic_weibel2d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
+        "wpic_ic_harris2d": [vp, C.c_ulonglong, C.c_int, C.c_int] + [C.c_double] * 8,
         "wpic_mom_accl2d": [vp],
         "wpic_mom_nvt2d": [vp],
         "wpic_bc_mom2d": [vp],

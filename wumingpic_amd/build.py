@@ -108,8 +108,22 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_app()
     build_fortran()
     return LIB
+
+
+APP_SRC = os.path.join(HERE, "app", "wpic_app2d.cpp")
+APP_EXE = os.path.join(HERE, "app", "wpic_app2d")
+
+
+def build_app():
+    """the command-line problem driver, linked to libwpic.so"""
+    cmd = [CXX, *CXXFLAGS, "-o", APP_EXE, APP_SRC, "-L" + HERE, "-lwpic", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build of the app driver failed:\n{r.stdout}\n{r.stderr}")
+    return APP_EXE
 
 
 if __name__ == "__main__":

@@ -318,6 +318,153 @@ hipError_t launch_ic_beams_mixed(wpic_ctx* c, unsigned long long seed, int n_ppc
 }
 
 // ---------------------------------------------------------------------------
+// The reference apps' own initial states, with the OS-seeded random draws of
+// the reference replaced by the counter-based draws above.
+// ---------------------------------------------------------------------------
+// 2d/proj/weibel/app.f90:366-438: n0 particles per cell per species, x on a
+// regular lattice along each row, (nxgs + nx*(ii-0.5)/np2)*delx, y uniform in
+// the row, shared by the species; Maxwellian momenta with spread sd and t_ani
+// times that along z; ids -(row-major count) per species (app.f90:455-470).
+struct IcWeibelArgs {
+  Grid2D g;
+  Parts A;
+  unsigned long long seed;
+  int n0;
+  double sd[WPIC_MAXSP], t_ani;
+};
+
+__global__ void k_ic_weibel(IcWeibelArgs a) {
+  const Grid2D& g = a.g;
+  const long long rowlen = (long long)a.n0 * g.nx, per = rowlen * g.ny;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= per * g.nsp) return;
+  const int s = (int)(t / per);
+  const long long k = t - s * per;
+  const int r = (int)(k / rowlen);
+  const int ii = (int)(k - r * rowlen) + 1;  // the reference's slot index
+  const long long d = s * g.ncap + k;
+  const int j = g.nys + r;
+  const int np2 = (int)rowlen;
+  a.A.x[d] = (g.nxgs + (g.nxge - g.nxgs + 1) * (ii - 0.5) / np2) * g.delx;
+  a.A.y[d] = (j + u01(rkey(a.seed, 3, r, 0, ii))) * g.delx;
+  double n0, n1, n2, n3;
+  normal2(rkey(a.seed, 20 + 4 * s, r, 0, ii), rkey(a.seed, 21 + 4 * s, r, 0, ii), n0, n1);
+  normal2(rkey(a.seed, 22 + 4 * s, r, 0, ii), rkey(a.seed, 23 + 4 * s, r, 0, ii), n2, n3);
+  a.A.ux[d] = a.sd[s] * n0;
+  a.A.uy[d] = a.sd[s] * n1;
+  a.A.uz[d] = a.t_ani * a.sd[s] * n2;
+  a.A.id[d] = (uint64_t)(-(k + 1));
+}
+
+// 2d/proj/reconnection/app.f90:373-452: the Harris sheet with the X-point
+// perturbation.  Per row, ibg background particles uniform in
+// [(nxgs+1)dx, (nxge-1)dx) and np2 - ibg sheet particles from the inverse of
+// the sheet's cosh^-2 profile, shared by the species; Maxwellian momenta plus
+// the drift that carries jz; written unsorted into B (the reference sorts
+// them next, app.f90:346-348).
+struct IcHarrisArgs {
+  Grid2D g;
+  Parts B;
+  unsigned long long seed;
+  int np2, ibg;
+  double x0, y0, lcs, b0, e1, ncs, nbg, f[2], sd[2];
+};
+
+__device__ inline double harris_arg(const IcHarrisArgs& a, double x, double y) {
+  return ((x - a.x0) * (x - a.x0) + (y - a.y0) * (y - a.y0)) / ((2 * a.lcs) * (2 * a.lcs));
+}
+
+__global__ void k_ic_harris(IcHarrisArgs a) {
+  const Grid2D& g = a.g;
+  const long long per = (long long)a.np2 * g.ny;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= per) return;
+  const int r = (int)(t / a.np2);
+  const int ii = (int)(t - (long long)r * a.np2) + 1;
+  const int j = g.nys + r;
+  const double pi = 3.141592653589793;
+  double x;
+  if (ii <= a.ibg) {
+    x = (g.nxgs + 1) * g.delx + u01(rkey(a.seed, 4, r, 0, ii)) * g.delx * (g.nxge - g.nxgs - 2);
+  } else {
+    double r1 = u01(rkey(a.seed, 5, r, 0, ii));
+    r1 = (2.0 * r1 - 1.0) * tanh(0.5 * (g.nxge - g.nxgs - 2) * g.delx / a.lcs);
+    x = a.lcs * 0.5 * (log(1.0 + r1) - log(1.0 - r1)) + a.x0;
+  }
+  const double y = (double)j * g.delx + u01(rkey(a.seed, 6, r, 0, ii)) * g.delx;
+  const double ch = cosh((x - a.x0) / a.lcs);
+  const double sech2 = 1.0 / (ch * ch);
+  const double e = harris_arg(a, x, y);
+  const double jz = +a.b0 / (4 * pi * a.lcs) * sech2 - 2 * a.e1 * a.b0 / (4 * pi * a.lcs) * (1.0 - e) * exp(-e);
+  const double den = a.ncs * sech2 + a.nbg;
+  for (int s = 0; s < 2; s++) {
+    double n0, n1, n2, n3;
+    normal2(rkey(a.seed, 30 + 4 * s, r, 0, ii), rkey(a.seed, 31 + 4 * s, r, 0, ii), n0, n1);
+    normal2(rkey(a.seed, 32 + 4 * s, r, 0, ii), rkey(a.seed, 33 + 4 * s, r, 0, ii), n2, n3);
+    const long long d = s * g.ncap + t;
+    a.B.x[d] = x;
+    a.B.y[d] = y;
+    a.B.ux[d] = a.sd[s] * n0;
+    a.B.uy[d] = a.sd[s] * n1;
+    a.B.uz[d] = a.sd[s] * n2 + a.f[s] * jz / den;
+    a.B.id[d] = (uint64_t)(-(t + 1));
+  }
+}
+
+// the Harris field with the perturbation at the nodes (i dx, j dx), ghosts
+// included (app.f90:397-412)
+__global__ void k_ic_harris_field(IcHarrisArgs a, double* uf) {
+  const Grid2D& g = a.g;
+  const size_t pl = (size_t)g.fw * g.fh;
+  const size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (n >= pl) return;
+  const int i = g.nxgs - 2 + (int)(n % g.fw), j = g.nys - 2 + (int)(n / g.fw);
+  const double x = i * g.delx, y = j * g.delx;
+  const double e = exp(-harris_arg(a, x, y));
+  uf[n] = 0.0 + a.e1 * a.b0 * ((y - a.y0) / a.lcs) * e;
+  uf[pl + n] = a.b0 * tanh((x - a.x0) / a.lcs) + -a.e1 * a.b0 * ((x - a.x0) / a.lcs) * e;
+  for (int c = 2; c < 6; c++) uf[c * pl + n] = 0.0;
+}
+
+hipError_t launch_ic_weibel(wpic_ctx* c, unsigned long long seed, int n0, const double* sd, double t_ani) {
+  IcWeibelArgs a;
+  a.g = c->g;
+  a.A = c->A;
+  a.seed = seed;
+  a.n0 = n0;
+  for (int s = 0; s < WPIC_MAXSP; s++) a.sd[s] = s < c->g.nsp ? sd[s] : 0.0;
+  a.t_ani = t_ani;
+  const long long n = (long long)n0 * c->g.nx * c->g.ny * c->g.nsp;
+  hipLaunchKernelGGL(k_ic_weibel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ic_harris(wpic_ctx* c, unsigned long long seed, int np2, int ibg, const double* hp) {
+  IcHarrisArgs a;
+  a.g = c->g;
+  a.B = c->B;
+  a.seed = seed;
+  a.np2 = np2;
+  a.ibg = ibg;
+  a.x0 = hp[0];
+  a.y0 = hp[1];
+  a.lcs = hp[2];
+  a.b0 = hp[3];
+  a.e1 = hp[4];
+  a.ncs = hp[5];
+  a.nbg = hp[6];
+  a.f[0] = hp[7];
+  a.f[1] = hp[8];
+  a.sd[0] = hp[9];
+  a.sd[1] = hp[10];
+  const long long n = (long long)np2 * c->g.ny;
+  hipLaunchKernelGGL(k_ic_harris, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  const size_t pl = (size_t)c->g.fw * c->g.fh;
+  hipLaunchKernelGGL(k_ic_harris_field, dim3((unsigned)((pl + 255) / 256)), dim3(256), 0, c->st, a, c->uf);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // diagnostics
 // ---------------------------------------------------------------------------
 constexpr int RB = 256;   // reduction blocks

@@ -810,6 +810,84 @@ int wpic_ic_beams2d(wpic_ctx* c, unsigned long long seed, int n_ppc, double udri
   return 0;
 }
 
+int wpic_ic_weibel2d(wpic_ctx* c, unsigned long long seed, int n0, double v_thi, double v_the, double t_ani,
+                     double b0) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
+  const Grid2D& g = c->g;
+  if (n0 <= 0 || (long long)n0 * g.nx * g.ny > g.ncap) return set_error(WPIC_EMEM, "Error: Too large number of particles");
+  if (g.nsp != 2) return set_error(WPIC_EINVAL, "the Weibel state has two species (ions, electrons)");
+  const int nr = g.nsp * g.ny;
+  std::vector<int> hrows(2 * nr);
+  std::vector<int> cc((size_t)nr * (g.nx + 1));
+  for (int s = 0; s < g.nsp; s++)
+    for (int r = 0; r < g.ny; r++) {
+      const int rw = s * g.ny + r;
+      hrows[rw] = (int)(s * g.ncap + (long long)r * n0 * g.nx);
+      hrows[nr + rw] = n0 * g.nx;
+      // x on the lattice: n0 per cell in order (app.f90:338-351)
+      for (int i = 0; i <= g.nx; i++) cc[(size_t)rw * (g.nx + 1) + i] = i * n0;
+    }
+  HIPCHK(hipMemcpyAsync(c->rows[c->ra], hrows.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->cumcnt, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice, c->st));
+  c->aperm = nullptr;
+  const double sd[WPIC_MAXSP] = {v_thi, v_the, 0.0, 0.0};
+  HIPCHK(launch_ic_weibel(c, seed, n0, sd, t_ani));
+  const size_t pl = (size_t)g.fw * g.fh;
+  HIPCHK(launch_zero(c, c->uf, 6 * pl));
+  HIPCHK(launch_fill(c, c->uf + 2 * pl, pl, b0));
+  HIPCHK(launch_zero(c, c->df, 6 * pl));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->h_rows[c->ra] = hrows;
+  c->h_rows_ok[c->ra] = true;
+  c->b_valid = false;
+  c->b_accl = false;
+  c->b_aligned = false;
+  c->exch_pending = false;
+  return check_err(c);
+}
+
+int wpic_ic_harris2d(wpic_ctx* c, unsigned long long seed, int nbg, int ncs, double lcs, double x0, double y0,
+                     double b0, double e1, double rtemp, double vti, double vte) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
+  const Grid2D& g = c->g;
+  if (g.nsp != 2) return set_error(WPIC_EINVAL, "the Harris state has two species (ions, electrons)");
+  if (!(lcs > 0.0)) return set_error(WPIC_EINVAL, "lcs must be positive");
+  // particles per row and background particles per row (app.f90:304, 383)
+  const double np2d = nbg * (g.nxge - g.nxgs) + ncs * 2 * lcs;
+  if (!(np2d >= 0.0) || np2d > 2.0e9) return set_error(WPIC_EMEM, "Error: Too large number of particles");
+  const int np2 = (int)np2d;
+  if ((long long)np2 * g.ny > g.ncap) return set_error(WPIC_EMEM, "Error: Too large number of particles");
+  const int ibg = nbg * (g.nxge - g.nxgs - 2);
+  if (ibg > np2 || ibg < 0) return set_error(WPIC_EINVAL, "background particles exceed the particles per row");
+  // the Box-Muller factor 1/sqrt(2.) with a default-real literal (app.f90:380-381)
+  const double s2 = (double)std::sqrt(2.0f);
+  const double hp[11] = {x0, y0, lcs, b0, e1, (double)ncs, (double)nbg,
+                         1.0 / ((1.0 + rtemp) * g.q[0]), rtemp / ((1.0 + rtemp) * g.q[1]), vti / s2, vte / s2};
+  const int nr = g.nsp * g.ny;
+  std::vector<int> hrows(2 * nr);
+  for (int s = 0; s < g.nsp; s++)
+    for (int r = 0; r < g.ny; r++) {
+      hrows[s * g.ny + r] = (int)(s * g.ncap + (long long)r * np2);
+      hrows[nr + s * g.ny + r] = np2;
+    }
+  if (c->rb == c->ra) c->rb = pick_free(c->ra, c->rn);
+  set_rows(c);
+  HIPCHK(hipMemcpyAsync(c->rows[c->rb], hrows.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
+  HIPCHK(launch_ic_harris(c, seed, np2, ibg, hp));
+  const size_t pl = (size_t)g.fw * g.fh;
+  HIPCHK(launch_zero(c, c->df, 6 * pl));
+  c->h_rows[c->rb] = hrows;
+  c->h_rows_ok[c->rb] = true;
+  c->b_valid = true;
+  c->b_accl = false;
+  c->b_aligned = false;
+  c->exch_pending = false;
+  // sort__bucket(gp, up, ...) of the loaded particles (app.f90:346-348)
+  return wpic_sort_bucket2d(c);
+}
+
 int wpic_set_timing(wpic_ctx* c, int on) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
   fold_timers(c);

@@ -275,6 +275,9 @@ hipError_t launch_planes_to_aos(wpic_ctx* c, const double* planes, double* aos, 
 hipError_t launch_ic_beams(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
                            double uth_i);
 hipError_t launch_fill(wpic_ctx* c, double* p, size_t n, double v);
+hipError_t launch_ic_weibel(wpic_ctx* c, unsigned long long seed, int n0, const double* sd, double t_ani);
+// hp: x0, y0, lcs, b0, e1, ncs, nbg, f1, f2, sd1, sd2
+hipError_t launch_ic_harris(wpic_ctx* c, unsigned long long seed, int np2, int ibg, const double* hp);
 hipError_t launch_gather_rows(wpic_ctx* c, const Parts& src, const int* ap, const Parts& dst, const int* rows);
 // moments (mom_calc.f90, boundary_*__mom): deposit kinds
 enum { MOM_TILE_KICK = 0, MOM_TILE_GP = 1, MOM_ANY = 2, MOM_PARITY_ALIGNED = 3, MOM_PARITY_ANY = 4 };

@@ -201,6 +201,16 @@ class Pic2D:
                                       float(uth_i), float(b0)))
 
     # -- adjoint (include/wpic_adj.h) ---------------------------------------------
+    def ic_weibel(self, seed, n0, v_thi, v_the, t_ani, b0):
+        """the reference Weibel app's state (2d/proj/weibel/app.f90:366-438)"""
+        check(self._L.wpic_ic_weibel2d(self._h, int(seed), int(n0), float(v_thi), float(v_the), float(t_ani),
+                                       float(b0)))
+
+    def ic_harris(self, seed, nbg, ncs, lcs, x0, y0, b0, e1, rtemp, vti, vte):
+        """the reference reconnection app's Harris sheet (2d/proj/reconnection/app.f90:260-452)"""
+        check(self._L.wpic_ic_harris2d(self._h, int(seed), int(nbg), int(ncs), float(lcs), float(x0), float(y0),
+                                       float(b0), float(e1), float(rtemp), float(vti), float(vte)))
+
     def set_cg_tol(self, err):
         """CG relative tolerance (the reference's err = 1e-6, 2d/common/field.f90:339)"""
         check(self._L.wpic_set_cg_tol2d(self._h, float(err)))
