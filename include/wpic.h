@@ -29,6 +29,10 @@ extern "C" {
 #define WPIC_BC_PERIODIC 0 /* 2d/common/boundary_periodic.f90 */
 #define WPIC_BC_WALLX 1    /* 2d/proj/reconnection/boundary_reconnection.f90 */
 
+/* particle kicks of the FP64 modes (wpic_set_pusher2d) */
+#define WPIC_PUSHER_BORIS 0 /* particle__solv: 2d/common/particle.f90:48-179 */
+#define WPIC_PUSHER_VAY 1   /* particle__solv_vay: 2d/common/particle.f90:182-315 */
+
 /* arithmetic modes of the FP64 path */
 #define WPIC_MODE_FAST 0   /* tiled deposit with atomics, tree dot products */
 #define WPIC_MODE_PARITY 1 /* reference summation order: bit-exact vs 1 thread */
@@ -94,6 +98,15 @@ int wpic_upload_dfield2d(wpic_ctx *ctx, const double *df);
 /* reference entry points ------------------------------------------------- */
 /* particle__solv(gp, up, uf, cumcnt, nxs, nxe): 2d/common/particle.f90:48 */
 int wpic_particle_solv2d(wpic_ctx *ctx);
+/* particle__solv_vay(gp, up, uf, cumcnt, nxs, nxe): 2d/common/particle.f90:182
+ * -- the Vay kick (Phys. Plasmas 15, 056701 (2008)) with particle__solv's
+ * field staging, gather and move; FP64 modes. */
+int wpic_particle_solv_vay2d(wpic_ctx *ctx);
+/* The kick that wpic_particle_solv2d and wpic_step2d use from now on:
+ * WPIC_PUSHER_BORIS (the default, the reference apps' choice) or
+ * WPIC_PUSHER_VAY; FP64 modes.  Moments keep mom_calc__accl's Boris kick, as
+ * the reference's mom_calc.f90 does. */
+int wpic_set_pusher2d(wpic_ctx *ctx, int pusher);
 /* field__fdtd_i(uf, up, gp, cumcnt, nxs, nxe, bc_dfield, bc_curre, bc_phi):
  * 2d/common/field.f90:66.  The boundary procedures are applied on the device
  * according to the context's boundary kind. */

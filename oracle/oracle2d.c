@@ -139,8 +139,47 @@ static inline void shape3(double dh, double s[3]) {
 /* ---------------------------------------------------------------------------
  * particle__solv: 2d/common/particle.f90:48-179
  * ------------------------------------------------------------------------- */
-void or2d_particle_solv(const or2d_params *p, double *gp, const double *up,
-                        const double *uf, const int *cumcnt) {
+/* The Vay kick (particle__solv_vay, 2d/common/particle.f90:262-295, after
+ * Vay, Phys. Plasmas 15, 056701 (2008)): u' = u + fac2 E + (fac1/gam) u x B,
+ * tau = fac1 B / c, then gamma_new from the quadratic and the implicit
+ * rotation; returns 1/gamma_new for the move. */
+static double vay_kick(double c, double fac1, double fac2, const double *u, double bpx, double bpy, double bpz,
+                       double epx, double epy, double epz, double *g) {
+  const double uvm1 = u[2], uvm2 = u[3], uvm3 = u[4];
+  double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+  const double fac1r = fac1 / gam;
+  const double uvm4 = uvm1 + fac2 * epx + fac1r * (+uvm2 * bpz - uvm3 * bpy);
+  const double uvm5 = uvm2 + fac2 * epy + fac1r * (+uvm3 * bpx - uvm1 * bpz);
+  const double uvm6 = uvm3 + fac2 * epz + fac1r * (+uvm1 * bpy - uvm2 * bpx);
+  const double taux = fac1 * bpx / c, tauy = fac1 * bpy / c, tauz = fac1 * bpz / c;
+  const double tau2 = taux * taux + tauy * tauy + tauz * tauz;
+  const double ua = (uvm4 * taux + uvm5 * tauy + uvm6 * tauz) / c;
+  const double sigma = 1.0 + (uvm4 * uvm4 + uvm5 * uvm5 + uvm6 * uvm6) / (c * c) - tau2;
+  const double gam2 = 0.5 * (sigma + sqrt(sigma * sigma + 4.0 * (tau2 + ua * ua)));
+  gam = sqrt(gam2);
+  const double txxx = 1.0 / (tau2 + gam2);
+  g[2] = txxx * (gam2 * uvm4 + c * ua * taux + gam * (uvm5 * tauz - uvm6 * tauy));
+  g[3] = txxx * (gam2 * uvm5 + c * ua * tauy + gam * (uvm6 * taux - uvm4 * tauz));
+  g[4] = txxx * (gam2 * uvm6 + c * ua * tauz + gam * (uvm4 * tauy - uvm5 * taux));
+  return 1.0 / gam;
+}
+
+static void particle_solv(const or2d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt,
+                          int vay);
+
+void or2d_particle_solv(const or2d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt) {
+  particle_solv(p, gp, up, uf, cumcnt, 0);
+}
+
+/* particle__solv_vay: 2d/common/particle.f90:182-315 (same field staging and
+ * gather as particle__solv) */
+void or2d_particle_solv_vay(const or2d_params *p, double *gp, const double *up, const double *uf,
+                            const int *cumcnt) {
+  particle_solv(p, gp, up, uf, cumcnt, 1);
+}
+
+static void particle_solv(const or2d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt,
+                          int vay) {
   const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
   const int tw = NX + 2;
   const double d_delx = 1. / p->delx;
@@ -183,6 +222,12 @@ void or2d_particle_solv(const or2d_params *p, double *gp, const double *up,
           const double bpx = GATH(1), bpy = GATH(2), bpz = GATH(3);
           const double epx = GATH(4), epy = GATH(5), epz = GATH(6);
 #undef GATH
+          if (vay) {
+            const double gi = vay_kick(c, fac1, fac2, u, bpx, bpy, bpz, epx, epy, epz, g);
+            g[0] = u[0] + g[2] * delt * gi;
+            g[1] = u[1] + g[3] * delt * gi;
+            continue;
+          }
           double uvm1 = u[2] + fac1 * epx;
           double uvm2 = u[3] + fac1 * epy;
           double uvm3 = u[4] + fac1 * epz;

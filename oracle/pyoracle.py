@@ -100,6 +100,7 @@ def lib():
         L.or2d_fstate_free.argtypes = [S]
         L.or2d_field_consts.argtypes = [P, np.ctypeslib.ndpointer(np.float64)]
         L.or2d_particle_solv.argtypes = [P, dp, dp, dp, ip]
+        L.or2d_particle_solv_vay.argtypes = [P, dp, dp, dp, ip]
         L.or2d_ele_cur.argtypes = [P, dp, dp, dp, ip]
         L.or2d_bc_curre.argtypes = [P, dp]
         L.or2d_bc_dfield.argtypes = [P, dp]
@@ -200,6 +201,9 @@ class Oracle2D:
     def particle_solv(self, gp, up, uf, cumcnt):
         self.L.or2d_particle_solv(C.byref(self.p), gp, up, uf, cumcnt)
 
+    def particle_solv_vay(self, gp, up, uf, cumcnt):
+        self.L.or2d_particle_solv_vay(C.byref(self.p), gp, up, uf, cumcnt)
+
     def ele_cur(self, uj, up, gp, cumcnt):
         self.L.or2d_ele_cur(C.byref(self.p), uj, up, gp, cumcnt)
 
@@ -256,8 +260,9 @@ class Oracle2D:
 
 
 def run_ref2d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
-    """Run the reference's own particle__solv (mode 1), sort__bucket (mode 2) or
-    mom_calc__accl + mom_calc__nvt (mode 3: returns gp, mom) through
+    """Run the reference's own particle__solv (mode 1), sort__bucket (mode 2),
+    mom_calc__accl + mom_calc__nvt (mode 3: returns gp, mom) or
+    particle__solv_vay (mode 4) through
     oracle/_ref/ref_golden2d on raw files in case_dir."""
     os.makedirs(case_dir, exist_ok=True)
     with open(os.path.join(case_dir, "params.txt"), "w") as f:
@@ -266,7 +271,7 @@ def run_ref2d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
         f.write(" ".join(repr(float(p.q[k])) for k in range(p.nsp)) + "\n")
         f.write(" ".join(repr(float(p.r[k])) for k in range(p.nsp)) + "\n")
     up.ravel(order="F").tofile(os.path.join(case_dir, "up.bin"))
-    if mode in (1, 3):
+    if mode in (1, 3, 4):
         uf.ravel(order="F").tofile(os.path.join(case_dir, "uf.bin"))
         cumcnt.ravel(order="F").tofile(os.path.join(case_dir, "cumcnt.bin"))
     if mode in (2, 3):
@@ -274,7 +279,7 @@ def run_ref2d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
     env = dict(os.environ, OMP_NUM_THREADS="1")
     subprocess.run([REF_DRIVER2D, case_dir], check=True, env=env)
     gp = np.fromfile(os.path.join(case_dir, "gp.bin"), dtype=np.float64).reshape(up.shape, order="F")
-    if mode == 1:
+    if mode in (1, 4):
         return gp
     if mode == 3:
         mom = np.fromfile(os.path.join(case_dir, "mom.bin"), dtype=np.float64).reshape(
@@ -419,7 +424,7 @@ def run_ref3d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
         f.write(" ".join(repr(float(p.q[k])) for k in range(p.nsp)) + "\n")
         f.write(" ".join(repr(float(p.r[k])) for k in range(p.nsp)) + "\n")
     up.ravel(order="F").tofile(os.path.join(case_dir, "up.bin"))
-    if mode in (1, 3):
+    if mode in (1, 3, 4):
         uf.ravel(order="F").tofile(os.path.join(case_dir, "uf.bin"))
         cumcnt.ravel(order="F").tofile(os.path.join(case_dir, "cumcnt.bin"))
     if mode in (2, 3):

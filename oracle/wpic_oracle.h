@@ -19,6 +19,10 @@
  * Pinning status of each function (what checks it, see tests/):
  *   or2d_particle_solv    bit-exact vs the reference particle.f90 built with
  *                         amdflang (oracle/_ref), plus closed forms.
+ *   or2d_particle_solv_vay bit-exact vs the reference particle__solv_vay
+ *                         built with amdflang (oracle/_ref, golden fixture),
+ *                         plus closed forms: B = 0 gives u + fac2 E, pure B
+ *                         keeps |u|, the E x B drift velocity is stationary.
  *   or2d_sort_bucket      bit-exact vs the reference sort.f90 (oracle/_ref),
  *                         plus brute-force histogram / stable permutation.
  *   or2d_ele_cur          discrete continuity (rho^{n+1}-rho^n+dt div J = 0),
@@ -82,6 +86,8 @@ void or2d_field_consts(const or2d_params *p, double f[5]);
 /* particle.f90:48-179 */
 void or2d_particle_solv(const or2d_params *p, double *gp, const double *up,
                         const double *uf, const int *cumcnt);
+void or2d_particle_solv_vay(const or2d_params *p, double *gp, const double *up, const double *uf,
+                            const int *cumcnt);
 /* field.f90:189-316 */
 void or2d_ele_cur(const or2d_params *p, double *uj, const double *up,
                   const double *gp, const int *cumcnt);

@@ -1,12 +1,12 @@
 """Write golden vectors from the reference's OWN 2D particle__solv,
-sort__bucket and mom_calc (2d/common/particle.f90:48-179,
+particle__solv_vay, sort__bucket and mom_calc (2d/common/particle.f90:48-315,
 2d/common/sort.f90:36-82, 2d/common/mom_calc.f90:48-249),
 compiled from the reference sources by `make -C oracle ref` and run at one
 OpenMP thread.
 
     python tests/golden/gen_ref2d.py        (needs oracle/_ref/ref_golden2d)
 
-Outputs tests/golden/ref2d_push.npz, ref2d_sort.npz and ref2d_mom.npz: the
+Outputs tests/golden/ref2d_push.npz, ref2d_vay.npz, ref2d_sort.npz and ref2d_mom.npz: the
 inputs, the parameters and the reference's outputs.  The fixtures are data
 only; tests/test_oracle_golden.py checks the oracle against them.
 """
@@ -52,6 +52,29 @@ def push_case(tmp):
     uf = S.random_fields(p, rng, scale=0.3, b0=0.2)
     gp = po.run_ref2d(os.path.join(tmp, "push"), 1, p, up, uf=uf, cumcnt=cumcnt)
     np.savez_compressed(os.path.join(HERE, "ref2d_push.npz"), up=up, uf=uf, cumcnt=cumcnt, np2=np2,
+                        gp=gp, **params_dict(p))
+
+
+def vay_case(tmp):
+    """particle__solv_vay on the push case's kind of state, with strong fields so that the implicit
+    rotation matters (|tau| up to about 1), and a particle at the E x B drift velocity"""
+    rng = np.random.default_rng(20261206)
+    nx, ny, n_ppc = 12, 10, 8
+    q, r = S.weibel_consts(n_ppc)
+    p = po.make_params(nx, ny, np_cap=160, delt=0.5, q=q, r=r)
+    counts = [rng.integers(4, 13, size=(nx, ny)) for _ in range(2)]
+    up, cumcnt, np2 = S.sorted_particles(p, n_ppc, rng, u_scale=(0.05, 0.4), drift=(0.0, 0.5),
+                                         colocate=False, counts=counts)
+    up[0, 0, 0, 0] = float(p.nxgs)
+    up[0, 3, 2, 1] = np.nextafter(float(p.nxgs + 1), 0.0)
+    up[2:5, 5, 3, 1] = [40.0, -25.0, 7.5]
+    up[2:5, 6, 4, 0] = [-3.0, 2.0, -1.0]
+    up[5, np2.max():, :, :] = rng.standard_normal(up[5, np2.max():, :, :].shape)
+    # |fac1 B| ~ 1 for the electrons: fac1 = q/r dt/2
+    bscale = 1.0 / abs(q[1] * 0.25)
+    uf = S.random_fields(p, rng, scale=0.3 * bscale, b0=0.5 * bscale)
+    gp = po.run_ref2d(os.path.join(tmp, "vay"), 4, p, up, uf=uf, cumcnt=cumcnt)
+    np.savez_compressed(os.path.join(HERE, "ref2d_vay.npz"), up=up, uf=uf, cumcnt=cumcnt, np2=np2,
                         gp=gp, **params_dict(p))
 
 
@@ -103,6 +126,7 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/ref_golden2d is missing: run `make -C oracle ref` first")
     with tempfile.TemporaryDirectory() as tmp:
         push_case(tmp)
+        vay_case(tmp)
         sort_case(tmp)
         mom_case(tmp)
-    print("wrote", *(os.path.join(HERE, f"ref2d_{k}.npz") for k in ("push", "sort", "mom")))
+    print("wrote", *(os.path.join(HERE, f"ref2d_{k}.npz") for k in ("push", "vay", "sort", "mom")))

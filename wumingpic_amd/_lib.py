@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libwpic.so")
 MAXSP = 4
 BC_PERIODIC, BC_WALLX = 0, 1
 MODE_FAST, MODE_PARITY, MODE_MIXED = 0, 1, 2
+PUSHER_BORIS, PUSHER_VAY = 0, 1
 E_OK, E_INVAL, E_HIP, E_CG, E_MEM, E_DOMAIN = 0, -1, -2, -3, -4, -5
 
 # every symbol include/wpic.h declares
@@ -23,7 +24,8 @@ EXPORTS = (
     "wpic_upload_up2d", "wpic_download_up2d", "wpic_upload_gp2d", "wpic_download_gp2d",
     "wpic_upload_fields2d", "wpic_download_fields2d", "wpic_download_current2d",
     "wpic_download_dfield2d", "wpic_upload_dfield2d",
-    "wpic_particle_solv2d", "wpic_field_fdtd_i2d", "wpic_bc_particle_x2d", "wpic_bc_particle_y2d",
+    "wpic_particle_solv2d", "wpic_particle_solv_vay2d", "wpic_set_pusher2d", "wpic_field_fdtd_i2d",
+    "wpic_bc_particle_x2d", "wpic_bc_particle_y2d",
     "wpic_sort_bucket2d", "wpic_cg_iters2d", "wpic_step2d",
     "wpic_energy2d", "wpic_gauss2d", "wpic_counts2d", "wpic_ic_beams2d",
     "wpic_set_timing", "wpic_timing",
@@ -121,6 +123,8 @@ def lib():
         "wpic_download_dfield2d": [vp, d],
         "wpic_upload_dfield2d": [vp, d],
         "wpic_particle_solv2d": [vp],
+        "wpic_particle_solv_vay2d": [vp],
+        "wpic_set_pusher2d": [vp, C.c_int],
         "wpic_field_fdtd_i2d": [vp],
         "wpic_bc_particle_x2d": [vp],
         "wpic_bc_particle_y2d": [vp],

@@ -32,6 +32,7 @@ struct PushArgs {
   int* code;
   int* err;
   int ntx, nty;
+  int vay;  // the Vay kick instead of Buneman-Boris (particle__solv_vay)
 };
 
 // register layout of one cell's current block:
@@ -282,8 +283,12 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
         double nxp, nyp, g3, g4, g5;
         if (PUSH) {
           double gmv;
-          boris_push(sf, li, lj, i, j, d_delx, delt, c, unit_c, fac1, txxx, fac2, x, y, a.A.ux[ia], a.A.uy[ia],
-                     a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
+          if (a.vay)
+            vay_push(sf, li, lj, i, j, d_delx, delt, c, fac1, fac2, x, y, a.A.ux[ia], a.A.uy[ia], a.A.uz[ia], nxp,
+                     nyp, g3, g4, g5, gmv);
+          else
+            boris_push(sf, li, lj, i, j, d_delx, delt, c, unit_c, fac1, txxx, fac2, x, y, a.A.ux[ia], a.A.uy[ia],
+                       a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
         } else {
           nxp = a.B.x[pi];
           nyp = a.B.y[pi];
@@ -369,12 +374,15 @@ __device__ inline void deposit2_full(double (*acc)[ACW * ACH], int li, int lj, c
 // Push of one particle of cell (i, j) and, with reflecting walls, the
 // reflection that precedes the deposit in the reconnection order
 // (2d/proj/reconnection/app.f90:101-103; boundary_reconnection.f90:79-91).
-template <int BC>
+template <int BC, bool VAY>
 __device__ inline void push_reflect(const Grid2D& g, const double (*sf)[SFW * SFH], int li, int lj, int i, int j,
                                     bool unit_c, const double* kc, double x, double y, double ux, double uy, double uz,
                                     double& nxp, double& nyp, double& g3, double& g4, double& g5, double& gmv) {
-  boris_push(sf, li, lj, i, j, g.d_delx, g.delt, g.c, unit_c, kc[0], kc[1], kc[2], x, y, ux, uy, uz, nxp, nyp, g3, g4,
-             g5, gmv);
+  if (VAY)
+    vay_push(sf, li, lj, i, j, g.d_delx, g.delt, g.c, kc[0], kc[2], x, y, ux, uy, uz, nxp, nyp, g3, g4, g5, gmv);
+  else
+    boris_push(sf, li, lj, i, j, g.d_delx, g.delt, g.c, unit_c, kc[0], kc[1], kc[2], x, y, ux, uy, uz, nxp, nyp, g3,
+               g4, g5, gmv);
   if (BC == WPIC_BC_WALLX) {
     const int ipos = cell_rn(nxp, g.delx);
     if (ipos < g.nxgs + 1) {
@@ -406,7 +414,7 @@ __device__ inline void push_reflect(const Grid2D& g, const double (*sf)[SFW * SF
 // beyond the queue's capacity a particle adds its block at once.  With FUSED
 // the wraps and the exchange code follow in the same pass; otherwise gp keeps
 // the unwrapped positions as particle__solv leaves them.
-template <int G, bool FUSED, int BC>
+template <int G, bool FUSED, int BC, bool VAY>
 __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
   __shared__ double sf[6][SFW * SFH];
   __shared__ double acc[3][ACW * ACH];
@@ -477,7 +485,7 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
         const int ia = a.ap ? a.ap[pi] : pi;
         const double x = a.A.x[ia], y = a.A.y[ia];
         double nxp, nyp, g3, g4, g5, gmv;
-        push_reflect<FUSED ? BC : WPIC_BC_PERIODIC>(g, sf, li, lj, i, j, unit_c, kc, x, y, a.A.ux[ia], a.A.uy[ia],
+        push_reflect<FUSED ? BC : WPIC_BC_PERIODIC, VAY>(g, sf, li, lj, i, j, unit_c, kc, x, y, a.A.ux[ia], a.A.uy[ia],
                                                     a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
         a.B.ux[pi] = g3;
         a.B.uy[pi] = g4;
@@ -633,6 +641,7 @@ hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused)
   a.err = c->err;
   a.ntx = (c->g.nx + TX - 1) / TX;
   a.nty = (c->g.ny + TY - 1) / TY;
+  a.vay = c->pusher == WPIC_PUSHER_VAY;
   const int nblk = xcd_grid(a.ntx * a.nty);
   const bool parity = c->mode == WPIC_MODE_PARITY && deposit;
   if (parity) {
@@ -641,10 +650,16 @@ hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused)
   }
   if (push && deposit && fused) {
     constexpr int GF = 16;  // a group's 16 lanes read and write whole 128-byte lines
-    if (c->g.bc == WPIC_BC_WALLX)
-      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_WALLX>), dim3(nblk), dim3(NT), 0, c->st, a);
+    const bool wall = c->g.bc == WPIC_BC_WALLX;
+    if (wall && a.vay)
+      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_WALLX, true>), dim3(nblk), dim3(NT), 0, c->st, a);
+    else if (wall)
+      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_WALLX, false>), dim3(nblk), dim3(NT), 0, c->st, a);
+    else if (a.vay)
+      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_PERIODIC, true>), dim3(nblk), dim3(NT), 0, c->st, a);
     else
-      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_PERIODIC>), dim3(nblk), dim3(NT), 0, c->st, a);
+      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_PERIODIC, false>), dim3(nblk), dim3(NT), 0, c->st,
+                         a);
     return hipGetLastError();
   }
   return c->g.bc == WPIC_BC_WALLX ? launch_t<8, false, WPIC_BC_WALLX>(c, push, deposit, fused, a, nblk)

@@ -197,6 +197,7 @@ int wpic_adj_forward2d(wpic_ctx* c, int nsteps, int every, double* wb) {
   if (!c || !wb) return set_error(WPIC_EINVAL, "null argument");
   if (c->mixed || c->g.bc != WPIC_BC_PERIODIC)
     return set_error(WPIC_EINVAL, "the adjoint covers the periodic FP64 loop");
+  if (c->pusher != WPIC_PUSHER_BORIS) return set_error(WPIC_EINVAL, "the adjoint covers the Buneman-Boris kick");
   if (nsteps < 1 || every < 1) return set_error(WPIC_EINVAL, "nsteps and every must be positive");
   int rc = alloc_adj(c, nsteps, every);
   if (rc) return rc;

@@ -311,6 +311,7 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
   std::memset((void*)&c->g, 0, sizeof(c->g));
   c->P = *p;
   c->mode = p->mode;
+  c->pusher = WPIC_PUSHER_BORIS;
   c->mixed = p->mode == WPIC_MODE_MIXED;
   Grid2D& g = c->g;
   g.nx = nx;
@@ -657,6 +658,25 @@ int wpic_particle_solv2d(wpic_ctx* c) {
   c->b_aligned = true;
   c->exch_pending = false;
   return check_err(c);
+}
+
+int wpic_particle_solv_vay2d(wpic_ctx* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
+  const int saved = c->pusher;
+  c->pusher = WPIC_PUSHER_VAY;
+  const int rc = wpic_particle_solv2d(c);
+  c->pusher = saved;
+  return rc;
+}
+
+int wpic_set_pusher2d(wpic_ctx* c, int pusher) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (pusher != WPIC_PUSHER_BORIS && pusher != WPIC_PUSHER_VAY) return set_error(WPIC_EINVAL, "bad pusher");
+  if (c->mixed && pusher != WPIC_PUSHER_BORIS)
+    return set_error(WPIC_EINVAL, "the Vay kick runs in the FP64 modes (the mixed form pushes with Buneman-Boris)");
+  c->pusher = pusher;
+  return 0;
 }
 
 int wpic_field_fdtd_i2d(wpic_ctx* c) {

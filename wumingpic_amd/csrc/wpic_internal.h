@@ -205,6 +205,7 @@ struct wpic_ctx {
   int* rowcnt;                    // [nsp*ny][4] leavers, up, down, incoming
   bool b_valid;       // B holds a gp array
   bool b_aligned;     // B's slots are A's slots (after particle__solv)
+  int pusher;         // WPIC_PUSHER_BORIS or WPIC_PUSHER_VAY
   bool b_accl;        // B is mom_calc__accl's gp: A's positions, A's cell order
   bool exch_pending;  // R / N rows describe B after bc__particle_y
   std::vector<int> h_rows[3];  // host mirrors of the row structures

@@ -1,6 +1,7 @@
 ! shim_modules.f90 -- the reference's 2D module entry points, same names and
 ! argument lists, implemented on the GPU through the C-ABI:
-!   particle:  particle__init, particle__solv      (2d/common/particle.f90:18,48)
+!   particle:  particle__init, particle__solv, particle__solv_vay
+!                                                  (2d/common/particle.f90:18,48,182)
 !   field:     field__init, field__fdtd_i          (2d/common/field.f90:22,66)
 !   sort:      sort__init, sort__bucket            (2d/common/sort.f90:16,36)
 !   mom_calc:  mom_calc__init, mom_calc__accl, mom_calc__nvt
@@ -22,7 +23,7 @@ module particle
   use wpic_shim_state
   implicit none
   private
-  public :: particle__init, particle__solv
+  public :: particle__init, particle__solv, particle__solv_vay
 
 contains
 
@@ -48,6 +49,26 @@ contains
     real(8), intent(in)  :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
     real(8), intent(in)  :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
     real(8), intent(out) :: gp(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    call push(gp, up, uf, cumcnt, nxe, .false.)
+  end subroutine particle__solv
+
+  ! the Vay kick (particle.f90:182-315)
+  subroutine particle__solv_vay(gp, up, uf, cumcnt, nxs, nxe)
+    integer, intent(in)  :: nxs, nxe
+    integer, intent(in)  :: cumcnt(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp)
+    real(8), intent(in)  :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    real(8), intent(in)  :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
+    real(8), intent(out) :: gp(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    call push(gp, up, uf, cumcnt, nxe, .true.)
+  end subroutine particle__solv_vay
+
+  subroutine push(gp, up, uf, cumcnt, nxe, vay)
+    integer, intent(in)  :: nxe
+    integer, intent(in)  :: cumcnt(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp)
+    real(8), intent(in)  :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    real(8), intent(in)  :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
+    real(8), intent(out) :: gp(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    logical, intent(in)  :: vay
     integer(c_int) :: np2(s_nys:s_nye, s_nsp)
     if (.not. have_particle) then
        write(6,*) 'Initialize first by calling particle__init()'
@@ -60,13 +81,17 @@ contains
        call shim_check(wpic_upload_up2d(ctx, up, np2, cumcnt, s_np))
        call shim_check(wpic_upload_fields2d(ctx, uf))
     end if
-    call shim_check(wpic_particle_solv2d(ctx))
+    if (vay) then
+       call shim_check(wpic_particle_solv_vay2d(ctx))
+    else
+       call shim_check(wpic_particle_solv2d(ctx))
+    end if
     if (.not. resident) then
        call shim_check(wpic_download_gp2d(ctx, gp, np2, s_np))
        ! the id slot is copied over the whole capacity (particle.f90:173-177)
        if (s_ndim == 6) gp(6, :, :, :) = up(6, :, :, :)
     end if
-  end subroutine particle__solv
+  end subroutine push
 
 end module particle
 

@@ -181,6 +181,11 @@ module wpic_capi
        type(c_ptr), value :: ctx
      end function wpic_particle_solv2d
 
+     integer(c_int) function wpic_particle_solv_vay2d(ctx) bind(C, name='wpic_particle_solv_vay2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function wpic_particle_solv_vay2d
+
      integer(c_int) function wpic_field_fdtd_i2d(ctx) bind(C, name='wpic_field_fdtd_i2d')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

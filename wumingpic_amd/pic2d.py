@@ -130,6 +130,14 @@ class Pic2D:
         """particle__solv(gp, up, uf, cumcnt, nxs, nxe): 2d/common/particle.f90:48"""
         check(self._L.wpic_particle_solv2d(self._h))
 
+    def particle_solv_vay(self):
+        """particle__solv_vay(gp, up, uf, cumcnt, nxs, nxe): 2d/common/particle.f90:182"""
+        check(self._L.wpic_particle_solv_vay2d(self._h))
+
+    def set_pusher(self, pusher):
+        """the kick of particle_solv and step: _lib.PUSHER_BORIS or _lib.PUSHER_VAY"""
+        check(self._L.wpic_set_pusher2d(self._h, int(pusher)))
+
     def field_fdtd_i(self):
         """field__fdtd_i(uf, up, gp, cumcnt, ...): 2d/common/field.f90:66.
         Returns the CG iteration count of each component."""
