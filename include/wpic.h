@@ -28,6 +28,11 @@ extern "C" {
 /* boundary kinds */
 #define WPIC_BC_PERIODIC 0 /* 2d/common/boundary_periodic.f90 */
 #define WPIC_BC_WALLX 1    /* 2d/proj/reconnection/boundary_reconnection.f90 */
+/* 2d/proj/shock/boundary_shock.f90 with the box fully expanded (nxe = nxge):
+ * the reconnection's left wall, the injection edge at the right
+ * (boundary_shock__injection as the step's x boundary, zero field and phi
+ * ghosts beyond nxe); FP64 modes */
+#define WPIC_BC_SHOCK 2
 
 /* particle kicks of the FP64 modes (wpic_set_pusher2d) */
 #define WPIC_PUSHER_BORIS 0 /* particle__solv: 2d/common/particle.f90:48-179 */
@@ -114,6 +119,13 @@ int wpic_field_fdtd_i2d(wpic_ctx *ctx);
 /* boundary_periodic__particle_x(gp, np2): boundary_periodic.f90:61, or
  * boundary_reconnection__particle_x(gp, np2, nxs, nxe): :61 */
 int wpic_bc_particle_x2d(wpic_ctx *ctx);
+/* boundary_shock__injection(up, np2, nxs, nxe, u0):
+ * 2d/proj/shock/boundary_shock.f90:255 on gp, with nxs = nxgs, nxe = nxge;
+ * also sets the u0 the step uses.  WPIC_BC_SHOCK only. */
+int wpic_bc_injection2d(wpic_ctx *ctx, double u0);
+/* the injection 4-velocity u0 (< 0: inflow from the right, the shock app's
+ * -|u_inject|, 2d/proj/shock/app.f90:319) of the step's injection edge */
+int wpic_set_injection2d(wpic_ctx *ctx, double u0);
 /* boundary_periodic__particle_y(gp, np2): boundary_periodic.f90:99 */
 int wpic_bc_particle_y2d(wpic_ctx *ctx);
 /* sort__bucket(up, gp, cumcnt, np2, nxs, nxe): 2d/common/sort.f90:36 */

@@ -425,7 +425,9 @@ void or2d_bc_curre(const or2d_params *p, double *uj) {
       UJ(cc, i, nys - 1) = snd[6 * (i - (nxs - 2)) + cc + 2];
     }
   free(snd);
-  if (p->bc == OR_BC_WALLX) return;
+  /* walls (reconnection, shock): no x fold (boundary_shock.f90:410-548 is
+   * the reconnection routine) */
+  if (p->bc != OR_BC_PERIODIC) return;
   /* x fold then x copy over all rows (boundary_periodic.f90:495-506) */
   for (int j = nys - 2; j <= nye + 2; j++)
     for (int cc = 1; cc <= 3; cc++) UJ(cc, nxe - 1, j) = UJ(cc, nxe - 1, j) + UJ(cc, nxs - 2, j);
@@ -487,6 +489,19 @@ void or2d_bc_dfield(const or2d_params *p, double *df) {
     }
     return;
   }
+  if (p->bc == OR_BC_SHOCK) {
+    /* boundary_shock.f90:395-405: the reconnection's left wall, zero ghosts
+     * beyond the injection edge */
+    for (int j = nys - 2; j <= nye + 2; j++) {
+      DF(1, nxs - 1, j) = -DF(1, nxs, j);
+      for (int cc = 2; cc <= 4; cc++) DF(cc, nxs - 1, j) = DF(cc, nxs + 1, j);
+      for (int cc = 5; cc <= 6; cc++) DF(cc, nxs - 1, j) = -DF(cc, nxs, j);
+      DF(1, nxe + 1, j) = 0.0;
+      for (int cc = 2; cc <= 4; cc++) DF(cc, nxe + 1, j) = 0.0;
+      for (int cc = 5; cc <= 6; cc++) DF(cc, nxe + 1, j) = 0.0;
+    }
+    return;
+  }
   for (int j = nys - 2; j <= nye + 2; j++)
     for (int cc = 1; cc <= 6; cc++) DF(cc, nxs - 2, j) = DF(cc, nxe - 1, j);
   for (int j = nys - 2; j <= nye + 2; j++)
@@ -522,6 +537,14 @@ void or2d_bc_phi(const or2d_params *p, double *phi, int l) {
         PH(nxs - 1, j) = PH(nxs + 1, j);
         PH(nxe + 1, j) = PH(nxe - 1, j);
       }
+    }
+    return;
+  }
+  if (p->bc == OR_BC_SHOCK) {
+    /* boundary_shock.f90:603-621 */
+    for (int j = nys - 1; j <= nye + 1; j++) {
+      PH(nxs - 1, j) = l == 1 ? -PH(nxs, j) : PH(nxs + 1, j);
+      PH(nxe + 1, j) = 0.0;
     }
     return;
   }
@@ -698,7 +721,7 @@ void or2d_bc_particle_x(const or2d_params *p, double *up, const int *np2) {
     for (int j = p->nys; j <= p->nye; j++)
       for (int ii = 1; ii <= np2[iNP(p, j, isp)]; ii++) {
         double *u = &up[iUP(p, 1, ii, j, isp)];
-        if (p->bc == OR_BC_WALLX) {
+        if (p->bc != OR_BC_PERIODIC) { /* boundary_shock__particle_x is the same wall pair */
           const int nxs = nxgs, nxe = nxge;
           const int ipos = (int)(u[0] / delx);
           if (ipos < nxs + 1) {
@@ -722,6 +745,33 @@ void or2d_bc_particle_x(const or2d_params *p, double *up, const int *np2) {
         }
       }
   }
+}
+
+/* boundary_shock__injection (2d/proj/shock/boundary_shock.f90:255-297) on
+ * the whole box (nxs = nxgs, nxe = nxge): the reconnection's left wall, and a
+ * mirror moving with the inflow at xend = nxe dx + v0 dt that sends ux to
+ * 2 u0 - ux. */
+void or2d_bc_injection(const or2d_params *p, double *up, const int *np2) {
+  const int nxs = p->nxgs, nxe = p->nxge;
+  const double delx = p->delx, c = p->c, u0 = p->u0;
+  const double xend = nxe * delx + u0 / sqrt(1 + (u0 * u0) / (c * c)) * p->delt;
+  for (int isp = 1; isp <= p->nsp; isp++)
+    for (int j = p->nys; j <= p->nye; j++)
+      for (int ii = 1; ii <= np2[iNP(p, j, isp)]; ii++) {
+        double *u = &up[iUP(p, 1, ii, j, isp)];
+        const int ipos = (int)(u[0] / delx);
+        if (ipos < nxs + 1) {
+          u[0] = +(double)(2.0f * (float)(nxs + 1)) * delx - u[0];
+          u[2] = -u[2];
+          u[3] = -u[3];
+          u[4] = -u[4];
+        } else if (u[0] > xend) {
+          u[0] = +2. * xend - u[0];
+          u[2] = +2. * u0 - u[2];
+          u[3] = -u[3];
+          u[4] = -u[4];
+        }
+      }
 }
 
 /* ---------------------------------------------------------------------------
@@ -878,6 +928,10 @@ int or2d_step(const or2d_params *p, or2d_fstate *st, double *up, double *gp, dou
     or2d_particle_solv(p, gp, up, uf, cumcnt);
     if (p->bc == OR_BC_WALLX) {
       or2d_bc_particle_x(p, gp, np2);
+      if (or2d_field_fdtd_i(p, st, uf, up, gp, cumcnt)) return -1;
+    } else if (p->bc == OR_BC_SHOCK) {
+      /* 2d/proj/shock/app.f90:111-116 (the box fully expanded) */
+      or2d_bc_injection(p, gp, np2);
       if (or2d_field_fdtd_i(p, st, uf, up, gp, cumcnt)) return -1;
     } else {
       if (or2d_field_fdtd_i(p, st, uf, up, gp, cumcnt)) return -1;
@@ -1069,8 +1123,9 @@ void or2d_mom_nvt(const or2d_params *p, double *mom, const double *up, const int
  * edge), then per species the y fold, nys-1 onto nye and then nye+1 onto nys,
  * over all x nodes. */
 void or2d_bc_mom(const or2d_params *p, double *mom) {
-  const int xa = p->bc == OR_BC_WALLX ? p->nxgs - 1 : p->nxge + 1;
-  const int xb = p->bc == OR_BC_WALLX ? p->nxge + 1 : p->nxgs - 1;
+  /* boundary_shock__mom (boundary_shock.f90:628-693) is the reconnection's */
+  const int xa = p->bc != OR_BC_PERIODIC ? p->nxgs - 1 : p->nxge + 1;
+  const int xb = p->bc != OR_BC_PERIODIC ? p->nxge + 1 : p->nxgs - 1;
   for (int isp = 1; isp <= p->nsp; isp++)
     for (int j = p->nys - 1; j <= p->nye + 1; j++)
       for (int k = 1; k <= 7; k++) {

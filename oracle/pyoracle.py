@@ -21,6 +21,7 @@ REF_DRIVER3D = os.path.join(HERE, "_ref", "ref_golden3d")
 MAXSP = 4
 BC_PERIODIC = 0
 BC_WALLX = 1
+BC_SHOCK = 2
 
 
 class Params2D(C.Structure):
@@ -30,7 +31,7 @@ class Params2D(C.Structure):
         ("nys", C.c_int), ("nye", C.c_int),
         ("delx", C.c_double), ("delt", C.c_double), ("c", C.c_double), ("gfac", C.c_double),
         ("q", C.c_double * MAXSP), ("r", C.c_double * MAXSP),
-        ("bc", C.c_int),
+        ("bc", C.c_int), ("u0", C.c_double),
     ]
 
     @property
@@ -101,6 +102,7 @@ def lib():
         L.or2d_field_consts.argtypes = [P, np.ctypeslib.ndpointer(np.float64)]
         L.or2d_particle_solv.argtypes = [P, dp, dp, dp, ip]
         L.or2d_particle_solv_vay.argtypes = [P, dp, dp, dp, ip]
+        L.or2d_bc_injection.argtypes = [P, dp, ip]
         L.or2d_ele_cur.argtypes = [P, dp, dp, dp, ip]
         L.or2d_bc_curre.argtypes = [P, dp]
         L.or2d_bc_dfield.argtypes = [P, dp]
@@ -149,7 +151,7 @@ def _check(rc):
 
 
 def make_params(nx, ny, nsp=2, ndim=6, np_cap=None, delx=1.0, delt=0.5, c=1.0, gfac=0.501,
-                q=(0.0, 0.0), r=(1.0, 1.0), bc=BC_PERIODIC, n_ppc=None):
+                q=(0.0, 0.0), r=(1.0, 1.0), bc=BC_PERIODIC, n_ppc=None, u0=0.0):
     p = Params2D()
     p.ndim, p.nsp = ndim, nsp
     p.nxgs, p.nxge = 2, 2 + nx - 1
@@ -163,6 +165,7 @@ def make_params(nx, ny, nsp=2, ndim=6, np_cap=None, delx=1.0, delt=0.5, c=1.0, g
         p.q[k] = q[k]
         p.r[k] = r[k]
     p.bc = bc
+    p.u0 = u0
     return p
 
 
@@ -226,6 +229,9 @@ class Oracle2D:
 
     def bc_particle_x(self, up, np2):
         self.L.or2d_bc_particle_x(C.byref(self.p), up, np2)
+
+    def bc_injection(self, up, np2):
+        self.L.or2d_bc_injection(C.byref(self.p), up, np2)
 
     def bc_particle_y(self, up, np2):
         _check(self.L.or2d_bc_particle_y(C.byref(self.p), up, np2))

@@ -23,6 +23,13 @@
  *                         built with amdflang (oracle/_ref, golden fixture),
  *                         plus closed forms: B = 0 gives u + fac2 E, pure B
  *                         keeps |u|, the E x B drift velocity is stationary.
+ *   OR_BC_SHOCK parts (or2d_bc_injection; the right-edge zeros of bc_dfield
+ *                         and bc_phi): the left edge is the reconnection wall
+ *                         (pinned with it); the injection is an involution and
+ *                         with u0 = 0 a static mirror at nxe; the zero ghosts
+ *                         are checked after a field solve.  No reference build
+ *                         (boundary_shock.f90 needs MPI): parity unpinned
+ *                         beyond those properties.
  *   or2d_sort_bucket      bit-exact vs the reference sort.f90 (oracle/_ref),
  *                         plus brute-force histogram / stable permutation.
  *   or2d_ele_cur          discrete continuity (rho^{n+1}-rho^n+dt div J = 0),
@@ -57,7 +64,7 @@ extern "C" {
 
 /* boundary kinds: boundary_periodic (2d/common/boundary_periodic.f90) and
  * boundary_reconnection (2d/proj/reconnection/boundary_reconnection.f90) */
-enum { OR_BC_PERIODIC = 0, OR_BC_WALLX = 1 };
+enum { OR_BC_PERIODIC = 0, OR_BC_WALLX = 1, OR_BC_SHOCK = 2 };
 
 typedef struct {
   int ndim, np, nsp;
@@ -65,6 +72,7 @@ typedef struct {
   double delx, delt, c, gfac;
   double q[OR_MAXSP], r[OR_MAXSP];
   int bc;
+  double u0; /* OR_BC_SHOCK: the injection 4-velocity (negative: inflow from the right) */
 } or2d_params;
 
 /* persistent module state of field.f90 (df, gkl, uj are `save`d) */
@@ -88,6 +96,8 @@ void or2d_particle_solv(const or2d_params *p, double *gp, const double *up,
                         const double *uf, const int *cumcnt);
 void or2d_particle_solv_vay(const or2d_params *p, double *gp, const double *up, const double *uf,
                             const int *cumcnt);
+/* boundary_shock__injection: 2d/proj/shock/boundary_shock.f90:255-297 */
+void or2d_bc_injection(const or2d_params *p, double *up, const int *np2);
 /* field.f90:189-316 */
 void or2d_ele_cur(const or2d_params *p, double *uj, const double *up,
                   const double *gp, const int *cumcnt);

@@ -2,7 +2,7 @@
 of include/wpic.h (2D) and include/wpic3d.h (3D), with host-side mirrors of
 the reference's module entry points (pic2d.Pic2D, pic3d.Pic3D) and an
 ISO_C_BINDING Fortran shim (fortran/)."""
-from ._lib import BC_PERIODIC, BC_WALLX, MODE_FAST, MODE_MIXED, MODE_PARITY, WpicError  # noqa: F401
+from ._lib import BC_PERIODIC, BC_SHOCK, BC_WALLX, MODE_FAST, MODE_MIXED, MODE_PARITY, WpicError  # noqa: F401
 
 __all__ = ["BC_PERIODIC", "BC_WALLX", "MODE_FAST", "MODE_MIXED", "MODE_PARITY", "WpicError", "Pic2D", "Pic3D",
            "weibel_beams_2d", "weibel_beams_3d"]

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwpic.so")
 
 MAXSP = 4
-BC_PERIODIC, BC_WALLX = 0, 1
+BC_PERIODIC, BC_WALLX, BC_SHOCK = 0, 1, 2
 MODE_FAST, MODE_PARITY, MODE_MIXED = 0, 1, 2
 PUSHER_BORIS, PUSHER_VAY = 0, 1
 E_OK, E_INVAL, E_HIP, E_CG, E_MEM, E_DOMAIN = 0, -1, -2, -3, -4, -5
@@ -25,7 +25,7 @@ EXPORTS = (
     "wpic_upload_fields2d", "wpic_download_fields2d", "wpic_download_current2d",
     "wpic_download_dfield2d", "wpic_upload_dfield2d",
     "wpic_particle_solv2d", "wpic_particle_solv_vay2d", "wpic_set_pusher2d", "wpic_field_fdtd_i2d",
-    "wpic_bc_particle_x2d", "wpic_bc_particle_y2d",
+    "wpic_bc_particle_x2d", "wpic_bc_particle_y2d", "wpic_bc_injection2d", "wpic_set_injection2d",
     "wpic_sort_bucket2d", "wpic_cg_iters2d", "wpic_step2d",
     "wpic_energy2d", "wpic_gauss2d", "wpic_counts2d", "wpic_ic_beams2d",
     "wpic_set_timing", "wpic_timing",
@@ -127,6 +127,8 @@ def lib():
         "wpic_set_pusher2d": [vp, C.c_int],
         "wpic_field_fdtd_i2d": [vp],
         "wpic_bc_particle_x2d": [vp],
+        "wpic_bc_injection2d": [vp, C.c_double],
+        "wpic_set_injection2d": [vp, C.c_double],
         "wpic_bc_particle_y2d": [vp],
         "wpic_sort_bucket2d": [vp],
         "wpic_cg_iters2d": [vp, np.ctypeslib.ndpointer(np.int32)],

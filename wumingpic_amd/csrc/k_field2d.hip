@@ -93,6 +93,15 @@ __global__ void k_dfield_x(Grid2D g, double* df) {
   const int J = g.nys - 2 + t;
   const size_t pl = (size_t)g.fw * g.fh;
   const int nxs = g.nxgs, nxe = g.nxge;
+  if (g.bc == WPIC_BC_SHOCK) {
+    // boundary_shock.f90:395-405: the left wall as the reconnection's, zeros
+    // beyond the injection edge
+    df[fidx(g, nxs - 1, J)] = -df[fidx(g, nxs, J)];
+    for (int c = 1; c <= 3; c++) df[pl * c + fidx(g, nxs - 1, J)] = df[pl * c + fidx(g, nxs + 1, J)];
+    for (int c = 4; c <= 5; c++) df[pl * c + fidx(g, nxs - 1, J)] = -df[pl * c + fidx(g, nxs, J)];
+    for (int c = 0; c < 6; c++) df[pl * c + fidx(g, nxe + 1, J)] = 0.0;
+    return;
+  }
   if (g.bc == WPIC_BC_WALLX) {
     double* d1 = df;
     d1[fidx(g, nxs - 1, J)] = -d1[fidx(g, nxs, J)];
@@ -128,11 +137,20 @@ struct CgArgs {
 
 // value the boundary routine would have put at (i, j) for component l (0..2);
 // (i, j) is at most one cell outside the interior, in x or in y
+// (sgn = 0: the boundary routine puts a zero there)
 __device__ inline int nbr_map(const Grid2D& g, int i, int j, int l, double& sgn) {
   sgn = 1.0;
   if (j < g.nys) j = g.nye;
   else if (j > g.nye) j = g.nys;
-  if (g.bc == WPIC_BC_WALLX) {
+  if (g.bc == WPIC_BC_SHOCK) {
+    // boundary_shock.f90:603-621
+    if (i < g.nxgs) {
+      if (l == 0) { sgn = -1.0; i = g.nxgs; } else i = g.nxgs + 1;
+    } else if (i > g.nxge) {
+      sgn = 0.0;
+      i = g.nxge;
+    }
+  } else if (g.bc == WPIC_BC_WALLX) {
     if (i < g.nxgs) {
       if (l == 0) { sgn = -1.0; i = g.nxgs; } else i = g.nxgs + 1;
     } else if (i > g.nxge) {
@@ -145,10 +163,13 @@ __device__ inline int nbr_map(const Grid2D& g, int i, int j, int l, double& sgn)
   return cidx(g, i, j);
 }
 
+// the ghost value for the sign nbr_map returned: v, -v or a +0 zero
+__device__ inline double signed_val(double sgn, double v) { return sgn > 0.0 ? v : (sgn < 0.0 ? -v : 0.0); }
+
 __device__ inline double nbr(const Grid2D& g, const double* v, int i, int j, int l) {
   double s;
   const int k = nbr_map(g, i, j, l, s);
-  return s < 0.0 ? -v[k] : v[k];
+  return signed_val(s, v[k]);
 }
 
 // gkl (field.f90:125-146), b = f5*gkl, warm start phi = df(l), sum b^2
@@ -283,13 +304,13 @@ __global__ __launch_bounds__(256) void k_cg_a(CgArgs a) {
     double sg;
     int m;
     m = nbr_map(g, i, j - 1, l, sg);
-    const double pS = sg * (r[m] + bv * po[m]);
+    const double pS = signed_val(sg, r[m] + bv * po[m]);
     m = nbr_map(g, i - 1, j, l, sg);
-    const double pW = sg * (r[m] + bv * po[m]);
+    const double pW = signed_val(sg, r[m] + bv * po[m]);
     m = nbr_map(g, i + 1, j, l, sg);
-    const double pE = sg * (r[m] + bv * po[m]);
+    const double pE = signed_val(sg, r[m] + bv * po[m]);
     m = nbr_map(g, i, j + 1, l, sg);
-    const double pN = sg * (r[m] + bv * po[m]);
+    const double pN = signed_val(sg, r[m] + bv * po[m]);
     const double pC = r[k] + bv * po[k];
     const double av = -pS - pW + f4 * pC - pE - pN;
     pn[k] = pC;

@@ -340,7 +340,7 @@ __global__ void k_mom_bc_x(Grid2D g, double* mom) {
   if (t >= 7 * mh * g.nsp) return;
   const int k = t % 7, jj = (t / 7) % mh, s = t / (7 * mh);
   const int j = g.nys - 1 + jj;
-  const bool wall = g.bc == WPIC_BC_WALLX;
+  const bool wall = walls(g);
   const int xa = wall ? g.nxgs - 1 : g.nxge + 1, xb = wall ? g.nxge + 1 : g.nxgs - 1;
   mom[midx(g, g.nxgs, j, s) + k] = mom[midx(g, g.nxgs, j, s) + k] + mom[midx(g, xa, j, s) + k];
   mom[midx(g, g.nxge, j, s) + k] = mom[midx(g, g.nxge, j, s) + k] + mom[midx(g, xb, j, s) + k];

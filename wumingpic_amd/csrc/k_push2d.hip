@@ -298,16 +298,7 @@ __global__ __launch_bounds__(NT) void k_push_deposit(PushArgs a) {
         }
         // reflecting walls act before the deposit in the reconnection order
         // (2d/proj/reconnection/app.f90:101-103; boundary_reconnection.f90:79-91)
-        if (FUSED && BC == WPIC_BC_WALLX) {
-          const int ipos = cell_rn(nxp, g.delx);
-          if (ipos < g.nxgs + 1) {
-            nxp = g.wall_lo - nxp;
-            g3 = -g3; g4 = -g4; g5 = -g5;
-          } else if (ipos >= g.nxge - 1) {
-            nxp = g.wall_hi - nxp;
-            g3 = -g3; g4 = -g4; g5 = -g5;
-          }
-        }
+        if (FUSED) reflect_x<BC>(g, nxp, g3, g4, g5);
         if (PUSH) {
           a.B.x[pi] = nxp;
           a.B.y[pi] = nyp;
@@ -383,16 +374,8 @@ __device__ inline void push_reflect(const Grid2D& g, const double (*sf)[SFW * SF
   else
     boris_push(sf, li, lj, i, j, g.d_delx, g.delt, g.c, unit_c, kc[0], kc[1], kc[2], x, y, ux, uy, uz, nxp, nyp, g3,
                g4, g5, gmv);
-  if (BC == WPIC_BC_WALLX) {
-    const int ipos = cell_rn(nxp, g.delx);
-    if (ipos < g.nxgs + 1) {
-      nxp = g.wall_lo - nxp;
-      g3 = -g3; g4 = -g4; g5 = -g5;
-    } else if (ipos >= g.nxge - 1) {
-      nxp = g.wall_hi - nxp;
-      g3 = -g3; g4 = -g4; g5 = -g5;
-    }
-  }
+  reflect_x<BC>(g, nxp, g3, g4, g5);
+
 }
 
 // Fast mode, push and deposit in one pass over each particle.  The push is the
@@ -645,13 +628,18 @@ hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused)
   const int nblk = xcd_grid(a.ntx * a.nty);
   const bool parity = c->mode == WPIC_MODE_PARITY && deposit;
   if (parity) {
+    if (c->g.bc == WPIC_BC_SHOCK) return launch_t<1, true, WPIC_BC_SHOCK>(c, push, deposit, fused, a, nblk);
     return c->g.bc == WPIC_BC_WALLX ? launch_t<1, true, WPIC_BC_WALLX>(c, push, deposit, fused, a, nblk)
                                     : launch_t<1, true, WPIC_BC_PERIODIC>(c, push, deposit, fused, a, nblk);
   }
   if (push && deposit && fused) {
     constexpr int GF = 16;  // a group's 16 lanes read and write whole 128-byte lines
     const bool wall = c->g.bc == WPIC_BC_WALLX;
-    if (wall && a.vay)
+    if (c->g.bc == WPIC_BC_SHOCK && a.vay)
+      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_SHOCK, true>), dim3(nblk), dim3(NT), 0, c->st, a);
+    else if (c->g.bc == WPIC_BC_SHOCK)
+      hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_SHOCK, false>), dim3(nblk), dim3(NT), 0, c->st, a);
+    else if (wall && a.vay)
       hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_WALLX, true>), dim3(nblk), dim3(NT), 0, c->st, a);
     else if (wall)
       hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_WALLX, false>), dim3(nblk), dim3(NT), 0, c->st, a);
@@ -662,6 +650,7 @@ hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused)
                          a);
     return hipGetLastError();
   }
+  if (c->g.bc == WPIC_BC_SHOCK) return launch_t<8, false, WPIC_BC_SHOCK>(c, push, deposit, fused, a, nblk);
   return c->g.bc == WPIC_BC_WALLX ? launch_t<8, false, WPIC_BC_WALLX>(c, push, deposit, fused, a, nblk)
                                   : launch_t<8, false, WPIC_BC_PERIODIC>(c, push, deposit, fused, a, nblk);
 }

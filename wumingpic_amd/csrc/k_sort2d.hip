@@ -31,28 +31,26 @@ struct SortArgs {
 
 __device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
-// boundary_periodic__particle_x (round-down wrap) or
-// boundary_reconnection__particle_x (reflection, round-to-nearest) on B
-__global__ __launch_bounds__(SB) void k_wrapx(SortArgs a) {
+// boundary_periodic__particle_x (round-down wrap), the walls of
+// boundary_reconnection__particle_x / boundary_shock__particle_x, or (inj)
+// boundary_shock__injection, on B
+__global__ __launch_bounds__(SB) void k_wrapx(SortArgs a, int inj) {
   const Grid2D& g = a.g;
   const int rw = blockIdx.x;
   const int base = a.rowbaseB[rw], n = a.np2B[rw];
   for (int k = threadIdx.x; k < n; k += SB) {
     const int p = base + k;
     double x = a.B.x[p];
-    if (g.bc == WPIC_BC_WALLX) {
-      const int ipos = cell_rn(x, g.delx);
-      if (ipos < g.nxgs + 1) {
-        a.B.x[p] = g.wall_lo - x;
-        a.B.ux[p] = -a.B.ux[p];
-        a.B.uy[p] = -a.B.uy[p];
-        a.B.uz[p] = -a.B.uz[p];
-      } else if (ipos >= g.nxge - 1) {
-        a.B.x[p] = g.wall_hi - x;
-        a.B.ux[p] = -a.B.ux[p];
-        a.B.uy[p] = -a.B.uy[p];
-        a.B.uz[p] = -a.B.uz[p];
-      }
+    if (walls(g)) {
+      double ux = a.B.ux[p], uy = a.B.uy[p], uz = a.B.uz[p];
+      if (inj)
+        reflect_x<WPIC_BC_SHOCK, true>(g, x, ux, uy, uz);
+      else
+        reflect_x<WPIC_BC_SHOCK, false>(g, x, ux, uy, uz);
+      a.B.x[p] = x;
+      a.B.ux[p] = ux;
+      a.B.uy[p] = uy;
+      a.B.uz[p] = uz;
     } else {
       const int ipos = cell_rd(x, g.delx);
       if (ipos < g.nxgs)
@@ -456,9 +454,9 @@ static SortArgs sort_args(wpic_ctx* c) {
   return a;
 }
 
-hipError_t launch_wrapx(wpic_ctx* c) {
+hipError_t launch_wrapx(wpic_ctx* c, bool injection) {
   SortArgs a = sort_args(c);
-  hipLaunchKernelGGL(k_wrapx, dim3(c->g.nsp * c->g.ny), dim3(SB), 0, c->st, a);
+  hipLaunchKernelGGL(k_wrapx, dim3(c->g.nsp * c->g.ny), dim3(SB), 0, c->st, a, injection ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -301,7 +301,10 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
     return set_error(WPIC_EINVAL, "delx must be 1: sort__bucket bins by int(x) (2d/common/sort.f90:60)");
   if (!(p->c * p->delt <= p->delx))
     return set_error(WPIC_EINVAL, "c*delt must not exceed delx (particles move less than one cell per step)");
-  if (p->bc != WPIC_BC_PERIODIC && p->bc != WPIC_BC_WALLX) return set_error(WPIC_EINVAL, "unknown boundary kind");
+  if (p->bc != WPIC_BC_PERIODIC && p->bc != WPIC_BC_WALLX && p->bc != WPIC_BC_SHOCK)
+    return set_error(WPIC_EINVAL, "unknown boundary kind");
+  if (p->bc == WPIC_BC_SHOCK && p->mode == WPIC_MODE_MIXED)
+    return set_error(WPIC_EINVAL, "the shock boundaries run in the FP64 modes");
   if (p->mode != WPIC_MODE_FAST && p->mode != WPIC_MODE_PARITY && p->mode != WPIC_MODE_MIXED)
     return set_error(WPIC_EINVAL, "bad mode");
   if (p->capacity <= 0 || (long long)p->capacity * p->nsp >= (1ll << 31))
@@ -345,6 +348,8 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
   g.spany = rd_mul((double)ny, p->delx);
   g.wall_lo = (double)(2.0f * (float)(p->nxgs + 1)) * p->delx;
   g.wall_hi = (double)(2.0f * (float)(p->nxge - 1)) * p->delx;
+  g.u0 = 0.0;
+  g.xend = p->nxge * p->delx;
   for (int s = 0; s < WPIC_MAXSP; s++) {
     g.q[s] = p->q[s];
     g.r[s] = p->r[s];
@@ -696,7 +701,36 @@ int wpic_bc_particle_x2d(wpic_ctx* c) {
   if (c->mixed) return mixed_only_step();
   if (!c->b_valid || c->exch_pending) return set_error(WPIC_EINVAL, "bc__particle_x needs gp before the y exchange");
   c->b_accl = false;
-  HIPCHK(launch_wrapx(c));
+  HIPCHK(launch_wrapx(c, false));
+  return check_err(c);
+}
+
+namespace {
+// xend = nxe dx + v0 dt with v0 = u0 / gamma0 (boundary_shock.f90:269)
+void set_u0(wpic_ctx* c, double u0) {
+  const double delx = c->g.delx, cc = c->g.c;
+  c->g.u0 = u0;
+  c->g.xend = c->g.nxge * delx + u0 / sqrt(1 + (u0 * u0) / (cc * cc)) * c->g.delt;
+}
+}  // namespace
+
+int wpic_set_injection2d(wpic_ctx* c, double u0) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->g.bc != WPIC_BC_SHOCK) return set_error(WPIC_EINVAL, "the injection edge needs WPIC_BC_SHOCK");
+  if (!std::isfinite(u0)) return set_error(WPIC_EINVAL, "u0 must be finite");
+  set_u0(c, u0);
+  return 0;
+}
+
+int wpic_bc_injection2d(wpic_ctx* c, double u0) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
+  if (c->g.bc != WPIC_BC_SHOCK) return set_error(WPIC_EINVAL, "bc__injection needs WPIC_BC_SHOCK");
+  if (!std::isfinite(u0)) return set_error(WPIC_EINVAL, "u0 must be finite");
+  if (!c->b_valid || c->exch_pending) return set_error(WPIC_EINVAL, "bc__injection needs gp before the y exchange");
+  set_u0(c, u0);
+  c->b_accl = false;
+  HIPCHK(launch_wrapx(c, true));
   return check_err(c);
 }
 

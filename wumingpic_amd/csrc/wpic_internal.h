@@ -26,10 +26,15 @@ struct Grid2D {
   double f1, f2, f3, f4, f5, pi4dt;  // field.f90:53-57; 4*pi*delt
   double spanx, spany;               // nx*delx, ny*delx rounded down
   double wall_lo, wall_hi;           // 2*(nxs+1)*delx, 2*(nxe-1)*delx
+  double u0, xend;                   // WPIC_BC_SHOCK: injection 4-velocity, moving mirror position
   double q[WPIC_MAXSP], r[WPIC_MAXSP];
   long long ncap;                    // particle capacity per species
   double cg_err;                     // CG relative tolerance (field.f90:339: 1e-6)
 };
+
+// reflecting x walls: the reconnection's pair, or the shock's left wall and
+// injection edge (their current, moment and y boundaries are the same)
+__host__ __device__ inline bool walls(const Grid2D& g) { return g.bc != WPIC_BC_PERIODIC; }
 
 __host__ __device__ inline int fidx(const Grid2D& g, int i, int j) {
   return (i - (g.nxgs - 2)) + g.fw * (j - (g.nys - 2));
@@ -114,6 +119,31 @@ __host__ __device__ inline double rd_div(double a, double b) {
 // x itself in every rounding mode
 __host__ __device__ inline int cell_rd(double x, double delx) { return (int)(delx == 1.0 ? x : rd_div(x, delx)); }
 __host__ __device__ inline int cell_rn(double x, double delx) { return (int)(delx == 1.0 ? x : x / delx); }
+
+// The x boundary that acts on a pushed particle before the deposit:
+// reconnection walls (boundary_reconnection.f90:79-91, WPIC_BC_WALLX) or the
+// shock's left wall and the mirror moving with the inflow at xend, which sends
+// ux to 2 u0 - ux (boundary_shock__injection, boundary_shock.f90:270-293,
+// WPIC_BC_SHOCK).  INJ false gives boundary_shock__particle_x's wall pair.
+template <int BC, bool INJ = true>
+__host__ __device__ inline void reflect_x(const Grid2D& g, double& x, double& ux, double& uy, double& uz) {
+  if (BC == WPIC_BC_PERIODIC) return;
+  const int ipos = cell_rn(x, g.delx);
+  if (ipos < g.nxgs + 1) {
+    x = g.wall_lo - x;
+    ux = -ux; uy = -uy; uz = -uz;
+  } else if (BC == WPIC_BC_SHOCK && INJ) {
+    if (x > g.xend) {
+      x = +2. * g.xend - x;
+      ux = +2. * g.u0 - ux;
+      uy = -uy; uz = -uz;
+    }
+  } else if (ipos >= g.nxge - 1) {
+    x = g.wall_hi - x;
+    ux = -ux; uy = -uy; uz = -uz;
+  }
+}
+
 
 // quadratic spline with nodes at cell centres (particle.f90:99-107)
 __host__ __device__ inline void shape3(double dh, double& s0, double& s1, double& s2) {
@@ -263,7 +293,7 @@ hipError_t launch_energy(wpic_ctx* c, double* dout);
 hipError_t launch_gauss(wpic_ctx* c, double* G, double* rho);
 hipError_t launch_codes_identity(wpic_ctx* c);
 // particles: boundaries, exchange, sort
-hipError_t launch_wrapx(wpic_ctx* c);
+hipError_t launch_wrapx(wpic_ctx* c, bool injection);
 hipError_t launch_rowscan(wpic_ctx* c, bool ywrap);
 hipError_t launch_rowcount(wpic_ctx* c);
 hipError_t launch_rowbuild(wpic_ctx* c);

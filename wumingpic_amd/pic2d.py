@@ -130,6 +130,14 @@ class Pic2D:
         """particle__solv(gp, up, uf, cumcnt, nxs, nxe): 2d/common/particle.f90:48"""
         check(self._L.wpic_particle_solv2d(self._h))
 
+    def bc_injection(self, u0):
+        """boundary_shock__injection(gp, np2, nxs, nxe, u0): 2d/proj/shock/boundary_shock.f90:255"""
+        check(self._L.wpic_bc_injection2d(self._h, float(u0)))
+
+    def set_injection(self, u0):
+        """the u0 of the step's injection edge (WPIC_BC_SHOCK)"""
+        check(self._L.wpic_set_injection2d(self._h, float(u0)))
+
     def particle_solv_vay(self):
         """particle__solv_vay(gp, up, uf, cumcnt, nxs, nxe): 2d/common/particle.f90:182"""
         check(self._L.wpic_particle_solv_vay2d(self._h))
