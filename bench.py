@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 PUSH_DEPOSIT_BYTES = 96
 STEP_BYTES = 96
 # the kernel that implements the fused push + deposit of the FP64 fast mode
-PUSH_DEPOSIT_KERNEL = "k_push_deposit_fast<16, true, 0>"
+PUSH_DEPOSIT_KERNEL = "k_push_deposit_fast<16, true, 0, false>"
 # mixed form: read and write the 32-byte record (cell, 2 offsets, 3 momenta, id)
 PUSH_DEPOSIT_BYTES_MIXED = 64
 
