@@ -81,6 +81,10 @@ int wpic_step3d(wpic_ctx3d *ctx, int nsteps);
  * (3d/proj/weibel/app.f90:510-560); Gauss residual G(nx, ny, nz) column-major
  * (no reference counterpart); particle counts per species */
 int wpic_energy3d(wpic_ctx3d *ctx, double out[3]);
+/* energy_history's columns of the 3D app (3d/proj/weibel/app.f90:509-577):
+ * out[0..nsp-1] the kinetic energy of each species, out[nsp] E^2/8pi,
+ * out[nsp+1] B^2/8pi */
+int wpic_energy_species3d(wpic_ctx3d *ctx, double *out);
 int wpic_gauss3d(wpic_ctx3d *ctx, double *G);
 int wpic_counts3d(wpic_ctx3d *ctx, long long *counts);
 
@@ -103,6 +107,14 @@ int wpic_ic_beams3d(wpic_ctx3d *ctx, unsigned long long seed, int n_ppc, double 
 
 /* HIP-event timing on the context's stream: names "push", "deposit", "field",
  * "cg", "exchange", "sort" */
+/* The reference 3D Weibel app's state (3d/proj/weibel/app.f90 set_initial_condition,
+ * set_particle_ids): Bz = b0, n0 particles per cell per species on a regular x
+ * lattice along each column, y and z uniform in the cell and shared by the
+ * species; Maxwellian momenta with spread v_thi (ions) / v_the (electrons),
+ * t_ani times that along z; ids -1, -2, ... column by column per species; the
+ * OS-seeded draws replaced by counter-based draws from `seed`.  FP64 modes. */
+int wpic_ic_weibel3d(wpic_ctx3d *ctx, unsigned long long seed, int n0, double v_thi, double v_the, double t_ani,
+                     double b0);
 int wpic_set_timing3d(wpic_ctx3d *ctx, int on);
 int wpic_timing3d(wpic_ctx3d *ctx, const char *name, double *ms, long long *count);
 

@@ -18,6 +18,11 @@ extern "C" {
 /* Runs the app described by the config file (NULL: "config.json", as the
  * reference's config_default) in WPIC_MODE_FAST or WPIC_MODE_PARITY. */
 int wpic_app2d_main(const char *config, int mode);
+/* The 3D Weibel app (3d/proj/weibel/app.f90 app__main :87-160, load_config,
+ * init :266-380, energy_history :509-577 with one kinetic column per species)
+ * the same way; "udrift" in the parameter section selects the
+ * counter-streaming beams of the 3D Weibel workload. */
+int wpic_app3d_main(const char *config, int mode);
 
 #ifdef __cplusplus
 }

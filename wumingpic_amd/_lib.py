@@ -41,7 +41,8 @@ EXPORTS3D = (
     "wpic_download_dfield3d", "wpic_upload_dfield3d",
     "wpic_particle_solv3d", "wpic_field_fdtd_i3d", "wpic_bc_particle_x3d", "wpic_bc_particle_yz3d",
     "wpic_sort_bucket3d", "wpic_cg_iters3d", "wpic_step3d",
-    "wpic_energy3d", "wpic_gauss3d", "wpic_counts3d", "wpic_ic_beams3d",
+    "wpic_energy3d", "wpic_energy_species3d", "wpic_gauss3d", "wpic_counts3d", "wpic_ic_beams3d",
+    "wpic_ic_weibel3d",
     "wpic_set_timing3d", "wpic_timing3d",
     "wpic_mom_accl3d", "wpic_mom_nvt3d", "wpic_bc_mom3d", "wpic_moments3d",
     "wpic_download_mom3d", "wpic_upload_mom3d",
@@ -56,7 +57,7 @@ EXPORTS_IO = (
 
 
 # every symbol include/wpic_app.h declares
-EXPORTS_APP = ("wpic_app2d_main",)
+EXPORTS_APP = ("wpic_app2d_main", "wpic_app3d_main")
 
 
 # every symbol include/wpic_adj.h declares
@@ -177,6 +178,8 @@ def lib():
         "wpic_gauss3d": [vp, d],
         "wpic_counts3d": [vp, np.ctypeslib.ndpointer(np.int64)],
         "wpic_ic_beams3d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
+        "wpic_ic_weibel3d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
+        "wpic_energy_species3d": [vp, np.ctypeslib.ndpointer(np.float64)],
         "wpic_set_timing3d": [vp, C.c_int],
         "wpic_timing3d": [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
         "wpic_mom_accl3d": [vp],

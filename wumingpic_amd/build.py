@@ -113,17 +113,17 @@ def build(verbose=False):
     return LIB
 
 
-APP_SRC = os.path.join(HERE, "app", "wpic_app2d.cpp")
-APP_EXE = os.path.join(HERE, "app", "wpic_app2d")
+APPS = ("wpic_app2d", "wpic_app3d")
 
 
 def build_app():
-    """the command-line problem driver, linked to libwpic.so"""
-    cmd = [CXX, *CXXFLAGS, "-o", APP_EXE, APP_SRC, "-L" + HERE, "-lwpic", "-Wl,-rpath,$ORIGIN/.."]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"build of the app driver failed:\n{r.stdout}\n{r.stderr}")
-    return APP_EXE
+    """the command-line problem drivers, linked to libwpic.so"""
+    for name in APPS:
+        exe = os.path.join(HERE, "app", name)
+        cmd = [CXX, *CXXFLAGS, "-o", exe, exe + ".cpp", "-L" + HERE, "-lwpic", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"build of {name} failed:\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -288,6 +288,57 @@ hipError_t launch_ic_beams3d(wpic_ctx3d* c, unsigned long long seed, int n_ppc, 
   return hipGetLastError();
 }
 
+// 3d/proj/weibel/app.f90 set_initial_condition + set_particle_ids: Bz = b0,
+// n0 particles per cell on a regular x lattice along each column, y and z
+// uniform in the column's cell, shared by the species; Maxwellian momenta with
+// spread sd, t_ani times that along z; ids -1, -2, ... column by column (j
+// fastest) per species.  FP64 form.
+struct IcW3Args {
+  Grid3D g;
+  Parts3 A;
+  unsigned long long seed;
+  int n0;
+  double sd[WPIC_MAXSP], t_ani;
+};
+
+__global__ void k_ic_weibel3(IcW3Args a) {
+  const Grid3D& g = a.g;
+  const long long colen = (long long)a.n0 * g.nx;
+  const long long per = colen * g.ncol;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= per * g.nsp) return;
+  const int s = (int)(t / per);
+  const long long q = t - s * per;
+  const int col = (int)(q / colen);
+  const int ii = (int)(q - col * colen) + 1;  // the reference's slot index
+  const long long d = s * g.ncap + q;
+  const int j = g.nys + col % g.ny, k = g.nzs + col / g.ny;
+  const int np2 = (int)colen;
+  a.A.x[d] = (g.nxgs + (g.nxge - g.nxgs + 1) * (ii - 5e-1) / np2) * g.delx;
+  a.A.y[d] = (j + u01_3(rkey3(a.seed, 4, col, 0, ii))) * g.delx;
+  a.A.z[d] = (k + u01_3(rkey3(a.seed, 5, col, 0, ii))) * g.delx;
+  double n0, n1, n2, n3;
+  normal2_3(rkey3(a.seed, 20 + 4 * s, col, 0, ii), rkey3(a.seed, 21 + 4 * s, col, 0, ii), n0, n1);
+  normal2_3(rkey3(a.seed, 22 + 4 * s, col, 0, ii), rkey3(a.seed, 23 + 4 * s, col, 0, ii), n2, n3);
+  a.A.ux[d] = a.sd[s] * n0;
+  a.A.uy[d] = a.sd[s] * n1;
+  a.A.uz[d] = a.t_ani * a.sd[s] * n2;
+  a.A.id[d] = (uint64_t)(-(q + 1));
+}
+
+hipError_t launch_ic_weibel3d(wpic_ctx3d* c, unsigned long long seed, int n0, const double* sd, double t_ani) {
+  IcW3Args a;
+  a.g = c->g;
+  a.A = c->A;
+  a.seed = seed;
+  a.n0 = n0;
+  for (int s = 0; s < WPIC_MAXSP; s++) a.sd[s] = s < c->g.nsp ? sd[s] : 0.0;
+  a.t_ani = t_ani;
+  const long long n = (long long)n0 * c->g.nx * c->g.ncol * c->g.nsp;
+  hipLaunchKernelGGL(k_ic_weibel3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // diagnostics
 // ---------------------------------------------------------------------------
@@ -306,12 +357,13 @@ __device__ inline double bsum3(double v, double* sh) {
 
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, PartsM3 M, const int* colbase, const int* np2,
-                                                 const int* ap, const double* uf, double* out) {
+                                                 const int* ap, const double* uf, double* out, int sp) {
   __shared__ double sh[4];
   double kin = 0.0, e2 = 0.0, b2 = 0.0;
   const int ncols = g.nsp * g.ncol;
   for (int rw = blockIdx.x; rw < ncols; rw += gridDim.x) {
     const int s = rw / g.ncol;
+    if (sp >= 0 && s != sp) continue;  // one species' kinetic energy
     const int base = colbase[rw], n = np2[rw];
     for (int q = threadIdx.x; q < n; q += 256) {
       const int p = ap ? ap[base + q] : base + q;
@@ -341,13 +393,13 @@ __global__ __launch_bounds__(256) void k_energy3(Grid3D g, Parts3 A, PartsM3 M, 
   }
 }
 
-hipError_t launch_energy3d(wpic_ctx3d* c, double* dout) {
+hipError_t launch_energy3d(wpic_ctx3d* c, double* dout, int sp) {
   if (c->mixed)
     hipLaunchKernelGGL(k_energy3<true>, dim3(RB3), dim3(256), 0, c->st, c->g, c->A, c->MA, c->colbaseA, c->np2A,
-                       c->aperm, c->uf, dout);
+                       c->aperm, c->uf, dout, sp);
   else
     hipLaunchKernelGGL(k_energy3<false>, dim3(RB3), dim3(256), 0, c->st, c->g, c->A, c->MA, c->colbaseA, c->np2A,
-                       c->aperm, c->uf, dout);
+                       c->aperm, c->uf, dout, sp);
   return hipGetLastError();
 }
 

@@ -759,6 +759,65 @@ int wpic_energy3d(wpic_ctx3d* c, double out[3]) {
   return 0;
 }
 
+int wpic_energy_species3d(wpic_ctx3d* c, double* out) {
+  if (!c || !out) return set_error(WPIC_EINVAL, "null argument");
+  const double pi = 3.141592653589793;
+  std::vector<double> h(256 * 3);
+  for (int s = 0; s < c->g.nsp; s++) {
+    HIPCHK(launch_energy3d(c, c->red, s));
+    HIPCHK(hipMemcpyAsync(h.data(), c->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double k = 0, e = 0, b = 0;
+    for (int i = 0; i < 256; i++) {
+      k += h[i * 3];
+      e += h[i * 3 + 1];
+      b += h[i * 3 + 2];
+    }
+    out[s] = k;
+    out[c->g.nsp] = e / (8 * pi);
+    out[c->g.nsp + 1] = b / (8 * pi);
+  }
+  return 0;
+}
+
+int wpic_ic_weibel3d(wpic_ctx3d* c, unsigned long long seed, int n0, double v_thi, double v_the, double t_ani,
+                     double b0) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return set_error(WPIC_EINVAL, "the Weibel lattice state is built in the FP64 modes");
+  const Grid3D& g = c->g;
+  if (g.nsp != 2) return set_error(WPIC_EINVAL, "the Weibel state has two species (ions, electrons)");
+  if (n0 <= 0 || (long long)n0 * g.nx * g.ncol > g.ncap)
+    return set_error(WPIC_EMEM, "Error: Too large number of particles");
+  const int nr = g.nsp * g.ncol;
+  std::vector<int> hcols(2 * (size_t)nr);
+  std::vector<int> cc((size_t)nr * (g.nx + 1));
+  for (int s = 0; s < g.nsp; s++)
+    for (int r = 0; r < g.ncol; r++) {
+      const int rw = s * g.ncol + r;
+      hcols[rw] = (int)(s * g.ncap + (long long)r * n0 * g.nx);
+      hcols[nr + rw] = n0 * g.nx;
+      // the x lattice puts n0 particles in each cell, in order
+      for (int i = 0; i <= g.nx; i++) cc[(size_t)rw * (g.nx + 1) + i] = i * n0;
+    }
+  HIPCHK(hipMemcpyAsync(c->cols[c->ca], hcols.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->cumcnt, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice, c->st));
+  c->aperm = nullptr;
+  const double sd[WPIC_MAXSP] = {v_thi, v_the, 0.0, 0.0};
+  HIPCHK(launch_ic_weibel3d(c, seed, n0, sd, t_ani));
+  const size_t pl = plane3(g);
+  HIPCHK(launch_fill3d(c, c->uf, 6 * pl, 0.0));
+  HIPCHK(launch_fill3d(c, c->uf + 2 * pl, pl, b0));
+  HIPCHK(launch_fill3d(c, c->df, 6 * pl, 0.0));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->h_cols[c->ca] = hcols;
+  c->h_cols_ok[c->ca] = true;
+  c->b_valid = false;
+  c->b_accl = false;
+  c->b_aligned = false;
+  c->exch_pending = false;
+  return 0;
+}
+
 int wpic_gauss3d(wpic_ctx3d* c, double* G) {
   if (!c || !G) return set_error(WPIC_EINVAL, "null argument");
   const size_t nc = (size_t)c->g.nx * c->g.ny * c->g.nz;

@@ -1,6 +1,7 @@
-// wpic_app.cpp -- the reference's 2D problem drivers on one MI355X: the
-// Weibel app (2d/proj/weibel/app.f90) and the reconnection app
-// (2d/proj/reconnection/app.f90), with their JSON config, derived
+// wpic_app.cpp -- the reference's problem drivers on one MI355X: the 2D
+// Weibel app (2d/proj/weibel/app.f90), the 2D reconnection app
+// (2d/proj/reconnection/app.f90) and the 3D Weibel app
+// (3d/proj/weibel/app.f90), with their JSON config, derived
 // parameters, initial state or restart, output cadence (particles, tracers,
 // moments + energy history, restart at the elapsed-time limit and at the
 // end), driven through the C-ABIs of wpic.h and wpic_io.h.
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "../../include/wpic.h"
+#include "../../include/wpic3d.h"
 #include "../../include/wpic_app.h"
 #include "../../include/wpic_io.h"
 #include "wpic_json.h"
@@ -103,6 +105,42 @@ int fail(const char* what) {
   return -1;
 }
 
+// '(i7.7, "_restart")'
+std::string restart_name(int it) {
+  char name[32];
+  if (it < 0 || it > 9999999)
+    std::snprintf(name, sizeof name, "*******_restart");
+  else
+    std::snprintf(name, sizeof name, "%07d_restart", it);
+  return name;
+}
+
+// save_restart's config update (app.f90:550-580): "restart_file" set in the
+// "config" section, printed back where the config was read
+void rewrite_config(Config& cfg, const std::string& name) {
+  JVal* c = nullptr;
+  for (auto& kv : cfg.root.obj)
+    if (kv.first == "config") c = &kv.second;
+  if (c) {
+    bool found = false;
+    for (auto& kv : c->obj)
+      if (kv.first == "restart_file") {
+        kv.second = JVal::string(name);
+        found = true;
+      }
+    if (!found) c->add("restart_file", JVal::string(name));
+    std::string txt;
+    print(cfg.root, 0, txt);
+    txt += "\n";
+    FILE* f = std::fopen(cfg.path.c_str(), "wb");
+    if (f) {
+      std::fwrite(txt.data(), 1, txt.size(), f);
+      std::fclose(f);
+    }
+  }
+  cfg.restart_file = name;
+}
+
 struct App {
   Config cfg;
   wpic_params2d P{};
@@ -153,34 +191,10 @@ struct App {
   // save_restart (app.f90:550-580): the restart pair, and the config with
   // "restart_file" set, printed back where it was read
   int save_restart(int it) {
-    char name[32];
-    if (it < 0 || it > 9999999)
-      std::snprintf(name, sizeof name, "*******_restart");
-    else
-      std::snprintf(name, sizeof name, "%07d_restart", it);
+    const std::string name = restart_name(it);
     if (pull_particles()) return -1;
-    if (chk(wpic_paraio_output(io, up.data(), uf.data(), np2.data(), P.nxgs, P.nxge, it, name))) return -1;
-    JVal* c = nullptr;
-    for (auto& kv : cfg.root.obj)
-      if (kv.first == "config") c = &kv.second;
-    if (c) {
-      bool found = false;
-      for (auto& kv : c->obj)
-        if (kv.first == "restart_file") {
-          kv.second = JVal::string(name);
-          found = true;
-        }
-      if (!found) c->add("restart_file", JVal::string(name));
-      std::string txt;
-      print(cfg.root, 0, txt);
-      txt += "\n";
-      FILE* f = std::fopen(cfg.path.c_str(), "wb");
-      if (f) {
-        std::fwrite(txt.data(), 1, txt.size(), f);
-        std::fclose(f);
-      }
-    }
-    cfg.restart_file = name;
+    if (chk(wpic_paraio_output(io, up.data(), uf.data(), np2.data(), P.nxgs, P.nxge, it, name.c_str()))) return -1;
+    rewrite_config(cfg, name);
     return 0;
   }
 
@@ -341,34 +355,207 @@ struct App {
   }
 };
 
-}  // namespace
+// The 3D Weibel app (3d/proj/weibel/app.f90:87-160 loop, :266-380 init):
+// cfl 1, np = 3 n_ppc nx, energy.dat with one kinetic column per species
+// ('(f10.2, 5(1x, e12.5))', :509-577), the final state saved as restart
+// max_it + 1.  "udrift" selects the counter-streaming beams of the 3D Weibel
+// workload instead of the lattice state.
+struct App3 {
+  Config cfg;
+  wpic_params3d P{};
+  wpic_ctx3d* ctx = nullptr;
+  wpic_paraio* io = nullptr;
+  int np = 0, nx = 0, ny = 0, nz = 0, n0 = 0;
+  double wpe = 0, wpi = 0, wge = 0, wgi = 0, vte = 0, vti = 0;
+  std::vector<double> up, uf, mom;
+  std::vector<int> np2, cumcnt;
 
-extern "C" int wpic_app2d_main(const char* config, int mode) {
+  ~App3() {
+    if (io) wpic_paraio_finalize(io);
+    if (ctx) wpic_destroy3d(ctx);
+  }
+  int chk(int rc) {
+    if (rc) std::fprintf(stderr, "%s\n", wpic_last_error());
+    return rc;
+  }
+  int energy_history(int it) {
+    double e[4];
+    if (chk(wpic_energy_species3d(ctx, e))) return -1;
+    const std::string fn = cfg.datadir + "energy.dat";
+    FILE* f = std::fopen(fn.c_str(), it == 0 ? "w" : "a");
+    if (!f) return fail(("cannot open " + fn).c_str());
+    const double total = ((e[0] + e[1]) + e[2]) + e[3];
+    std::fprintf(f, "%s %s %s %s %s %s\n", fmt_f(it * P.delt, 10, 2).c_str(), fmt_e(e[0], 12, 5).c_str(),
+                 fmt_e(e[1], 12, 5).c_str(), fmt_e(e[2], 12, 5).c_str(), fmt_e(e[3], 12, 5).c_str(),
+                 fmt_e(total, 12, 5).c_str());
+    std::fclose(f);
+    return 0;
+  }
+  int pull_particles() {
+    if (up.empty()) {
+      up.assign((size_t)P.ndim * np * ny * nz * P.nsp, 0.0);
+      np2.assign((size_t)ny * nz * P.nsp, 0);
+      cumcnt.assign((size_t)(nx + 1) * ny * nz * P.nsp, 0);
+    }
+    if (chk(wpic_download_up3d(ctx, up.data(), np2.data(), cumcnt.data(), np))) return -1;
+    return chk(wpic_download_fields3d(ctx, uf.data()));
+  }
+  int save_restart(int it) {
+    const std::string name = restart_name(it);
+    if (pull_particles()) return -1;
+    if (chk(wpic_paraio_output(io, up.data(), uf.data(), np2.data(), P.nxgs, P.nxge, it, name.c_str()))) return -1;
+    rewrite_config(cfg, name);
+    return 0;
+  }
+  int run(int mode) {
+    const JVal* par = cfg.root.get("parameter");
+    int num_process = 1;
+    get_int(par, "num_process", num_process);
+    if (!get_int(par, "n_x", nx) || !get_int(par, "n_y", ny) || !get_int(par, "n_z", nz))
+      return fail("Error: n_x, n_y and n_z are required");
+    double mass_ratio;
+    if (!get_num(par, "mass_ratio", mass_ratio)) return fail("Error: mass_ratio is required");
+    double seedd = 20261202;
+    get_num(par, "seed", seedd);
+    const unsigned long long seed = (unsigned long long)seedd;
+    const bool weibel = par && par->get("t_ani"), beams = par && par->get("udrift");
+    if ((int)weibel + (int)beams != 1)
+      return fail("Error: the parameter section must describe one problem (t_ani or udrift)");
+    double sigma_e = 0, omega_pe = 0, t_ani = 0, udrift = 0;
+    get_int(par, "n_ppc", n0);
+    get_num(par, "sigma_e", sigma_e);
+    get_num(par, "omega_pe", omega_pe);
+    get_num(par, "v_the", vte);
+    get_num(par, "v_thi", vti);
+    get_num(par, "t_ani", t_ani);
+    get_num(par, "udrift", udrift);
+    P.ndim = 7;
+    P.nsp = 2;
+    P.nxgs = 2;
+    P.nxge = P.nxgs + nx - 1;
+    P.nygs = 2;
+    P.nyge = P.nygs + ny - 1;
+    P.nzgs = 2;
+    P.nzge = P.nzgs + nz - 1;
+    P.delx = 1.0;
+    P.c = 1.0;
+    P.gfac = 0.501;
+    P.mode = mode;
+    P.delt = 1.0 * P.delx / P.c;  // cfl = 1
+    get_num(par, "delt", P.delt);  // extension: the benchmark's dt = 0.5
+    np = n0 * nx * 3;
+    const double pi = PI;
+    wpe = omega_pe;
+    wge = omega_pe * std::sqrt(sigma_e);
+    wpi = wpe / std::sqrt(mass_ratio);
+    wgi = wge / mass_ratio;
+    P.r[0] = mass_ratio;
+    P.r[1] = 1.0;
+    P.q[0] = +std::sqrt(P.r[0] / (4.0 * pi * n0)) * wpi;
+    P.q[1] = -std::sqrt(P.r[1] / (4.0 * pi * n0)) * wpe;
+    const double b0 = P.r[0] * P.c / P.q[0] * wgi;
+    if ((long long)n0 * nx > np || n0 <= 0) return fail("Error: Too large number of particles");
+    P.capacity = (long long)n0 * nx * ny * nz;
+    if (num_process != 1)
+      std::fprintf(stderr, "note: num_process = %d in the config; this driver runs one rank on one GPU\n",
+                   num_process);
+    uf.assign((size_t)6 * (nx + 4) * (ny + 4) * (nz + 4), 0.0);
+    int it0 = 0;
+    if (chk(wpic_paraio_init3d(&io, P.ndim, np, P.nsp, P.nxgs, P.nxge, P.nygs, P.nyge, P.nzgs, P.nzge, P.nygs,
+                               P.nyge, P.nzgs, P.nzge, 1, 0, P.delx, P.delt, P.c, P.q, P.r, cfg.datadir.c_str())))
+      return -1;
+    if (!cfg.restart_file.empty()) {
+      up.assign((size_t)P.ndim * np * ny * nz * P.nsp, 0.0);
+      np2.assign((size_t)ny * nz * P.nsp, 0);
+      cumcnt.assign((size_t)(nx + 1) * ny * nz * P.nsp, 0);
+      int nxs, nxe;
+      if (chk(wpic_paraio_input(io, up.data(), uf.data(), np2.data(), &nxs, &nxe, &it0, cfg.restart_file.c_str())))
+        return -1;
+      if (chk(wpic_create3d(&ctx, &P))) return -1;
+      if (chk(wpic_upload_gp3d(ctx, up.data(), np2.data(), np)) || chk(wpic_upload_fields3d(ctx, uf.data())))
+        return -1;
+      if (chk(wpic_sort_bucket3d(ctx))) return -1;
+    } else {
+      if (chk(wpic_create3d(&ctx, &P))) return -1;
+      if (chk(wpic_paraio_param(io, n0, wpe, wpi, wge, wgi, vti, vte, "init_param"))) return -1;
+      const int rc = weibel ? wpic_ic_weibel3d(ctx, seed, n0, vti, vte, t_ani, b0)
+                            : wpic_ic_beams3d(ctx, seed, n0, udrift, vte, vti, b0);
+      if (chk(rc)) return -1;
+      if (energy_history(0)) return -1;
+    }
+    mom.assign((size_t)7 * (nx + 2) * (ny + 2) * (nz + 2) * P.nsp, 0.0);
+    const double t0 = now_s();
+    for (int it = it0 + 1; it <= cfg.max_it; it++) {
+      if (chk(wpic_step3d(ctx, 1))) return -1;
+      if (cfg.intvl_ptcl > 0 && it % cfg.intvl_ptcl == 0) {
+        if (pull_particles() || chk(wpic_paraio_ptcl(io, up.data(), uf.data(), np2.data(), it))) return -1;
+      }
+      if (cfg.intvl_orb > 0 && it % cfg.intvl_orb == 0) {
+        if (pull_particles() || chk(wpic_paraio_orb(io, up.data(), uf.data(), np2.data(), it))) return -1;
+      }
+      if (cfg.intvl_mom > 0 && it % cfg.intvl_mom == 0) {
+        if (chk(wpic_moments3d(ctx)) || chk(wpic_download_mom3d(ctx, mom.data())) ||
+            chk(wpic_download_fields3d(ctx, uf.data())) || chk(wpic_paraio_mom(io, mom.data(), uf.data(), it)))
+          return -1;
+        if (energy_history(it)) return -1;
+      }
+      const double etime = now_s() - t0;
+      if (etime >= cfg.max_elapsed) {
+        if (save_restart(it)) return -1;
+        std::fprintf(stderr, "*** Elapsed time limit exceeded \n");
+        std::fprintf(stderr, "*** A snapshot %s has been saved\n", cfg.restart_file.c_str());
+        return 0;
+      }
+      if (cfg.verbose >= 1)
+        std::printf("*** Time step: %7d completed in %s sec.\n", it, fmt_e(etime, 10, 2).c_str());
+    }
+    return save_restart(cfg.max_it + 1);
+  }
+};
+
+// load_config (weibel/app.f90:160-260): the "config" section
+int load_config(const char* config, Config& cfg) {
   const std::string path = config && *config ? config : "config.json";
   std::string txt;
   if (!read_file(path, txt)) {
     std::fprintf(stderr, "Error: %s does not exists\n", path.c_str());
     return set_error(WPIC_EINVAL, "Error: " + path + " does not exists");
   }
-  App app;
   Parser ps{txt.data(), txt.data() + txt.size()};
-  app.cfg.root = ps.val();
-  if (!ps.ok || app.cfg.root.k != JVal::OBJ) {
+  cfg.root = ps.val();
+  if (!ps.ok || cfg.root.k != JVal::OBJ) {
     std::fprintf(stderr, "Error: failed to load %s\n", path.c_str());
     return set_error(WPIC_EINVAL, "Error: failed to load " + path);
   }
-  app.cfg.path = path;
-  const JVal* c = app.cfg.root.get("config");
-  get_int(c, "verbose", app.cfg.verbose);
+  cfg.path = path;
+  const JVal* c = cfg.root.get("config");
+  get_int(c, "verbose", cfg.verbose);
   const JVal* dd = c ? c->get("datadir") : nullptr;
-  app.cfg.datadir = (dd && dd->k == JVal::STR ? dd->s : std::string(".")) + "/";
-  get_num(c, "max_elapsed", app.cfg.max_elapsed);
-  get_int(c, "max_it", app.cfg.max_it);
-  get_int(c, "intvl_ptcl", app.cfg.intvl_ptcl);
-  get_int(c, "intvl_mom", app.cfg.intvl_mom);
-  get_int(c, "intvl_orb", app.cfg.intvl_orb);
+  cfg.datadir = (dd && dd->k == JVal::STR ? dd->s : std::string(".")) + "/";
+  get_num(c, "max_elapsed", cfg.max_elapsed);
+  get_int(c, "max_it", cfg.max_it);
+  get_int(c, "intvl_ptcl", cfg.intvl_ptcl);
+  get_int(c, "intvl_mom", cfg.intvl_mom);
+  get_int(c, "intvl_orb", cfg.intvl_orb);
   const JVal* rf = c ? c->get("restart_file") : nullptr;
-  if (rf && rf->k == JVal::STR && !rf->s.empty()) app.cfg.restart_file = rf->s;
+  if (rf && rf->k == JVal::STR && !rf->s.empty()) cfg.restart_file = rf->s;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int wpic_app2d_main(const char* config, int mode) {
+  App app;
+  const int lc = load_config(config, app.cfg);
+  if (lc) return lc;
+  const int rc = app.run(mode);
+  return rc ? (rc < 0 ? WPIC_EINVAL : rc) : 0;
+}
+
+extern "C" int wpic_app3d_main(const char* config, int mode) {
+  App3 app;
+  const int lc = load_config(config, app.cfg);
+  if (lc) return lc;
   const int rc = app.run(mode);
   return rc ? (rc < 0 ? WPIC_EINVAL : rc) : 0;
 }

@@ -135,7 +135,8 @@ hipError_t launch_cg_iter3d(wpic_ctx3d* c, int it);
 hipError_t launch_cg_store3d(wpic_ctx3d* c);
 hipError_t launch_e_update3d(wpic_ctx3d* c);
 hipError_t launch_uf_add3d(wpic_ctx3d* c);
-hipError_t launch_energy3d(wpic_ctx3d* c, double* dout);
+hipError_t launch_energy3d(wpic_ctx3d* c, double* dout, int sp = -1);
+hipError_t launch_ic_weibel3d(wpic_ctx3d* c, unsigned long long seed, int n0, const double* sd, double t_ani);
 hipError_t launch_gauss3d(wpic_ctx3d* c, double* G, double* rho);
 // particles
 hipError_t launch_wrapx3d(wpic_ctx3d* c);

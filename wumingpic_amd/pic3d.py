@@ -187,6 +187,17 @@ class Pic3D:
         check(self._L.wpic_ic_beams3d(self._h, int(seed), int(n_ppc), float(udrift), float(uth_e),
                                       float(uth_i), float(b0)))
 
+    def energy_species(self):
+        """[kinetic of each species..., E^2/8pi, B^2/8pi] (3d/proj/weibel/app.f90:509-577)"""
+        out = np.zeros(self.nsp + 2)
+        check(self._L.wpic_energy_species3d(self._h, out))
+        return out
+
+    def ic_weibel(self, seed, n0, v_thi, v_the, t_ani, b0):
+        """the reference 3D Weibel app's state (3d/proj/weibel/app.f90 set_initial_condition)"""
+        check(self._L.wpic_ic_weibel3d(self._h, int(seed), int(n0), float(v_thi), float(v_the), float(t_ani),
+                                       float(b0)))
+
     def set_timing(self, on=True):
         check(self._L.wpic_set_timing3d(self._h, 1 if on else 0))
 
