@@ -126,6 +126,17 @@ int wpic_bc_injection2d(wpic_ctx *ctx, double u0);
 /* the injection 4-velocity u0 (< 0: inflow from the right, the shock app's
  * -|u_inject|, 2d/proj/shock/app.f90:319) of the step's injection edge */
 int wpic_set_injection2d(wpic_ctx *ctx, double u0);
+/* inject (2d/proj/shock/app.f90:697-856) on the sorted state, box fully
+ * expanded: nl[r] particles per species join row r at x = nxe dx + (m-1/2)/nl
+ * |v0| dt moved by (v0 + ux) dt, y uniform in the row (shared by the
+ * species), Maxwellian momenta (v_thi ions, v_the electrons) boosted by the
+ * inflow profile vprofile(x) = v0/2 (1 + tanh((x - l_damp - nxgs dx) /
+ * (l_damp/10))), ids continuing each species' count; they close cell nxe-1
+ * (cumcnt(nxe) and cumcnt(nxe+1) grow by nl[r]); the upstream B and E are set
+ * at nxe-1 and nxe.  The split of the injected number over rows (random in the
+ * reference) is the caller's; `it` keys the draws of this call. */
+int wpic_inject2d(wpic_ctx *ctx, unsigned long long seed, int it, const int *nl, double v0, double v_thi,
+                  double v_the, double l_damp, double b0, double theta_bn, double phi_bn);
 /* boundary_periodic__particle_y(gp, np2): boundary_periodic.f90:99 */
 int wpic_bc_particle_y2d(wpic_ctx *ctx);
 /* sort__bucket(up, gp, cumcnt, np2, nxs, nxe): 2d/common/sort.f90:36 */

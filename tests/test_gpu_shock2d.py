@@ -105,3 +105,63 @@ def test_shock_rejected_in_the_mixed_form():
     from wumingpic_amd.pic2d import Pic2D
     with pytest.raises(WpicError, match="FP64"):
         Pic2D(32, 16, [0.1, -0.1], [25.0, 1.0], bc=po.BC_SHOCK, mode=MODE_MIXED, capacity=1024)
+
+
+def test_inject_then_steps_match_oracle():
+    """inject (2d/proj/shock/app.f90:697-856) between parity steps: the sorted rows keep their
+    particles in order and gain nl[r] particles at their end with the reference's positions, ids and
+    row membership; the upstream field is set at nxe-1 and nxe; the steps that follow equal the
+    oracle's from the injected state bit for bit"""
+    from wumingpic_amd import MODE_PARITY
+    p, up, uf, cc, np2 = shock_state(48, 16, 12, 11, u0=-0.35)
+    u0 = p.u0
+    v0 = u0 / np.sqrt(1 + u0 * u0)
+    b0, theta, phi, l_damp = 0.5 / abs(p.q[0]), np.pi / 3, np.pi / 2, 10.0
+    nl = np.random.default_rng(12).integers(2, 7, size=p.ny).astype(np.int32)
+    print("shock inject", flush=True)
+    with _pic(p, MODE_PARITY) as sim:
+        sim.upload_up(up, np2, cc)
+        sim.upload_fields(uf)
+        sim.set_injection(u0)
+        sim.step(2)
+        u_a, n_a, c_a = sim.download_up(p.np)
+        sim.inject(5, 2, nl, v0, 0.02, 0.1, l_damp, b0, theta, phi)
+        u_b, n_b, c_b = sim.download_up(p.np)
+        f_b = sim.download_fields()
+        sim.step(4)
+        u_c, n_c, c_c = sim.download_up(p.np)
+        f_c = sim.download_fields()
+    # the injection
+    assert np.array_equal(n_b, n_a + nl[:, None])
+    for s in range(2):
+        tot = int(n_a[:, s].sum())
+        for j in range(p.ny):
+            n = n_a[j, s]
+            assert np.array_equal(_bits(u_b[:, :n, j, s]), _bits(u_a[:, :n, j, s]))
+            new = u_b[:, n:n + nl[j], j, s]
+            m = np.arange(1, nl[j] + 1)
+            ids = new[5].view(np.int64)
+            assert np.array_equal(ids, -(m + tot + nl[:j].sum()))
+            assert np.all(np.floor(new[1]) == p.nys + j)
+            x_lat = p.nxge + (m - 0.5) / nl[j] * abs(v0) * p.delt + v0 * p.delt
+            assert np.all(np.abs(new[0] - x_lat) < 6 * (0.02, 0.1)[s] * p.delt)
+        assert np.array_equal(u_b[1, n_a[0, 0]:n_b[0, 0], 0, 0], u_b[1, n_a[0, 1]:n_b[0, 1], 0, 1])
+    assert np.array_equal(c_b[:-2], c_a[:-2])
+    assert np.array_equal(c_b[-2:], c_a[-2:] + nl[None, :, None])
+    by, bz = b0 * np.sin(theta) * np.cos(phi), b0 * np.sin(theta) * np.sin(phi)
+    ie = p.nxge - (p.nxgs - 2)
+    assert np.all(f_b[1, ie - 1:ie + 1, :] == by) and np.all(f_b[2, ie - 1:ie + 1, :] == bz)
+    assert np.all(f_b[4, ie - 1, :] == v0 * bz) and np.all(f_b[5, ie - 1, :] == -v0 * by)
+    # the oracle: the same first two steps, then the injected state
+    o = po.Oracle2D(p)
+    gp = np.zeros_like(up, order="F")
+    o.step(up, gp, uf, cc, np2, 2)
+    ok, info = _active_equal(u_a, up, np2)
+    assert ok, info
+    up2, uf2 = u_b.copy(order="F"), f_b.copy(order="F")
+    cc2, np22 = c_b.copy(order="F"), n_b.copy(order="F")
+    o.step(up2, gp, uf2, cc2, np22, 4)
+    assert np.array_equal(n_c, np22) and np.array_equal(c_c, cc2)
+    ok, info = _active_equal(u_c, up2, np22)
+    assert ok, info
+    assert np.array_equal(_bits(f_c), _bits(uf2))

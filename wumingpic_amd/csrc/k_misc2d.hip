@@ -464,6 +464,115 @@ hipError_t launch_ic_harris(wpic_ctx* c, unsigned long long seed, int np2, int i
   return hipGetLastError();
 }
 
+// The shock app's inject (2d/proj/shock/app.f90:697-856) with the box fully
+// expanded (nxe = nxge), fused with the re-layout it needs: one block per
+// destination row copies the row's sorted particles (read through the
+// permutation) into B at the packed new base, then draws the row's nl
+// injected particles after them: positions spread over |v0| dt beyond nxe,
+// moved by (v0 + ux) dt, y uniform in the row and shared by the species,
+// Maxwellian momenta boosted by the inflow profile v1(x), ids continuing the
+// species' count.  cumcnt(nxe) and cumcnt(nxe+1) grow by nl: the injected
+// particles close cell nxe-1's range, where the reference counts them.
+struct InjArgs {
+  Grid2D g;
+  Parts A, B;
+  const int* ap;
+  const int *base_old, *np2_old, *base_new;
+  const int* nl;             // [ny]
+  const long long* idoff;    // [nsp*ny]
+  int* cumcnt;
+  unsigned long long seed;
+  int it;
+  double v0, x0, sd[2], xc, xs;
+};
+
+__global__ __launch_bounds__(256) void k_inject(InjArgs a) {
+  const Grid2D& g = a.g;
+  const int rw = blockIdx.x;
+  const int s = rw / g.ny, r = rw - s * g.ny;
+  const int n_old = a.np2_old[rw], nl = a.nl[r];
+  const int bo = a.base_old[rw], bn = a.base_new[rw];
+  for (int k = threadIdx.x; k < n_old; k += 256) {
+    const int ia = a.ap ? a.ap[bo + k] : bo + k;
+    const int d = bn + k;
+    a.B.x[d] = a.A.x[ia];
+    a.B.y[d] = a.A.y[ia];
+    a.B.ux[d] = a.A.ux[ia];
+    a.B.uy[d] = a.A.uy[ia];
+    a.B.uz[d] = a.A.uz[ia];
+    a.B.id[d] = a.A.id[ia];
+  }
+  const double delx = g.delx, delt = g.delt, c = g.c;
+  const int j = g.nys + r;
+  for (int m = 1 + threadIdx.x; m <= nl; m += 256) {
+    const int d = bn + n_old + m - 1;
+    double x = g.nxge * delx + (m - 0.5) / nl * a.x0;
+    const double y = (j + u01(rkey(a.seed, 40, r, (uint64_t)a.it, m))) * delx;
+    double n0, n1, n2, n3;
+    normal2(rkey(a.seed, 41 + 2 * s, r, (uint64_t)a.it, m), rkey(a.seed, 42 + 2 * s, r, (uint64_t)a.it, m), n0, n1);
+    normal2(rkey(a.seed, 45 + 2 * s, r, (uint64_t)a.it, m), rkey(a.seed, 46 + 2 * s, r, (uint64_t)a.it, m), n2, n3);
+    double ux = a.sd[s] * n0;
+    const double uy = a.sd[s] * n1, uz = a.sd[s] * n2;
+    x = x + (a.v0 + ux) * delt;
+    const double v1 = 0.5 * a.v0 * (1 + tanh((x - a.xc) / a.xs));
+    const double gam1 = 1 / sqrt(1 - (v1 / c) * (v1 / c));
+    const double gamp = sqrt(1 + (ux * ux + uy * uy + uz * uz) / (c * c));
+    ux = gam1 * (ux + v1 * gamp);
+    a.B.x[d] = x;
+    a.B.y[d] = y;
+    a.B.ux[d] = ux;
+    a.B.uy[d] = uy;
+    a.B.uz[d] = uz;
+    a.B.id[d] = (uint64_t)(-((long long)m + a.idoff[rw]));
+  }
+  if (threadIdx.x == 0 && nl > 0) {
+    int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
+    cr[g.nx - 1] += nl;
+    cr[g.nx] += nl;
+  }
+}
+
+// the upstream field at i = nxe-1 and nxe over every row (app.f90:840-850)
+__global__ void k_inject_fields(Grid2D g, double* uf, double by, double bz, double v0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.fh) return;
+  const int J = g.nys - 2 + t;
+  const size_t pl = (size_t)g.fw * g.fh;
+  const int f1 = fidx(g, g.nxge - 1, J), f2 = fidx(g, g.nxge, J);
+  uf[pl + f1] = by;
+  uf[2 * pl + f1] = bz;
+  uf[4 * pl + f1] = +v0 * uf[2 * pl + f1] / g.c;
+  uf[5 * pl + f1] = -v0 * uf[pl + f1] / g.c;
+  uf[pl + f2] = by;
+  uf[2 * pl + f2] = bz;
+}
+
+hipError_t launch_inject(wpic_ctx* c, unsigned long long seed, int it, const int* base_new, const int* nl,
+                         const long long* idoff, double v0, const double* sd, double l_damp, double by, double bz) {
+  InjArgs a;
+  a.g = c->g;
+  a.A = c->A;
+  a.B = c->B;
+  a.ap = c->aperm;
+  a.base_old = c->rowbaseA;
+  a.np2_old = c->np2A;
+  a.base_new = base_new;
+  a.nl = nl;
+  a.idoff = idoff;
+  a.cumcnt = c->cumcnt;
+  a.seed = seed;
+  a.it = it;
+  a.v0 = v0;
+  a.x0 = fabs(v0) * c->g.delt;
+  a.sd[0] = sd[0];
+  a.sd[1] = sd[1];
+  a.xc = l_damp + c->g.nxgs * c->g.delx;
+  a.xs = l_damp * 0.1;
+  hipLaunchKernelGGL(k_inject, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_inject_fields, dim3((c->g.fh + 255) / 256), dim3(256), 0, c->st, c->g, c->uf, by, bz, v0);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // diagnostics
 // ---------------------------------------------------------------------------

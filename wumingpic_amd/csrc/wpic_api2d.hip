@@ -722,6 +722,63 @@ int wpic_set_injection2d(wpic_ctx* c, double u0) {
   return 0;
 }
 
+int wpic_inject2d(wpic_ctx* c, unsigned long long seed, int it, const int* nl, double v0, double v_thi, double v_the,
+                  double l_damp, double b0, double theta_bn, double phi_bn) {
+  if (!c || !nl) return set_error(WPIC_EINVAL, "null argument");
+  if (c->mixed) return mixed_only_step();
+  if (c->g.bc != WPIC_BC_SHOCK) return set_error(WPIC_EINVAL, "inject needs WPIC_BC_SHOCK");
+  const Grid2D& g = c->g;
+  if (g.nsp != 2) return set_error(WPIC_EINVAL, "inject loads two species (ions, electrons)");
+  if (c->exch_pending) return set_error(WPIC_EINVAL, "inject needs the sorted state (after sort__bucket)");
+  if (!(std::fabs(v0) < g.c) || !(l_damp > 0.0)) return set_error(WPIC_EINVAL, "need |v0| < c and l_damp_ini > 0");
+  const int nr = g.nsp * g.ny;
+  if (fetch_rows(c, c->ra)) return WPIC_EHIP;
+  const std::vector<int> old = c->h_rows[c->ra];
+  std::vector<int> hrows(2 * nr);
+  std::vector<long long> idoff(nr);
+  for (int s = 0; s < g.nsp; s++) {
+    long long total = 0, base = 0, ninj = 0;
+    for (int r = 0; r < g.ny; r++) total += old[nr + s * g.ny + r];
+    for (int r = 0; r < g.ny; r++) {
+      if (nl[r] < 0) return set_error(WPIC_EINVAL, "negative injection count");
+      const int rw = s * g.ny + r;
+      const long long n = (long long)old[nr + rw] + nl[r];
+      hrows[rw] = (int)(s * g.ncap + base);
+      hrows[nr + rw] = (int)n;
+      // pid = ii - np2 + ncinj_grid(j) + nptotal(isp) (app.f90:833)
+      idoff[rw] = ninj + total;
+      base += n;
+      ninj += nl[r];
+      if (base > g.ncap) return set_error(WPIC_EMEM, "injection beyond the particle capacity");
+    }
+  }
+  // scratch: idoff (8-byte aligned at the start of rowcnt), then nl
+  long long* d_idoff = reinterpret_cast<long long*>(c->rowcnt);
+  int* d_nl = c->rowcnt + 2 * nr;
+  HIPCHK(hipMemcpyAsync(d_idoff, idoff.data(), sizeof(long long) * nr, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(d_nl, nl, sizeof(int) * g.ny, hipMemcpyHostToDevice, c->st));
+  if (c->rb == c->ra) c->rb = pick_free(c->ra, c->rn);
+  HIPCHK(hipMemcpyAsync(c->rows[c->rb], hrows.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
+  const double sd[2] = {v_thi, v_the};
+  const double by = b0 * sin(theta_bn) * cos(phi_bn), bz = b0 * sin(theta_bn) * sin(phi_bn);
+  HIPCHK(launch_inject(c, seed, it, c->rows[c->rb], d_nl, d_idoff, v0, sd, l_damp, by, bz));
+  HIPCHK(hipStreamSynchronize(c->st));
+  // B holds the new sorted state in identity order
+  std::swap(c->A, c->B);
+  std::swap(c->MA, c->MB);
+  std::swap(c->ra, c->rb);
+  c->aperm = nullptr;
+  c->h_rows[c->ra] = hrows;
+  c->h_rows_ok[c->ra] = true;
+  c->h_rows_ok[c->rb] = false;
+  set_rows(c);
+  c->b_valid = false;
+  c->b_accl = false;
+  c->b_aligned = false;
+  c->exch_pending = false;
+  return check_err(c);
+}
+
 int wpic_bc_injection2d(wpic_ctx* c, double u0) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
   if (c->mixed) return mixed_only_step();

@@ -134,6 +134,12 @@ class Pic2D:
         """boundary_shock__injection(gp, np2, nxs, nxe, u0): 2d/proj/shock/boundary_shock.f90:255"""
         check(self._L.wpic_bc_injection2d(self._h, float(u0)))
 
+    def inject(self, seed, it, nl, v0, v_thi, v_the, l_damp, b0, theta_bn, phi_bn):
+        """the shock app's inject (2d/proj/shock/app.f90:697-856); nl[r] particles per species join row r"""
+        check(self._L.wpic_inject2d(self._h, int(seed), int(it), np.ascontiguousarray(nl, dtype=np.int32),
+                                    float(v0), float(v_thi), float(v_the), float(l_damp), float(b0),
+                                    float(theta_bn), float(phi_bn)))
+
     def set_injection(self, u0):
         """the u0 of the step's injection edge (WPIC_BC_SHOCK)"""
         check(self._L.wpic_set_injection2d(self._h, float(u0)))
