@@ -8,10 +8,11 @@
 !                                                  (2d/common/mom_calc.f90:18,48,167)
 !   paraio:    paraio__init, __finalize, __output, __input, __param, __mom,
 !              __ptcl, __orb                       (2d/common/paraio.f90:37-741)
-!   boundary_periodic / boundary_reconnection:
+!   boundary_periodic / boundary_reconnection / boundary_shock:
 !              *__init, *__particle_x, *__particle_y, *__mom and the boundary
 !              procedures handed to field__fdtd_i  (boundary_periodic.f90:26-636,
-!              2d/proj/reconnection/boundary_reconnection.f90:26-650)
+!              2d/proj/reconnection/boundary_reconnection.f90:26-650,
+!              2d/proj/shock/boundary_shock.f90:27-693, with __injection)
 ! An app built against these modules instead of the reference's runs its
 ! per-step loop on one MI355X.  The boundary procedures are accepted by
 ! field__fdtd_i for source compatibility and applied on the device according
@@ -426,6 +427,107 @@ contains
   end subroutine boundary_reconnection__mom
 
 end module boundary_reconnection
+
+
+! the shock app's boundaries (2d/proj/shock/boundary_shock.f90) with the box
+! fully expanded; the field boundaries are applied on the device
+module boundary_shock
+  use iso_c_binding
+  use wpic_capi
+  use wpic_shim_state
+  implicit none
+  private
+  public :: boundary_shock__init, boundary_shock__dfield, boundary_shock__particle_x
+  public :: boundary_shock__particle_y, boundary_shock__curre, boundary_shock__phi, boundary_shock__mom
+  public :: boundary_shock__injection
+
+contains
+
+  subroutine boundary_shock__init(ndim_in, np_in, nsp_in, nxgs_in, nxge_in, nygs_in, nyge_in, nys_in, &
+                                         nye_in, nup_in, ndown_in, mnpi_in, mnpr_in, ncomw_in, nerr_in, nstat_in, &
+                                         delx_in, delt_in, c_in)
+    integer, intent(in) :: ndim_in, np_in, nsp_in
+    integer, intent(in) :: nxgs_in, nxge_in, nygs_in, nyge_in, nys_in, nye_in
+    integer, intent(in) :: nup_in, ndown_in, mnpi_in, mnpr_in, ncomw_in, nerr_in, nstat_in(:)
+    real(8), intent(in) :: delx_in, delt_in, c_in
+    call shim_set_grid(ndim_in, np_in, nsp_in, nxgs_in, nxge_in, nygs_in, nyge_in, nys_in, nye_in)
+    prm%bc = WPIC_BC_SHOCK
+    have_bc = .true.
+  end subroutine boundary_shock__init
+
+  subroutine boundary_shock__particle_x(up, np2, nxs, nxe)
+    integer, intent(in)    :: nxs, nxe
+    integer, intent(in)    :: np2(s_nys:s_nye, s_nsp)
+    real(8), intent(inout) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    integer(c_int) :: n2(s_nys:s_nye, s_nsp)
+    call shim_ctx(np2)
+    if (.not. resident) call shim_check(wpic_upload_gp2d(ctx, up, np2, s_np))
+    call shim_check(wpic_bc_particle_x2d(ctx))
+    if (.not. resident) call shim_check(wpic_download_gp2d(ctx, up, n2, s_np))
+  end subroutine boundary_shock__particle_x
+
+  ! the injection edge moving with the inflow (boundary_shock.f90:255-297),
+  ! with nxs = nxgs, nxe = nxge
+  subroutine boundary_shock__injection(up, np2, nxs, nxe, u0)
+    integer, intent(in)    :: nxs, nxe
+    integer, intent(in)    :: np2(s_nys:s_nye, s_nsp)
+    real(8), intent(inout) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    real(8), intent(in)    :: u0
+    integer(c_int) :: n2(s_nys:s_nye, s_nsp)
+    call shim_ctx(np2)
+    if (.not. resident) call shim_check(wpic_upload_gp2d(ctx, up, np2, s_np))
+    call shim_check(wpic_bc_injection2d(ctx, u0))
+    if (.not. resident) call shim_check(wpic_download_gp2d(ctx, up, n2, s_np))
+  end subroutine boundary_shock__injection
+
+  subroutine boundary_shock__particle_y(up, np2)
+    integer, intent(inout) :: np2(s_nys:s_nye, s_nsp)
+    real(8), intent(inout) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
+    integer(c_int) :: n2(s_nys:s_nye, s_nsp)
+    call shim_ctx(np2)
+    if (.not. resident) call shim_check(wpic_upload_gp2d(ctx, up, np2, s_np))
+    call shim_check(wpic_bc_particle_y2d(ctx))
+    if (.not. resident) then
+       call shim_check(wpic_download_gp2d(ctx, up, n2, s_np))
+       np2 = n2
+    end if
+  end subroutine boundary_shock__particle_y
+
+  subroutine boundary_shock__dfield(df, nxs, nxe, nys, nye, nxgs, nxge)
+    integer, intent(in)    :: nxs, nxe, nys, nye, nxgs, nxge
+    real(8), intent(inout) :: df(6, nxgs-2:nxge+2, nys-2:nye+2)
+    write(6,*) 'boundary_shock__dfield: applied on the device inside field__fdtd_i'
+    stop
+  end subroutine boundary_shock__dfield
+
+  subroutine boundary_shock__curre(uj, nxs, nxe, nys, nye, nxgs, nxge)
+    integer, intent(in)    :: nxs, nxe, nys, nye, nxgs, nxge
+    real(8), intent(inout) :: uj(3, nxgs-2:nxge+2, nys-2:nye+2)
+    write(6,*) 'boundary_shock__curre: applied on the device inside field__fdtd_i'
+    stop
+  end subroutine boundary_shock__curre
+
+  subroutine boundary_shock__phi(phi, nxs, nxe, nys, nye, l)
+    integer, intent(in)    :: nxs, nxe, nys, nye, l
+    real(8), intent(inout) :: phi(nxs-1:nxe+1, nys-1:nye+1)
+    write(6,*) 'boundary_shock__phi: applied on the device inside field__fdtd_i'
+    stop
+  end subroutine boundary_shock__phi
+
+  ! the moment folds on the device's moments; mirror mode sends the host
+  ! array up and brings the folded one back
+  subroutine boundary_shock__mom(mom)
+    real(8), intent(inout) :: mom(7, s_nxgs-1:s_nxge+1, s_nys-1:s_nye+1, s_nsp)
+    if (.not. c_associated(ctx)) then
+       write(6,*) 'boundary_shock__mom: no device state (call mom_calc__accl first)'
+       stop
+    end if
+    if (.not. resident) call shim_check(wpic_upload_mom2d(ctx, mom))
+    call shim_check(wpic_bc_mom2d(ctx))
+    call shim_check(wpic_download_mom2d(ctx, mom))
+  end subroutine boundary_shock__mom
+
+end module boundary_shock
 
 
 ! resident-mode controls (no reference counterpart)

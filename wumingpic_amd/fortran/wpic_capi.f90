@@ -4,7 +4,7 @@ module wpic_capi
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: WPIC_BC_PERIODIC = 0, WPIC_BC_WALLX = 1
+  integer(c_int), parameter :: WPIC_BC_PERIODIC = 0, WPIC_BC_WALLX = 1, WPIC_BC_SHOCK = 2
   integer(c_int), parameter :: WPIC_MODE_FAST = 0, WPIC_MODE_PARITY = 1, WPIC_MODE_MIXED = 2
 
   type, bind(C) :: wpic_params2d
@@ -180,6 +180,12 @@ module wpic_capi
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function wpic_particle_solv2d
+
+     integer(c_int) function wpic_bc_injection2d(ctx, u0) bind(C, name='wpic_bc_injection2d')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: u0
+     end function wpic_bc_injection2d
 
      integer(c_int) function wpic_particle_solv_vay2d(ctx) bind(C, name='wpic_particle_solv_vay2d')
        import :: c_int, c_ptr
