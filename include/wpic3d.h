@@ -63,6 +63,12 @@ int wpic_upload_dfield3d(wpic_ctx3d *ctx, const double *df);
 /* reference entry points ------------------------------------------------- */
 /* particle__solv(gp, up, uf, cumcnt, nxs, nxe): 3d/common/particle.f90:52 */
 int wpic_particle_solv3d(wpic_ctx3d *ctx);
+/* particle__solv_vay(gp, up, uf, cumcnt, nxs, nxe): 3d/common/particle.f90:236
+ * -- the Vay kick with particle__solv's staging, gather and move; FP64 modes */
+int wpic_particle_solv_vay3d(wpic_ctx3d *ctx);
+/* the kick of wpic_particle_solv3d and wpic_step3d: WPIC_PUSHER_BORIS (the
+ * default) or WPIC_PUSHER_VAY; FP64 modes (moments keep mom_calc's Boris kick) */
+int wpic_set_pusher3d(wpic_ctx3d *ctx, int pusher);
 /* field__fdtd_i(uf, up, gp, cumcnt, nxs, nxe, bc_dfield, bc_curre, bc_phi):
  * 3d/common/field.f90:70, with boundary_periodic's procedures on the device */
 int wpic_field_fdtd_i3d(wpic_ctx3d *ctx);

@@ -81,7 +81,22 @@ static inline void shape3(double dh, double *m, double *z, double *pp) {
 }
 
 /* particle__solv: 3d/common/particle.f90:52-233 */
+static void particle_solv3(const or3d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt,
+                           int vay);
+
 void or3d_particle_solv(const or3d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt) {
+  particle_solv3(p, gp, up, uf, cumcnt, 0);
+}
+
+/* particle__solv_vay: 3d/common/particle.f90:236-419 (the Vay kick after the
+ * same staging and gather; Vay, Phys. Plasmas 15, 056701 (2008)) */
+void or3d_particle_solv_vay(const or3d_params *p, double *gp, const double *up, const double *uf,
+                            const int *cumcnt) {
+  particle_solv3(p, gp, up, uf, cumcnt, 1);
+}
+
+static void particle_solv3(const or3d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt,
+                           int vay) {
   const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye, nzs = p->nzs, nze = p->nze;
   const double d_delx = 1. / p->delx, delt = p->delt, c = p->c;
   const int tx = NX + 2, ty = NY + 2;
@@ -121,6 +136,29 @@ void or3d_particle_solv(const or3d_params *p, double *gp, const double *up, cons
 #undef G3
 #undef RY
 #undef RX
+            if (vay) {
+              const double w1 = u[3], w2 = u[4], w3 = u[5];
+              double gam = sqrt(c * c + w1 * w1 + w2 * w2 + w3 * w3);
+              const double fac1r = fac1 / gam;
+              const double uvm4 = w1 + fac2 * epx + fac1r * (+w2 * bpz - w3 * bpy);
+              const double uvm5 = w2 + fac2 * epy + fac1r * (+w3 * bpx - w1 * bpz);
+              const double uvm6 = w3 + fac2 * epz + fac1r * (+w1 * bpy - w2 * bpx);
+              const double taux = fac1 * bpx / c, tauy = fac1 * bpy / c, tauz = fac1 * bpz / c;
+              const double tau2 = taux * taux + tauy * tauy + tauz * tauz;
+              const double ua = (uvm4 * taux + uvm5 * tauy + uvm6 * tauz) / c;
+              const double sigma = 1.0 + (uvm4 * uvm4 + uvm5 * uvm5 + uvm6 * uvm6) / (c * c) - tau2;
+              const double gam2 = 0.5 * (sigma + sqrt(sigma * sigma + 4.0 * (tau2 + ua * ua)));
+              gam = sqrt(gam2);
+              const double s = 1.0 / (tau2 + gam2);
+              g[3] = s * (gam2 * uvm4 + c * ua * taux + gam * (uvm5 * tauz - uvm6 * tauy));
+              g[4] = s * (gam2 * uvm5 + c * ua * tauy + gam * (uvm6 * taux - uvm4 * tauz));
+              g[5] = s * (gam2 * uvm6 + c * ua * tauz + gam * (uvm4 * tauy - uvm5 * taux));
+              gam = 1.0 / gam;
+              g[0] = u[0] + g[3] * delt * gam;
+              g[1] = u[1] + g[4] * delt * gam;
+              g[2] = u[2] + g[5] * delt * gam;
+              continue;
+            }
             double uvm1 = u[3] + fac1 * epx;
             double uvm2 = u[4] + fac1 * epy;
             double uvm3 = u[5] + fac1 * epz;

@@ -123,6 +123,7 @@ def lib():
         L.or3d_fstate_free.argtypes = [S]
         L.or3d_field_consts.argtypes = [P3, np.ctypeslib.ndpointer(np.float64)]
         L.or3d_particle_solv.argtypes = [P3, dp, dp, dp, ip]
+        L.or3d_particle_solv_vay.argtypes = [P3, dp, dp, dp, ip]
         L.or3d_ele_cur.argtypes = [P3, dp, dp, dp, ip]
         L.or3d_bc_curre.argtypes = [P3, dp]
         L.or3d_bc_dfield.argtypes = [P3, dp]
@@ -358,6 +359,9 @@ class Oracle3D:
     def particle_solv(self, gp, up, uf, cumcnt):
         self.L.or3d_particle_solv(C.byref(self.p), gp, up, uf, cumcnt)
 
+    def particle_solv_vay(self, gp, up, uf, cumcnt):
+        self.L.or3d_particle_solv_vay(C.byref(self.p), gp, up, uf, cumcnt)
+
     def ele_cur(self, uj, up, gp, cumcnt):
         self.L.or3d_ele_cur(C.byref(self.p), uj, up, gp, cumcnt)
 
@@ -419,8 +423,9 @@ def have_ref3d():
 
 
 def run_ref3d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
-    """Run the reference's own 3D particle__solv (mode 1), sort__bucket (mode 2)
-    or mom_calc__accl + mom_calc__nvt (mode 3: returns gp, mom) through
+    """Run the reference's own 3D particle__solv (mode 1), sort__bucket (mode 2),
+    mom_calc__accl + mom_calc__nvt (mode 3: returns gp, mom) or particle__solv_vay
+    (mode 4) through
     oracle/_ref/ref_golden3d on raw files in case_dir."""
     os.makedirs(case_dir, exist_ok=True)
     with open(os.path.join(case_dir, "params.txt"), "w") as f:
@@ -438,7 +443,7 @@ def run_ref3d(case_dir, mode, p, up, np2=None, uf=None, cumcnt=None):
     env = dict(os.environ, OMP_NUM_THREADS="1")
     subprocess.run([REF_DRIVER3D, case_dir], check=True, env=env)
     gp = np.fromfile(os.path.join(case_dir, "gp.bin"), dtype=np.float64).reshape(up.shape, order="F")
-    if mode == 1:
+    if mode in (1, 4):
         return gp
     if mode == 3:
         mom = np.fromfile(os.path.join(case_dir, "mom.bin"), dtype=np.float64).reshape(

@@ -7,11 +7,12 @@
 !                           nys nye nzs nze / delx delt c / q(1:nsp) / r(1:nsp)
 !   mode 1 (push):  in  up.bin uf.bin cumcnt.bin      out gp.bin
 !   mode 2 (sort):  in  up.bin np2.bin                out gp.bin cumcnt.bin
+!   mode 4 (vay):   in  up.bin uf.bin cumcnt.bin      out gp.bin
 !   mode 3 (mom):   in  up.bin uf.bin cumcnt.bin np2.bin
 !                   out gp.bin (mom_calc__accl) mom.bin (mom_calc__nvt of that gp)
 ! Arrays are raw little-endian stream files in the reference's own layout.
 program ref_golden3d
-  use particle, only: particle__init, particle__solv
+  use particle, only: particle__init, particle__solv, particle__solv_vay
   use sort, only: sort__init, sort__bucket
   use mom_calc, only: mom_calc__init, mom_calc__accl, mom_calc__nvt
   implicit none
@@ -42,6 +43,13 @@ program ref_golden3d
      call read_i4(trim(dir)//'/cumcnt.bin', cumcnt, size(cumcnt))
      call particle__init(ndim,np,nsp,nxgs,nxge,nygs,nyge,nzgs,nzge,nys,nye,nzs,nze,delx,delt,c,q,r)
      call particle__solv(gp,up,uf,cumcnt,nxgs,nxge)
+     call write_r8(trim(dir)//'/gp.bin', gp, size(gp))
+  else if (mode == 4) then
+     allocate(uf(6,nxgs-2:nxge+2,nys-2:nye+2,nzs-2:nze+2))
+     call read_r8(trim(dir)//'/uf.bin', uf, size(uf))
+     call read_i4(trim(dir)//'/cumcnt.bin', cumcnt, size(cumcnt))
+     call particle__init(ndim,np,nsp,nxgs,nxge,nygs,nyge,nzgs,nzge,nys,nye,nzs,nze,delx,delt,c,q,r)
+     call particle__solv_vay(gp,up,uf,cumcnt,nxgs,nxge)
      call write_r8(trim(dir)//'/gp.bin', gp, size(gp))
   else if (mode == 3) then
      allocate(uf(6,nxgs-2:nxge+2,nys-2:nye+2,nzs-2:nze+2))

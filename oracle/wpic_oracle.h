@@ -144,7 +144,7 @@ void or2d_bc_mom(const or2d_params *p, double *mom);
  *   up(ndim=7, np, nys:nye, nzs:nze, nsp), uf(6, nxgs-2:nxge+2, nys-2:nye+2,
  *   nzs-2:nze+2), cumcnt(nxgs:nxge+1, nys:nye, nzs:nze, nsp), np2(nys:nye,
  *   nzs:nze, nsp).
- * Pinning: or3d_particle_solv and or3d_sort_bucket are bit-exact against the
+ * Pinning: or3d_particle_solv, or3d_particle_solv_vay and or3d_sort_bucket are bit-exact against the
  * reference's own 3d/common/particle.f90 and sort.f90 (oracle/_ref); the
  * deposit by discrete continuity; the CG by an FFT solve; the step by Gauss's
  * law; the y-z exchange order (boundary_periodic__particle_yz) is parity
@@ -166,6 +166,8 @@ int or3d_fstate_init(const or3d_params *p, or3d_fstate *st);
 void or3d_fstate_free(or3d_fstate *st);
 void or3d_field_consts(const or3d_params *p, double f[5]);
 /* 3d/common/particle.f90:52-233 */
+void or3d_particle_solv_vay(const or3d_params *p, double *gp, const double *up, const double *uf,
+                            const int *cumcnt);
 void or3d_particle_solv(const or3d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt);
 /* 3d/common/field.f90:211-406 */
 void or3d_ele_cur(const or3d_params *p, double *uj, const double *up, const double *gp, const int *cumcnt);

@@ -1,12 +1,12 @@
 """Write golden vectors from the reference's OWN 3D particle__solv,
-sort__bucket and mom_calc (3d/common/particle.f90:52-233,
+particle__solv_vay, sort__bucket and mom_calc (3d/common/particle.f90:52-419,
 3d/common/sort.f90:40-88, 3d/common/mom_calc.f90:49-333),
 compiled from the reference sources by `make -C oracle ref` and run at one
 OpenMP thread.
 
     python tests/golden/gen_ref3d.py        (needs oracle/_ref/ref_golden3d)
 
-Outputs tests/golden/ref3d_push.npz, ref3d_sort.npz and ref3d_mom.npz: the
+Outputs tests/golden/ref3d_push.npz, ref3d_vay.npz, ref3d_sort.npz and ref3d_mom.npz: the
 inputs, the parameters and the reference's outputs.  The fixtures are data
 only; tests/test_oracle3d.py checks the oracle against them.
 """
@@ -56,6 +56,25 @@ def push_case(tmp):
                         gp=gp, **params_dict(p))
 
 
+def vay_case(tmp):
+    """particle__solv_vay with strong fields (|fac1 B| ~ 1 for the electrons)"""
+    rng = np.random.default_rng(20261207)
+    nx, ny, nz, n_ppc = 8, 6, 5, 4
+    q, r = S.weibel_consts(n_ppc)
+    p = po.make_params3d(nx, ny, nz, np_cap=64, delt=0.5, q=q, r=r)
+    counts = [rng.integers(1, 7, size=(nx, ny, nz)) for _ in range(2)]
+    up, cumcnt, np2 = S.sorted_particles3d(p, n_ppc, rng, u_scale=(0.05, 0.4), drift=(0.0, 0.5),
+                                           colocate=False, counts=counts)
+    up[3:6, 2, 3, 1, 1] = [40.0, -25.0, 7.5]
+    nmax = int(np2.max())
+    up[6, nmax:] = rng.standard_normal(up[6, nmax:].shape)
+    bscale = 1.0 / abs(q[1] * 0.25)
+    uf = S.random_fields3d(p, rng, scale=0.3 * bscale, b0=0.5 * bscale)
+    gp = po.run_ref3d(os.path.join(tmp, "vay"), 4, p, up, uf=uf, cumcnt=cumcnt)
+    np.savez_compressed(os.path.join(HERE, "ref3d_vay.npz"), up=up, uf=uf, cumcnt=cumcnt, np2=np2,
+                        gp=gp, **params_dict(p))
+
+
 def sort_case(tmp):
     rng = np.random.default_rng(8)
     nx, ny, nz = 8, 6, 5
@@ -102,6 +121,7 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/ref_golden3d is missing: run `make -C oracle ref` first")
     with tempfile.TemporaryDirectory() as tmp:
         push_case(tmp)
+        vay_case(tmp)
         sort_case(tmp)
         mom_case(tmp)
-    print("wrote", *(os.path.join(HERE, f"ref3d_{k}.npz") for k in ("push", "sort", "mom")))
+    print("wrote", *(os.path.join(HERE, f"ref3d_{k}.npz") for k in ("push", "vay", "sort", "mom")))

@@ -148,3 +148,63 @@ def test_vay_rejected_in_the_mixed_form():
         with pytest.raises(WpicError, match="FP64"):
             sim.set_pusher(PUSHER_VAY)
         sim.set_pusher(PUSHER_BORIS)
+
+
+# ---------------------------------------------------------------------------
+# 3D (3d/common/particle.f90:236-419)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("mode", ["fast", "parity"])
+def test_vay3d_matches_reference_golden(mode):
+    from wumingpic_amd import MODE_FAST, MODE_PARITY
+    from wumingpic_amd.pic3d import Pic3D
+    d = np.load(os.path.join(GOLD, "ref3d_vay.npz"))
+    nx, ny, nz = (int(d[f"n{a}ge"] - d[f"n{a}gs"] + 1) for a in "xyz")
+    npc = int(d["np"])
+    up, uf, cc, np2 = (np.asfortranarray(d[k]) for k in ("up", "uf", "cumcnt", "np2"))
+    print(f"vay3d golden {mode}", flush=True)
+    with Pic3D(nx, ny, nz, list(d["q"]), list(d["r"]), delt=float(d["delt"]),
+               mode=MODE_FAST if mode == "fast" else MODE_PARITY, capacity=npc * ny * nz) as sim:
+        sim.upload_up(up, np2, cc)
+        sim.upload_fields(uf)
+        sim.particle_solv_vay()
+        gp, n2 = sim.download_gp(npc)
+    assert np.array_equal(n2, np2)
+    for s in range(2):
+        for k in range(nz):
+            for j in range(ny):
+                n = np2[j, k, s]
+                assert np.array_equal(_bits(gp[:, :n, j, k, s]), _bits(d["gp"][:, :n, j, k, s])), (s, j, k)
+
+
+def test_vay3d_steps_parity_mode_bit_exact():
+    """the 3D Weibel loop order (3d/proj/weibel/app.f90:99-108) with the Vay kick, 4 steps"""
+    from wumingpic_amd import MODE_PARITY
+    from wumingpic_amd._lib import PUSHER_VAY
+    from wumingpic_amd.pic3d import weibel_beams_3d
+    nx, ny, nz, n_ppc = 16, 8, 8, 4
+    sim = weibel_beams_3d(nx, ny, nz, n_ppc, seed=31, mode=MODE_PARITY, capacity=2 * n_ppc * nx * ny * nz)
+    q, r = S.weibel_consts(n_ppc)
+    p = po.make_params3d(nx, ny, nz, np_cap=3 * n_ppc * nx, delt=0.5, q=q, r=r)
+    print("vay3d steps parity", flush=True)
+    with sim:
+        up, np2, cc = sim.download_up(p.np)
+        uf = sim.download_fields()
+        sim.set_pusher(PUSHER_VAY)
+        sim.step(4)
+        u_dev, n2_dev, cc_dev = sim.download_up(p.np)
+        uf_dev = sim.download_fields()
+    o = po.Oracle3D(p)
+    gp = np.zeros_like(up, order="F")
+    for _ in range(4):
+        o.particle_solv_vay(gp, up, uf, cc)
+        o.field_fdtd_i(uf, up, gp, cc)
+        o.bc_particle_x(gp, np2)
+        o.bc_particle_yz(gp, np2)
+        o.sort_bucket(up, gp, cc, np2)
+    assert np.array_equal(n2_dev, np2) and np.array_equal(cc_dev, cc)
+    for s in range(2):
+        for k in range(nz):
+            for j in range(ny):
+                n = np2[j, k, s]
+                assert np.array_equal(_bits(u_dev[:, :n, j, k, s]), _bits(up[:, :n, j, k, s])), (s, j, k)
+    assert np.array_equal(_bits(uf_dev), _bits(uf))

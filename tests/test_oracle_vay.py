@@ -105,3 +105,14 @@ def test_vay_keeps_the_exb_drift(vd):
     boris = gb[2:5, :4, :, 1].reshape(3, -1)
     if vd >= 0.9:
         assert np.max(np.abs(boris - u[:, None])) > 1e-3
+
+
+def test_vay3d_golden_bit_exact():
+    """3d/common/particle.f90:236-419 against the reference's own build"""
+    d = np.load(os.path.join(GOLD, "ref3d_vay.npz"))
+    nx, ny, nz = (int(d[f"n{a}ge"] - d[f"n{a}gs"] + 1) for a in "xyz")
+    p = po.make_params3d(nx, ny, nz, np_cap=int(d["np"]), delt=float(d["delt"]), q=d["q"], r=d["r"])
+    gp = np.zeros_like(d["up"], order="F")
+    po.Oracle3D(p).particle_solv_vay(gp, np.asfortranarray(d["up"]), np.asfortranarray(d["uf"]),
+                                     np.asfortranarray(d["cumcnt"]))
+    assert np.array_equal(_bits(gp), _bits(d["gp"]))

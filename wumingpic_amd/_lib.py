@@ -40,7 +40,7 @@ EXPORTS3D = (
     "wpic_upload_up3d", "wpic_download_up3d", "wpic_upload_gp3d", "wpic_download_gp3d",
     "wpic_upload_fields3d", "wpic_download_fields3d", "wpic_download_current3d",
     "wpic_download_dfield3d", "wpic_upload_dfield3d",
-    "wpic_particle_solv3d", "wpic_field_fdtd_i3d", "wpic_bc_particle_x3d", "wpic_bc_particle_yz3d",
+    "wpic_particle_solv3d", "wpic_particle_solv_vay3d", "wpic_set_pusher3d", "wpic_field_fdtd_i3d", "wpic_bc_particle_x3d", "wpic_bc_particle_yz3d",
     "wpic_sort_bucket3d", "wpic_cg_iters3d", "wpic_step3d",
     "wpic_energy3d", "wpic_energy_species3d", "wpic_gauss3d", "wpic_counts3d", "wpic_ic_beams3d",
     "wpic_ic_weibel3d",
@@ -171,6 +171,8 @@ def lib():
         "wpic_download_dfield3d": [vp, d],
         "wpic_upload_dfield3d": [vp, d],
         "wpic_particle_solv3d": [vp],
+        "wpic_particle_solv_vay3d": [vp],
+        "wpic_set_pusher3d": [vp, C.c_int],
         "wpic_field_fdtd_i3d": [vp],
         "wpic_bc_particle_x3d": [vp],
         "wpic_bc_particle_yz3d": [vp],

@@ -38,14 +38,12 @@ __device__ inline void tile_origin(const Grid3D& g, int tile, int ntx, int nty, 
   k0 = g.nzs + (rest / nty) * p3::TZ;
 }
 
-// Buneman-Boris push of one particle of cell (i, j, k) (particle.f90:93-217)
-// from the staged tmpf; so is the LDS offset of the cell.  sh returns the
-// particle's old shape (x, y, z triplets), which the deposit reuses.  Shared by
-// k_push3d and the fused fast kernel, so both give the same gp bit for bit.
-__device__ inline void boris_push3(const double (*sf)[p3::SN], int so, int i, int j, int k, double d_delx, double delt,
-                                   double c, double fac1, double txxx, double fac2, double x, double y, double z,
-                                   double ux, double uy, double uz, double& nxp, double& nyp, double& nzp, double& g4,
-                                   double& g5, double& g6, double sh[9]) {
+// The six fields at a particle of cell (i, j, k) from the staged tmpf (so is
+// the LDS offset of the cell) with the second-order shape
+// (particle.f90:93-197); sh returns the particle's old shape (x, y, z
+// triplets), which the deposit reuses.
+__device__ inline void gather6_3(const double (*sf)[p3::SN], int so, int i, int j, int k, double d_delx, double x,
+                                 double y, double z, double* fl, double sh[9]) {
   using namespace p3;
   double shxm, shx, shxp, shym, shy, shyp, shzm, shz, shzp;
   shape3(x * d_delx - 5e-1 - i, shxm, shx, shxp);
@@ -54,7 +52,6 @@ __device__ inline void boris_push3(const double (*sf)[p3::SN], int so, int i, in
   // re-read the cell's staged fields from LDS for every particle instead of
   // holding all 162 in registers across the particle loop
   asm volatile("" : "+v"(so));
-  double fl[6];
 #pragma unroll
   for (int cm = 0; cm < 6; cm++) {
     const double* f = &sf[cm][so];
@@ -64,6 +61,20 @@ __device__ inline void boris_push3(const double (*sf)[p3::SN], int so, int i, in
 #undef RY
 #undef RX
   }
+  sh[0] = shxm; sh[1] = shx; sh[2] = shxp;
+  sh[3] = shym; sh[4] = shy; sh[5] = shyp;
+  sh[6] = shzm; sh[7] = shz; sh[8] = shzp;
+}
+
+// Buneman-Boris push of one particle of cell (i, j, k) (particle.f90:93-217).
+// Shared by k_push3d and the fused fast kernel, so both give the same gp bit
+// for bit.
+__device__ inline void boris_push3(const double (*sf)[p3::SN], int so, int i, int j, int k, double d_delx, double delt,
+                                   double c, double fac1, double txxx, double fac2, double x, double y, double z,
+                                   double ux, double uy, double uz, double& nxp, double& nyp, double& nzp, double& g4,
+                                   double& g5, double& g6, double sh[9]) {
+  double fl[6];
+  gather6_3(sf, so, i, j, k, d_delx, x, y, z, fl, sh);
   const double bpx = fl[0], bpy = fl[1], bpz = fl[2], epx = fl[3], epy = fl[4], epz = fl[5];
   double uvm1 = ux + fac1 * epx;
   double uvm2 = uy + fac1 * epy;
@@ -85,9 +96,38 @@ __device__ inline void boris_push3(const double (*sf)[p3::SN], int so, int i, in
   nxp = x + g4 * delt * gam;
   nyp = y + g5 * delt * gam;
   nzp = z + g6 * delt * gam;
-  sh[0] = shxm; sh[1] = shx; sh[2] = shxp;
-  sh[3] = shym; sh[4] = shy; sh[5] = shyp;
-  sh[6] = shzm; sh[7] = shz; sh[8] = shzp;
+}
+
+// Vay push of one particle of cell (i, j, k) (particle__solv_vay,
+// 3d/common/particle.f90:370-403; Vay, Phys. Plasmas 15, 056701 (2008)): the
+// same gather, u' = u + fac2 E + (fac1/gam) u x B, tau = fac1 B / c, the new
+// gamma from sigma and u'.tau, the implicit rotation, the move with that gamma.
+__device__ inline void vay_push3(const double (*sf)[p3::SN], int so, int i, int j, int k, double d_delx, double delt,
+                                 double c, double fac1, double fac2, double x, double y, double z, double ux,
+                                 double uy, double uz, double& nxp, double& nyp, double& nzp, double& g4, double& g5,
+                                 double& g6, double sh[9]) {
+  double fl[6];
+  gather6_3(sf, so, i, j, k, d_delx, x, y, z, fl, sh);
+  const double bpx = fl[0], bpy = fl[1], bpz = fl[2], epx = fl[3], epy = fl[4], epz = fl[5];
+  double gam = sqrt(c * c + ux * ux + uy * uy + uz * uz);
+  const double fac1r = fac1 / gam;
+  const double uvm4 = ux + fac2 * epx + fac1r * (+uy * bpz - uz * bpy);
+  const double uvm5 = uy + fac2 * epy + fac1r * (+uz * bpx - ux * bpz);
+  const double uvm6 = uz + fac2 * epz + fac1r * (+ux * bpy - uy * bpx);
+  const double taux = fac1 * bpx / c, tauy = fac1 * bpy / c, tauz = fac1 * bpz / c;
+  const double tau2 = taux * taux + tauy * tauy + tauz * tauz;
+  const double ua = (uvm4 * taux + uvm5 * tauy + uvm6 * tauz) / c;
+  const double sigma = 1.0 + (uvm4 * uvm4 + uvm5 * uvm5 + uvm6 * uvm6) / (c * c) - tau2;
+  const double gam2 = 0.5 * (sigma + sqrt(sigma * sigma + 4.0 * (tau2 + ua * ua)));
+  gam = sqrt(gam2);
+  const double s = 1.0 / (tau2 + gam2);
+  g4 = s * (gam2 * uvm4 + c * ua * taux + gam * (uvm5 * tauz - uvm6 * tauy));
+  g5 = s * (gam2 * uvm5 + c * ua * tauy + gam * (uvm6 * taux - uvm4 * tauz));
+  g6 = s * (gam2 * uvm6 + c * ua * tauz + gam * (uvm4 * tauy - uvm5 * taux));
+  gam = 1.0 / gam;
+  nxp = x + g4 * delt * gam;
+  nyp = y + g5 * delt * gam;
+  nzp = z + g6 * delt * gam;
 }
 
 }  // namespace wpic

@@ -36,6 +36,7 @@ struct Push3Args {
   double* uj;
   double* pjbuf;
   int ntx, nty, ntz;
+  int vay;  // the Vay kick instead of Buneman-Boris (particle__solv_vay)
 };
 
 __global__ __launch_bounds__(p3::NT) void k_push3d(Push3Args a) {
@@ -76,8 +77,13 @@ __global__ __launch_bounds__(p3::NT) void k_push3d(Push3Args a) {
         const int ia = a.ap ? a.ap[pi] : pi;
         const double x = a.A.x[ia], y = a.A.y[ia], z = a.A.z[ia];
         double nxp, nyp, nzp, g4, g5, g6, sh[9];
-        boris_push3(sf, (lk + 1) * (SW * SH) + (lj + 1) * SW + (li + 1), i, j, k, d_delx, delt, c, fac1, txxx, fac2,
-                    x, y, z, a.A.ux[ia], a.A.uy[ia], a.A.uz[ia], nxp, nyp, nzp, g4, g5, g6, sh);
+        const int so = (lk + 1) * (SW * SH) + (lj + 1) * SW + (li + 1);
+        if (a.vay)
+          vay_push3(sf, so, i, j, k, d_delx, delt, c, fac1, fac2, x, y, z, a.A.ux[ia], a.A.uy[ia], a.A.uz[ia], nxp,
+                    nyp, nzp, g4, g5, g6, sh);
+        else
+          boris_push3(sf, so, i, j, k, d_delx, delt, c, fac1, txxx, fac2, x, y, z, a.A.ux[ia], a.A.uy[ia],
+                      a.A.uz[ia], nxp, nyp, nzp, g4, g5, g6, sh);
         a.B.x[pi] = nxp;
         a.B.y[pi] = nyp;
         a.B.z[pi] = nzp;
@@ -463,6 +469,7 @@ constexpr int G = 8;       // lanes per cell
 constexpr int NXQ = 512;   // queued cell-changing particles per round (beyond: a global list)
 }
 
+template <bool VAY>
 __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) {
   using namespace p3;
   constexpr int GF = p3f::G;
@@ -546,8 +553,12 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
       const double ux = a.A.ux[ia], uy = a.A.uy[ia], uz = a.A.uz[ia];
       const double* kc = spc[sc];
       double nw[3], g4, g5, g6, sh[9];
-      boris_push3(sf, so, i, j, k, d_delx, delt, c, kc[0], kc[1], kc[2], x, y, z, ux, uy, uz, nw[0], nw[1], nw[2], g4,
-                  g5, g6, sh);
+      if (VAY)
+        vay_push3(sf, so, i, j, k, d_delx, delt, c, kc[0], kc[2], x, y, z, ux, uy, uz, nw[0], nw[1], nw[2], g4, g5,
+                  g6, sh);
+      else
+        boris_push3(sf, so, i, j, k, d_delx, delt, c, kc[0], kc[1], kc[2], x, y, z, ux, uy, uz, nw[0], nw[1], nw[2],
+                    g4, g5, g6, sh);
       a.B.x[pc] = nw[0];
       a.B.y[pc] = nw[1];
       a.B.z[pc] = nw[2];
@@ -704,6 +715,7 @@ static Push3Args push_args(wpic_ctx3d* c) {
   a.ntx = (c->g.nx + p3::TX - 1) / p3::TX;
   a.nty = (c->g.ny + p3::TY - 1) / p3::TY;
   a.ntz = (c->g.nz + p3::TZ - 1) / p3::TZ;
+  a.vay = c->pusher == WPIC_PUSHER_VAY;
   return a;
 }
 
@@ -735,7 +747,10 @@ hipError_t launch_push_deposit3d(wpic_ctx3d* c) {
   a.ovcell = c->holes;
   const int nblk = xcd_grid(a.ntx * a.nty * a.ntz);
   hipLaunchKernelGGL(k_zero_int3, dim3(1), dim3(1), 0, c->st, c->nov);
-  hipLaunchKernelGGL(k_push_deposit3d_fast, dim3(nblk), dim3(p3::NT), 0, c->st, a);
+  if (a.vay)
+    hipLaunchKernelGGL(k_push_deposit3d_fast<true>, dim3(nblk), dim3(p3::NT), 0, c->st, a);
+  else
+    hipLaunchKernelGGL(k_push_deposit3d_fast<false>, dim3(nblk), dim3(p3::NT), 0, c->st, a);
   hipLaunchKernelGGL(k_deposit3d_overflow, dim3(256), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }

@@ -411,6 +411,7 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   std::memset((void*)&c->g, 0, sizeof(c->g));
   c->P = *p;
   c->mode = p->mode;
+  c->pusher = WPIC_PUSHER_BORIS;
   c->mixed = p->mode == WPIC_MODE_MIXED;
   Grid3D& g = c->g;
   g.nx = nx;
@@ -653,6 +654,24 @@ int wpic_particle_solv3d(wpic_ctx3d* c) {
   int rc = push3(c);
   if (rc) return rc;
   return check_err3(c);
+}
+
+int wpic_particle_solv_vay3d(wpic_ctx3d* c) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  const int saved = c->pusher;
+  c->pusher = WPIC_PUSHER_VAY;
+  const int rc = wpic_particle_solv3d(c);
+  c->pusher = saved;
+  return rc;
+}
+
+int wpic_set_pusher3d(wpic_ctx3d* c, int pusher) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (pusher != WPIC_PUSHER_BORIS && pusher != WPIC_PUSHER_VAY) return set_error(WPIC_EINVAL, "bad pusher");
+  if (c->mixed && pusher != WPIC_PUSHER_BORIS)
+    return set_error(WPIC_EINVAL, "the Vay kick runs in the FP64 modes (the mixed form pushes with Buneman-Boris)");
+  c->pusher = pusher;
+  return 0;
 }
 
 int wpic_field_fdtd_i3d(wpic_ctx3d* c) {

@@ -85,6 +85,7 @@ struct wpic_ctx3d {
   int* seg;     // [ncol][8] arrival segments (source column << 4 | dir), in order
   int* nseg;    // [ncol]
   bool b_valid, b_aligned, exch_pending;
+  int pusher;   // WPIC_PUSHER_BORIS or WPIC_PUSHER_VAY
   bool b_accl;  // B is mom_calc__accl's gp: A's positions, A's cell order
   std::vector<int> h_cols[3];
   bool h_cols_ok[3];

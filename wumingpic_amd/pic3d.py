@@ -110,6 +110,14 @@ class Pic3D:
         check(self._L.wpic_upload_dfield3d(self._h, np.asfortranarray(df, dtype=np.float64)))
 
     # -- reference entry points ---------------------------------------------------
+    def particle_solv_vay(self):
+        """particle__solv_vay(gp, up, uf, cumcnt, nxs, nxe): 3d/common/particle.f90:236"""
+        check(self._L.wpic_particle_solv_vay3d(self._h))
+
+    def set_pusher(self, pusher):
+        """the kick of particle_solv and step: _lib.PUSHER_BORIS or _lib.PUSHER_VAY"""
+        check(self._L.wpic_set_pusher3d(self._h, int(pusher)))
+
     def particle_solv(self):
         """particle__solv(gp, up, uf, cumcnt, nxs, nxe): 3d/common/particle.f90:52"""
         check(self._L.wpic_particle_solv3d(self._h))
