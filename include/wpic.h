@@ -135,6 +135,14 @@ int wpic_set_injection2d(wpic_ctx *ctx, double u0);
  * (cumcnt(nxe) and cumcnt(nxe+1) grow by nl[r]); the upstream B and E are set
  * at nxe-1 and nxe.  The split of the injected number over rows (random in the
  * reference) is the caller's; `it` keys the draws of this call. */
+/* The shock app's initial state (2d/proj/shock/app.f90:405-470), box fully
+ * expanded: n0 (nxe - nxs - 1) particles per row per species on a regular x
+ * lattice over [nxs, nxe) dx, y uniform in the row and shared, Maxwellian
+ * momenta (v_thi, v_the) boosted by vprofile(x); B = b0 (cos theta,
+ * sin theta cos phi, sin theta sin phi) and E = -vprofile(i dx) x B / c at
+ * every node; sorted as the reference sorts them; counter-based draws. */
+int wpic_ic_shock2d(wpic_ctx *ctx, unsigned long long seed, int n0, double v0, double v_thi, double v_the,
+                    double l_damp, double b0, double theta_bn, double phi_bn);
 int wpic_inject2d(wpic_ctx *ctx, unsigned long long seed, int it, const int *nl, double v0, double v_thi,
                   double v_the, double l_damp, double b0, double theta_bn, double phi_bn);
 /* boundary_periodic__particle_y(gp, np2): boundary_periodic.f90:99 */

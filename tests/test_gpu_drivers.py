@@ -302,3 +302,61 @@ def test_driver_restart_at_the_elapsed_limit():
     o2.step(up2, gp2, ufr, cc2, np2r, steps - 1)
     _same_particles(snapn[1], snapn[2], up2, np2r)
     assert np.array_equal(_bits(snapn[3]), _bits(ufr))
+
+
+def test_shock_initial_state_matches_its_formulas():
+    """2d/proj/shock/app.f90:405-470 with the box fully expanded"""
+    from wumingpic_amd import MODE_PARITY
+    nx, ny, n0 = 32, 6, 4
+    u0 = -0.3
+    v0 = u0 / np.sqrt(1 + u0 * u0)
+    b0, th, ph, ld = 2.0, np.pi / 3, np.pi / 2, 6.0
+    np2_row = n0 * (nx - 2)
+    print("shock initial state", flush=True)
+    with _pic(nx, ny, (0.01, -0.01), (25.0, 1.0), po.BC_SHOCK, MODE_PARITY, np2_row * ny, 1.0) as sim:
+        sim.ic_shock(4, n0, v0, 0.01, 0.05, ld, b0, th, ph)
+        up, np2, cc = sim.download_up(5 * n0 * nx)
+        uf = sim.download_fields()
+    assert np.all(np2 == np2_row)
+    ii = np.arange(1, np2_row + 1)
+    x = (2 + (nx - 1) * (ii - 0.5) / np2_row) * 1.0
+    for s in range(2):
+        for j in range(ny):
+            assert np.array_equal(up[0, :np2_row, j, s], x)  # the lattice is already in cell order
+            assert np.all(np.floor(up[1, :np2_row, j, s]) == 2 + j)
+            ids = -up[5, :np2_row, j, s].view(np.int64)
+            assert np.array_equal(np.sort(ids), j * np2_row + ii)
+    vprof = lambda xx: 0.5 * v0 * (1 + np.tanh((xx - (ld + 2)) / (0.1 * ld)))  # noqa: E731
+    i = np.arange(0, nx + 4)[:, None] * 1.0
+    assert np.all(uf[0] == b0 * np.cos(th)) and np.all(uf[3] == 0)
+    assert np.allclose(uf[4], vprof(i) * uf[2], rtol=1e-14, atol=1e-16)
+    assert np.allclose(uf[5], -vprof(i) * uf[1], rtol=1e-14, atol=1e-16)
+    # the drift: far upstream the mean ux of the electrons is gamma v0
+    far = up[0, :np2_row, :, 1] > ld + 2 + 5
+    assert abs(up[2, :np2_row, :, 1][far].mean() - v0 / np.sqrt(1 - v0 * v0)) < 0.02
+
+
+def test_shock_app_run():
+    """the shock app loop (2d/proj/shock/app.f90:99-172) with the box fully expanded: injection each
+    step, the particle and moment files at their cadence, the final restart"""
+    nx, ny, n0, steps = 32, 8, 4, 6
+    par = {"num_process": 1, "n_ppc": n0, "n_x": nx, "n_x_ini": nx - 1, "n_y": ny, "u_inject": 0.3,
+           "mass_ratio": 25.0, "sigma_e": 0.1, "omega_pe": 0.1, "v_the": 0.05, "v_thi": 0.01, "theta_bn": 60.0,
+           "phi_bn": 90.0, "l_damp_ini": 6.0, "seed": 3}
+    cfg = {"config": {"verbose": 0, "datadir": ".", "max_elapsed": 1.0e6, "max_it": steps, "intvl_ptcl": steps,
+                      "intvl_mom": 3, "intvl_orb": 100, "intvl_expand": 1}, "parameter": par}
+    print("shock app run", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        path = _run_app(cfg, d)
+        names = set(os.listdir(d))
+        meta = json.load(open(os.path.join(d, "%07d_ptcl.json" % steps)))
+        cfg2 = json.load(open(path))
+    for n in ("%07d_mom.json" % 3, "%07d_mom.json" % 6, "%07d_ptcl.raw" % 6, "%07d_restart.raw" % (steps + 1),
+              "init_param.json"):
+        assert n in names, n
+    assert cfg2["config"]["restart_file"] == "%07d_restart" % (steps + 1)
+    v0 = 0.3 / np.sqrt(1 + 0.09)
+    pflux = n0 * v0 * 1.0 * 1.0 * ny
+    n_ion = meta["dataset"]["up01"]["shape"][1]
+    n_init = n0 * (nx - 2) * ny
+    assert n_init + steps * int(pflux) <= n_ion <= n_init + steps * (int(pflux) + 1)

@@ -464,6 +464,89 @@ hipError_t launch_ic_harris(wpic_ctx* c, unsigned long long seed, int np2, int i
   return hipGetLastError();
 }
 
+// The shock app's initial state (2d/proj/shock/app.f90:405-470) with the box
+// fully expanded (nxs = nxgs, nxe = nxge): per row np2 particles on a regular
+// x lattice over [nxs, nxe) dx, y uniform in the row and shared by the
+// species, Maxwellian momenta boosted by the inflow profile; written unsorted
+// into B (the reference sorts them next); the upstream field with
+// E = -vprofile(i dx) x B / c at every node.
+struct IcShockArgs {
+  Grid2D g;
+  Parts B;
+  unsigned long long seed;
+  int np2;
+  double v0, xc, xs, sd[2];
+};
+
+__device__ inline double vprofile(const IcShockArgs& a, double x) {
+  return 0.5 * a.v0 * (1 + tanh((x - a.xc) / a.xs));
+}
+
+__global__ void k_ic_shock(IcShockArgs a) {
+  const Grid2D& g = a.g;
+  const long long per = (long long)a.np2 * g.ny;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= per) return;
+  const int r = (int)(t / a.np2);
+  const int ii = (int)(t - (long long)r * a.np2) + 1;
+  const int j = g.nys + r;
+  const double c = g.c;
+  const double x = (g.nxgs + (g.nxge - g.nxgs) * (ii - 0.5) / a.np2) * g.delx;
+  const double y = (j + u01(rkey(a.seed, 7, r, 0, ii))) * g.delx;
+  for (int s = 0; s < 2; s++) {
+    double n0, n1, n2, n3;
+    normal2(rkey(a.seed, 50 + 4 * s, r, 0, ii), rkey(a.seed, 51 + 4 * s, r, 0, ii), n0, n1);
+    normal2(rkey(a.seed, 52 + 4 * s, r, 0, ii), rkey(a.seed, 53 + 4 * s, r, 0, ii), n2, n3);
+    double ux = a.sd[s] * n0;
+    const double uy = a.sd[s] * n1, uz = a.sd[s] * n2;
+    const double v1 = vprofile(a, x);
+    const double gam1 = 1 / sqrt(1 - (v1 / c) * (v1 / c));
+    const double gamp = sqrt(1 + (ux * ux + uy * uy + uz * uz) / (c * c));
+    ux = gam1 * (ux + v1 * gamp);
+    const long long d = s * g.ncap + t;
+    a.B.x[d] = x;
+    a.B.y[d] = y;
+    a.B.ux[d] = ux;
+    a.B.uy[d] = uy;
+    a.B.uz[d] = uz;
+    a.B.id[d] = (uint64_t)(-(t + 1));
+  }
+}
+
+__global__ void k_ic_shock_field(IcShockArgs a, double* uf, double bx, double by, double bz) {
+  const Grid2D& g = a.g;
+  const size_t pl = (size_t)g.fw * g.fh;
+  const size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (n >= pl) return;
+  const int i = g.nxgs - 2 + (int)(n % g.fw);
+  uf[n] = bx;
+  uf[pl + n] = by;
+  uf[2 * pl + n] = bz;
+  uf[3 * pl + n] = 0.0;
+  uf[4 * pl + n] = +vprofile(a, i * g.delx) * uf[2 * pl + n] / g.c;
+  uf[5 * pl + n] = -vprofile(a, i * g.delx) * uf[pl + n] / g.c;
+}
+
+hipError_t launch_ic_shock(wpic_ctx* c, unsigned long long seed, int np2, double v0, const double* sd, double l_damp,
+                           const double* b) {
+  IcShockArgs a;
+  a.g = c->g;
+  a.B = c->B;
+  a.seed = seed;
+  a.np2 = np2;
+  a.v0 = v0;
+  a.xc = l_damp + c->g.nxgs * c->g.delx;
+  a.xs = l_damp * 0.1;
+  a.sd[0] = sd[0];
+  a.sd[1] = sd[1];
+  const long long n = (long long)np2 * c->g.ny;
+  hipLaunchKernelGGL(k_ic_shock, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
+  const size_t pl = (size_t)c->g.fw * c->g.fh;
+  hipLaunchKernelGGL(k_ic_shock_field, dim3((unsigned)((pl + 255) / 256)), dim3(256), 0, c->st, a, c->uf, b[0], b[1],
+                     b[2]);
+  return hipGetLastError();
+}
+
 // The shock app's inject (2d/proj/shock/app.f90:697-856) with the box fully
 // expanded (nxe = nxge), fused with the re-layout it needs: one block per
 // destination row copies the row's sorted particles (read through the

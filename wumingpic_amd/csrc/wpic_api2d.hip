@@ -722,6 +722,42 @@ int wpic_set_injection2d(wpic_ctx* c, double u0) {
   return 0;
 }
 
+int wpic_ic_shock2d(wpic_ctx* c, unsigned long long seed, int n0, double v0, double v_thi, double v_the,
+                    double l_damp, double b0, double theta_bn, double phi_bn) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (c->mixed) return mixed_only_step();
+  if (c->g.bc != WPIC_BC_SHOCK) return set_error(WPIC_EINVAL, "the shock state needs WPIC_BC_SHOCK");
+  const Grid2D& g = c->g;
+  if (g.nsp != 2) return set_error(WPIC_EINVAL, "the shock state has two species (ions, electrons)");
+  if (!(std::fabs(v0) < g.c) || !(l_damp > 0.0)) return set_error(WPIC_EINVAL, "need |v0| < c and l_damp_ini > 0");
+  // np2 = n0 (nxe - nxs - 1) per row (app.f90:334)
+  const long long np2l = (long long)n0 * (g.nxge - g.nxgs - 1);
+  if (n0 <= 0 || np2l * g.ny > g.ncap) return set_error(WPIC_EMEM, "Error: Too large number of particles");
+  const int np2 = (int)np2l;
+  const int nr = g.nsp * g.ny;
+  std::vector<int> hrows(2 * nr);
+  for (int s = 0; s < g.nsp; s++)
+    for (int r = 0; r < g.ny; r++) {
+      hrows[s * g.ny + r] = (int)(s * g.ncap + (long long)r * np2);
+      hrows[nr + s * g.ny + r] = np2;
+    }
+  if (c->rb == c->ra) c->rb = pick_free(c->ra, c->rn);
+  set_rows(c);
+  HIPCHK(hipMemcpyAsync(c->rows[c->rb], hrows.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
+  const double sd[2] = {v_thi, v_the};
+  const double b[3] = {b0 * cos(theta_bn), b0 * sin(theta_bn) * cos(phi_bn), b0 * sin(theta_bn) * sin(phi_bn)};
+  HIPCHK(launch_ic_shock(c, seed, np2, v0, sd, l_damp, b));
+  const size_t pl = (size_t)g.fw * g.fh;
+  HIPCHK(launch_zero(c, c->df, 6 * pl));
+  c->h_rows[c->rb] = hrows;
+  c->h_rows_ok[c->rb] = true;
+  c->b_valid = true;
+  c->b_accl = false;
+  c->b_aligned = false;
+  c->exch_pending = false;
+  return wpic_sort_bucket2d(c);
+}
+
 int wpic_inject2d(wpic_ctx* c, unsigned long long seed, int it, const int* nl, double v0, double v_thi, double v_the,
                   double l_damp, double b0, double theta_bn, double phi_bn) {
   if (!c || !nl) return set_error(WPIC_EINVAL, "null argument");

@@ -10,7 +10,9 @@
 // The problem is the one whose parameters the config holds: "t_ani" for the
 // Weibel app, "nbg"/"ncs" for the reconnection app, "udrift" for the
 // counter-streaming electron beams of the headline benchmark (an extension:
-// the reference's Weibel app loads an anisotropic Maxwellian instead).
+// the reference's Weibel app loads an anisotropic Maxwellian instead),
+// "u_inject" for the shock app (2d/proj/shock/app.f90) with its box fully
+// expanded (n_x_ini = n_x - 1: relocate returns at once there).
 // Optional extension keys in "parameter": "seed" (the counter-based draws
 // replacing the reference's OS-seeded ones), "e1" (the X-point perturbation
 // the reconnection app hard-codes as 0.12).
@@ -94,6 +96,13 @@ std::string fmt_e(double v, int w, int d) {
   std::string s = std::string(std::signbit(v) ? "-" : "") + "0." + digits + eb;
   if ((int)s.size() > w) return std::string(w, '*');
   return std::string(w - (int)s.size(), ' ') + s;
+}
+
+uint64_t splitmix(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
 }
 
 double now_s() {
@@ -209,9 +218,9 @@ struct App {
     get_num(par, "seed", seedd);
     const unsigned long long seed = (unsigned long long)seedd;
     const bool weibel = par && par->get("t_ani"), beams = par && par->get("udrift"),
-               recon = par && par->get("nbg");
-    if ((int)weibel + (int)beams + (int)recon != 1)
-      return fail("Error: the parameter section must describe one problem (t_ani, udrift or nbg/ncs)");
+               recon = par && par->get("nbg"), shock = par && par->get("u_inject");
+    if ((int)weibel + (int)beams + (int)recon + (int)shock != 1)
+      return fail("Error: the parameter section must describe one problem (t_ani, udrift, nbg/ncs or u_inject)");
 
     P.ndim = 6;
     P.nsp = 2;
@@ -226,8 +235,45 @@ struct App {
     P.r[0] = mass_ratio;
     P.r[1] = 1.0;
     double b0 = 0, x0 = 0, y0 = 0, lcs = 0, rtemp = 0, t_ani = 0, udrift = 0, e1 = 0.12;
+    double u0 = 0, v0 = 0, theta = 0, phi = 0, l_damp = 0, pflux = 0;
     int nbg = 0, ncs = 0;
-    if (weibel || beams) {
+    if (shock) {
+      // 2d/proj/shock/app.f90:240-345 with the box fully expanded
+      int n_x_ini = 0;
+      double sigma_e = 0, omega_pe = 0, u_inject = 0;
+      get_int(par, "n_ppc", n0);
+      get_int(par, "n_x_ini", n_x_ini);
+      if (n_x_ini != nx - 1)
+        return fail("Error: this driver runs the fully expanded box (n_x_ini = n_x - 1); relocate is not built");
+      get_num(par, "u_inject", u_inject);
+      get_num(par, "sigma_e", sigma_e);
+      get_num(par, "omega_pe", omega_pe);
+      get_num(par, "v_the", vte);
+      get_num(par, "v_thi", vti);
+      get_num(par, "theta_bn", theta);
+      get_num(par, "phi_bn", phi);
+      get_num(par, "l_damp_ini", l_damp);
+      theta = theta * PI / 180;
+      phi = phi * PI / 180;
+      np = n0 * nx * 5;
+      P.bc = WPIC_BC_SHOCK;
+      P.delt = 1.0 * P.delx / P.c;  // cfl = 1
+      get_num(par, "delt", P.delt);
+      u0 = -std::fabs(u_inject);
+      const double gam0 = std::sqrt(1 + (u0 * u0) / (P.c * P.c));
+      v0 = u0 / gam0;
+      wpe = omega_pe;
+      wge = omega_pe * std::sqrt(sigma_e);
+      wpi = wpe / std::sqrt(mass_ratio);
+      wgi = wge / mass_ratio;
+      P.q[0] = +std::sqrt(gam0 * P.r[0] / (4 * PI * n0 / (P.delx * P.delx))) * wpi;
+      P.q[1] = -std::sqrt(gam0 * P.r[1] / (4 * PI * n0 / (P.delx * P.delx))) * wpe;
+      b0 = P.r[0] * P.c / P.q[0] * wgi * gam0;
+      if ((long long)n0 * (nx - 2) > np) return fail("Error: Too large number of particles");
+      // inject: n0 |v0| dt dx (nyge - nygs + 1) particles per step (app.f90:735)
+      pflux = n0 * std::fabs(v0) * P.delt * P.delx * (P.nyge - P.nygs + 1);
+      P.capacity = (long long)n0 * (nx - 1) * ny + (long long)(cfg.max_it + 1) * ((long long)pflux + 1);
+    } else if (weibel || beams) {
       // 2d/proj/weibel/app.f90:255-318
       double sigma_e = 0, omega_pe = 0, v_the = 0, v_thi = 0;
       get_int(par, "n_ppc", n0);
@@ -316,16 +362,39 @@ struct App {
         rc = wpic_ic_weibel2d(ctx, seed, n0, vti, vte, t_ani, b0);
       else if (beams)
         rc = wpic_ic_beams2d(ctx, seed, n0, udrift, vte, vti, b0);
+      else if (shock)
+        rc = wpic_ic_shock2d(ctx, seed, n0, v0, vti, vte, l_damp, b0, theta, phi);
       else
         rc = wpic_ic_harris2d(ctx, seed, nbg, ncs, lcs, x0, y0, b0, e1, rtemp, vti, vte);
       if (chk(rc)) return -1;
-      if (energy_history(0)) return -1;
+      if (!shock && energy_history(0)) return -1;  // the shock app keeps no energy history
     }
+    if (shock && chk(wpic_set_injection2d(ctx, u0))) return -1;
+    std::vector<int> nl(ny);
 
     mom.assign((size_t)7 * (nx + 2) * (ny + 2) * P.nsp, 0.0);
     const double t0 = now_s();
     for (int it = it0 + 1; it <= cfg.max_it; it++) {
       if (chk(wpic_step2d(ctx, 1))) return -1;
+      if (shock) {
+        // inject (app.f90:731-760): int(pflux) particles plus one with the
+        // fractional probability, split evenly over the rows with the
+        // remainder given to randomly chosen rows
+        uint64_t h = splitmix(seed ^ (0x9e3779b97f4a7c15ull * (uint64_t)it));
+        int nginj = (int)pflux;
+        if ((double)(h >> 11) * 0x1.0p-53 < pflux - (int)pflux) nginj++;
+        std::vector<int> idx(ny);
+        for (int r = 0; r < ny; r++) {
+          nl[r] = nginj / ny;
+          idx[r] = r;
+        }
+        for (int r = ny - 1; r > 0; r--) {
+          h = splitmix(h);
+          std::swap(idx[r], idx[(int)(h % (uint64_t)(r + 1))]);
+        }
+        for (int r = 0; r < nginj % ny; r++) nl[idx[r]]++;
+        if (chk(wpic_inject2d(ctx, seed, it, nl.data(), v0, vti, vte, l_damp, b0, theta, phi))) return -1;
+      }
       if (cfg.intvl_ptcl > 0 && it % cfg.intvl_ptcl == 0) {
         if (pull_particles() || chk(wpic_paraio_ptcl(io, up.data(), uf.data(), np2.data(), it))) return -1;
       }
@@ -336,7 +405,7 @@ struct App {
         if (chk(wpic_moments2d(ctx)) || chk(wpic_download_mom2d(ctx, mom.data())) ||
             chk(wpic_download_fields2d(ctx, uf.data())) || chk(wpic_paraio_mom(io, mom.data(), uf.data(), it)))
           return -1;
-        if (energy_history(it)) return -1;
+        if (!shock && energy_history(it)) return -1;
       }
       const double etime = now_s() - t0;
       if (etime >= cfg.max_elapsed) {
@@ -351,7 +420,7 @@ struct App {
     // the Weibel app saves the final state, numbered max_it + 1
     // (weibel/app.f90:150-153); the reconnection app only finalizes
     // (reconnection/app.f90:146-148)
-    return recon ? 0 : save_restart(cfg.max_it + 1);
+    return recon ? 0 : save_restart(cfg.max_it + 1);  // the shock app saves it too (app.f90:168-171)
   }
 };
 

@@ -294,6 +294,8 @@ hipError_t launch_gauss(wpic_ctx* c, double* G, double* rho);
 hipError_t launch_codes_identity(wpic_ctx* c);
 // particles: boundaries, exchange, sort
 hipError_t launch_wrapx(wpic_ctx* c, bool injection);
+hipError_t launch_ic_shock(wpic_ctx* c, unsigned long long seed, int np2, double v0, const double* sd, double l_damp,
+                           const double* b);
 hipError_t launch_inject(wpic_ctx* c, unsigned long long seed, int it, const int* base_new, const int* nl,
                          const long long* idoff, double v0, const double* sd, double l_damp, double by, double bz);
 hipError_t launch_rowscan(wpic_ctx* c, bool ywrap);

@@ -140,6 +140,11 @@ class Pic2D:
                                     float(v0), float(v_thi), float(v_the), float(l_damp), float(b0),
                                     float(theta_bn), float(phi_bn)))
 
+    def ic_shock(self, seed, n0, v0, v_thi, v_the, l_damp, b0, theta_bn, phi_bn):
+        """the shock app's initial state (2d/proj/shock/app.f90:405-470), box fully expanded"""
+        check(self._L.wpic_ic_shock2d(self._h, int(seed), int(n0), float(v0), float(v_thi), float(v_the),
+                                      float(l_damp), float(b0), float(theta_bn), float(phi_bn)))
+
     def set_injection(self, u0):
         """the u0 of the step's injection edge (WPIC_BC_SHOCK)"""
         check(self._L.wpic_set_injection2d(self._h, float(u0)))
