@@ -54,6 +54,10 @@ int wpic_paraio_input(wpic_paraio *io, double *up, double *uf, int *np2, int *nx
 /* paraio__param (2d/common/paraio.f90:431-550, 3d/common/paraio.f90:532-655) */
 int wpic_paraio_param(wpic_paraio *io, int n0, double wpe, double wpi, double wge, double wgi, double vti,
                       double vte, const char *filename);
+/* the shock app's save_param (2d/proj/shock/app.f90:567-612): the pair of
+ * wpic_paraio_param with the attribute u0 appended */
+int wpic_paraio_param_shock(wpic_paraio *io, int n0, double wpe, double wpi, double wge, double wgi, double vti,
+                            double vte, double u0, const char *filename);
 /* paraio__mom (2d/common/paraio.f90:555-713, 3d/common/paraio.f90:660-838)
  * -> <it>_mom.raw/.json.  Divides mom(2:7) by mom(1) in place, as the
  * reference does. */

@@ -239,3 +239,28 @@ def test_fresh_state_all_routines_against_the_reference_library(tag):
     for rt in G.ROUTINES:
         assert mine[rt] == ref[rt], rt
     assert np.array_equal(m.view(np.uint64), divided.view(np.uint64))
+
+
+def test_shock_param_appends_u0():
+    """the shock app's save_param (2d/proj/shock/app.f90:567-612) re-prints paraio__param's JSON
+    with the attribute u0 added last, its 8 bytes appended to the raw file: everything before it is
+    paraio__param's pair byte for byte"""
+    import json as _json
+    import tempfile as _tf
+    from wumingpic_amd.paraio import ParaIO
+    with _tf.TemporaryDirectory() as t:
+        io = ParaIO(ndim=6, np_cap=40, nsp=2, nxgs=2, nxge=9, nygs=2, nyge=5, delx=1.0, delt=1.0, c=1.0,
+                    q=[0.01, -0.01], r=[25.0, 1.0], dir=t + "/")
+        io.param(4, 0.1, 0.02, 0.03, 0.0012, 0.01, 0.05, "a")
+        io.param_shock(4, 0.1, 0.02, 0.03, 0.0012, 0.01, 0.05, -0.3, "b")
+        io.close()
+        ra, rb = (open(os.path.join(t, n + ".raw"), "rb").read() for n in "ab")
+        ja, jb = (_json.load(open(os.path.join(t, n + ".json"))) for n in "ab")
+        ta = open(os.path.join(t, "a.json")).read()
+        tb = open(os.path.join(t, "b.json")).read()
+    assert rb[:len(ra)] == ra and len(rb) == len(ra) + 8
+    assert np.frombuffer(rb[-8:], dtype="<f8")[0] == -0.3
+    assert list(jb["attribute"])[-1] == "u0" and jb["attribute"]["u0"]["offset"] == len(ra)
+    assert list(ja["attribute"]) == list(jb["attribute"])[:-1]
+    # the JSON text differs only by the new entry and the raw file name
+    assert tb.replace('"b.raw"', '"a.raw"').startswith(ta[:ta.index('"n0"')])

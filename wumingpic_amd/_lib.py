@@ -53,7 +53,8 @@ EXPORTS3D = (
 # every symbol include/wpic_io.h declares
 EXPORTS_IO = (
     "wpic_paraio_init2d", "wpic_paraio_init3d", "wpic_paraio_finalize", "wpic_paraio_output",
-    "wpic_paraio_input", "wpic_paraio_param", "wpic_paraio_mom", "wpic_paraio_ptcl", "wpic_paraio_orb",
+    "wpic_paraio_input", "wpic_paraio_param", "wpic_paraio_param_shock", "wpic_paraio_mom", "wpic_paraio_ptcl",
+    "wpic_paraio_orb",
 )
 
 
@@ -209,6 +210,7 @@ def lib():
         "wpic_paraio_output": [vp, d, d, i, ci, ci, ci, C.c_char_p],
         "wpic_paraio_input": [vp, d, d, i, ip, ip, ip, C.c_char_p],
         "wpic_paraio_param": [vp, ci, cd, cd, cd, cd, cd, cd, C.c_char_p],
+        "wpic_paraio_param_shock": [vp, ci, cd, cd, cd, cd, cd, cd, cd, C.c_char_p],
         "wpic_paraio_mom": [vp, d, d, ci],
         "wpic_paraio_ptcl": [vp, d, d, i, ci],
         "wpic_paraio_orb": [vp, d, d, i, ci],

@@ -356,7 +356,9 @@ struct App {
       if (chk(wpic_sort_bucket2d(ctx))) return -1;
     } else {
       if (chk(wpic_create2d(&ctx, &P))) return -1;
-      if (chk(wpic_paraio_param(io, n0, wpe, wpi, wge, wgi, vti, vte, "init_param"))) return -1;
+      if (chk(shock ? wpic_paraio_param_shock(io, n0, wpe, wpi, wge, wgi, vti, vte, u0, "init_param")
+                    : wpic_paraio_param(io, n0, wpe, wpi, wge, wgi, vti, vte, "init_param")))
+        return -1;
       int rc;
       if (weibel)
         rc = wpic_ic_weibel2d(ctx, seed, n0, vti, vte, t_ani, b0);

@@ -502,8 +502,27 @@ int wpic_paraio_input(wpic_paraio* io, double* up, double* uf, int* np2, int* nx
 }
 
 // paraio__param: 2d/common/paraio.f90:431-550, 3d/common/paraio.f90:532-655
+namespace {
+int write_param(wpic_paraio* io, int n0, double wpe, double wpi, double wge, double wgi, double vti, double vte,
+                const double* u0, const char* filename);
+}
+
 int wpic_paraio_param(wpic_paraio* io, int n0, double wpe, double wpi, double wge, double wgi, double vti, double vte,
                       const char* filename) {
+  return write_param(io, n0, wpe, wpi, wge, wgi, vti, vte, nullptr, filename);
+}
+
+// the shock app's save_param (2d/proj/shock/app.f90:567-612): paraio__param's
+// pair with the attribute u0 appended (JSON entry last, 8 bytes at the end of
+// the raw file)
+int wpic_paraio_param_shock(wpic_paraio* io, int n0, double wpe, double wpi, double wge, double wgi, double vti,
+                            double vte, double u0, const char* filename) {
+  return write_param(io, n0, wpe, wpi, wge, wgi, vti, vte, &u0, filename);
+}
+
+namespace {
+int write_param(wpic_paraio* io, int n0, double wpe, double wpi, double wge, double wgi, double vti, double vte,
+                const double* u0, const char* filename) {
   if (!io) return set_error(WPIC_EINVAL, "Initialize first by calling paraio__init()");
   if (!filename) return set_error(WPIC_EINVAL, "null argument");
   const std::string name = filename;
@@ -532,8 +551,10 @@ int wpic_paraio_param(wpic_paraio* io, int n0, double wpe, double wpi, double wg
   w.attr_r8("vai", vai);
   w.attr_r8("vae", vae);
   w.attr_i4("n0", n0);
+  if (u0) w.attr_r8("u0", *u0);
   return close_writer(w, io, name);
 }
+}  // namespace
 
 // paraio__mom: 2d/common/paraio.f90:555-713, 3d/common/paraio.f90:660-838.
 // Divides mom(2:4) and mom(5:7) by mom(1) in place, as the reference does.

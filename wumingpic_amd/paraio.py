@@ -54,6 +54,10 @@ class ParaIO:
         """paraio__param: 2d/common/paraio.f90:431"""
         check(self._L.wpic_paraio_param(self._h, n0, wpe, wpi, wge, wgi, vti, vte, filename.encode()))
 
+    def param_shock(self, n0, wpe, wpi, wge, wgi, vti, vte, u0, filename):
+        """the shock app's save_param: paraio__param's pair with the attribute u0 appended"""
+        check(self._L.wpic_paraio_param_shock(self._h, n0, wpe, wpi, wge, wgi, vti, vte, u0, filename.encode()))
+
     def mom(self, mom, uf, it):
         """paraio__mom: divides mom(2:7) by mom(1) in place, as the reference"""
         check(self._L.wpic_paraio_mom(self._h, mom, np.asfortranarray(uf), it))
