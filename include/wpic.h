@@ -134,7 +134,8 @@ int wpic_set_injection2d(wpic_ctx *ctx, double u0);
  * (l_damp/10))), ids continuing each species' count; they close cell nxe-1
  * (cumcnt(nxe) and cumcnt(nxe+1) grow by nl[r]); the upstream B and E are set
  * at nxe-1 and nxe.  The split of the injected number over rows (random in the
- * reference) is the caller's; `it` keys the draws of this call. */
+ * reference) is the caller's; `it` keys the draws of this call.  nl must
+ * hold ny = nyge - nygs + 1 entries (one per row); the call reads that many. */
 /* The shock app's initial state (2d/proj/shock/app.f90:405-470), box fully
  * expanded: n0 (nxe - nxs - 1) particles per row per species on a regular x
  * lattice over [nxs, nxe) dx, y uniform in the row and shared, Maxwellian

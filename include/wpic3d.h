@@ -79,6 +79,11 @@ int wpic_bc_particle_yz3d(wpic_ctx3d *ctx);
 /* sort__bucket(up, gp, cumcnt, np2, nxs, nxe): 3d/common/sort.f90:40 */
 int wpic_sort_bucket3d(wpic_ctx3d *ctx);
 int wpic_cg_iters3d(wpic_ctx3d *ctx, int iters[3]);
+/* relative CG stopping tolerance of field__fdtd_i (the reference's err =
+ * 1e-6, 3d/common/field.f90:429); no counterpart in the reference, which
+ * fixes it: for callers that need a tighter solve (closed-form checks,
+ * finite differences) */
+int wpic_set_cg_tol3d(wpic_ctx3d *ctx, double err);
 
 /* the loop body of 3d/proj/weibel/app.f90:103-108, fused on the device */
 int wpic_step3d(wpic_ctx3d *ctx, int nsteps);

@@ -454,7 +454,7 @@ int or3d_cgm(const or3d_params *p, or3d_fstate *st) {
           sum = sum + IN(b, i, j, k) * IN(b, i, j, k);
         }
     double sum_g = sum;
-    const double eps = sqrt(sum_g) * 1e-6;
+    const double eps = sqrt(sum_g) * or_cg_err();
     or3d_bc_phi(p, phi, l);
     double sumr = 0e0;
     for (int k = nzs; k <= nze; k++)
@@ -521,14 +521,21 @@ int or3d_cgm(const or3d_params *p, or3d_fstate *st) {
 /* field__fdtd_i: 3d/common/field.f90:70-208 */
 int or3d_field_fdtd_i(const or3d_params *p, or3d_fstate *st, double *uf, const double *up, const double *gp,
                       const int *cumcnt) {
+  double *uj = st->uj;
+  or3d_ele_cur(p, uj, up, gp, cumcnt);
+  or3d_bc_curre(p, uj);
+  return or3d_field_from_current(p, st, uf);
+}
+
+/* 3d/common/field.f90:127-206: the RHS, the three solves, E, the ghost fills
+ * and the update from the current in st->uj (after its boundary fold) */
+int or3d_field_from_current(const or3d_params *p, or3d_fstate *st, double *uf) {
   const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye, nzs = p->nzs, nze = p->nze;
   const double pi = M_PI, delt = p->delt, gfac = p->gfac;
   double f[5];
   or3d_field_consts(p, f);
   const double f1 = f[0], f2 = f[1], f3 = f[2];
   double *uj = st->uj, *df = st->df;
-  or3d_ele_cur(p, uj, up, gp, cumcnt);
-  or3d_bc_curre(p, uj);
 #pragma omp parallel for
   for (int k = nzs; k <= nze; k++)
     for (int j = nys; j <= nye; j++)

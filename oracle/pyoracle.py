@@ -96,6 +96,8 @@ def lib():
         P = C.POINTER(Params2D)
         S = C.POINTER(FState2D)
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_cg_err.argtypes = [C.c_double]
+        L.or_cg_err.restype = C.c_double
         L.or_last_error.restype = C.c_char_p
         L.or2d_fstate_init.argtypes = [P, S]
         L.or2d_fstate_free.argtypes = [S]
@@ -109,6 +111,7 @@ def lib():
         L.or2d_bc_phi.argtypes = [P, dp, C.c_int]
         L.or2d_field_fdtd_i.argtypes = [P, S, dp, dp, dp, ip]
         L.or2d_cgm.argtypes = [P, S]
+        L.or2d_field_from_current.argtypes = [P, S, dp]
         L.or2d_bc_particle_x.argtypes = [P, dp, ip]
         L.or2d_bc_particle_y.argtypes = [P, dp, ip]
         L.or2d_sort_bucket.argtypes = [P, dp, dp, ip, ip]
@@ -130,6 +133,7 @@ def lib():
         L.or3d_bc_phi.argtypes = [P3, dp, C.c_int]
         L.or3d_field_fdtd_i.argtypes = [P3, S, dp, dp, dp, ip]
         L.or3d_cgm.argtypes = [P3, S]
+        L.or3d_field_from_current.argtypes = [P3, S, dp]
         L.or3d_bc_particle_x.argtypes = [P3, dp, ip]
         L.or3d_bc_particle_yz.argtypes = [P3, dp, ip]
         L.or3d_sort_bucket.argtypes = [P3, dp, dp, ip, ip]
@@ -144,6 +148,15 @@ def lib():
             getattr(L, f).restype = C.c_double
         _lib = L
     return _lib
+
+
+def set_cg_err(err):
+    """cgm's relative stopping tolerance for every later solve (the
+    reference's 1e-6 by default); returns the previous value"""
+    L = lib()
+    old = L.or_cg_err()
+    L.or_set_cg_err(err)
+    return old
 
 
 def _check(rc):
@@ -226,6 +239,11 @@ class Oracle2D:
 
     def cgm(self):
         _check(self.L.or2d_cgm(C.byref(self.p), C.byref(self.st)))
+        return tuple(self.st.cg_iters)
+
+    def field_from_current(self, uf):
+        """field__fdtd_i after ele_cur and the current fold, from self.uj"""
+        _check(self.L.or2d_field_from_current(C.byref(self.p), C.byref(self.st), uf))
         return tuple(self.st.cg_iters)
 
     def bc_particle_x(self, up, np2):
@@ -380,6 +398,11 @@ class Oracle3D:
 
     def cgm(self):
         _check(self.L.or3d_cgm(C.byref(self.p), C.byref(self.st)))
+        return tuple(self.st.cg_iters)
+
+    def field_from_current(self, uf):
+        """field__fdtd_i after ele_cur and the current fold, from self.uj"""
+        _check(self.L.or3d_field_from_current(C.byref(self.p), C.byref(self.st), uf))
         return tuple(self.st.cg_iters)
 
     def bc_particle_x(self, up, np2):

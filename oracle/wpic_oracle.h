@@ -87,6 +87,9 @@ int or2d_fstate_init(const or2d_params *p, or2d_fstate *st);
 void or2d_fstate_free(or2d_fstate *st);
 
 void or_set_threads(int n);
+/* cgm's relative stopping tolerance (default 1e-6, field.f90:339) */
+void or_set_cg_err(double e);
+double or_cg_err(void);
 const char *or_last_error(void);
 
 /* field.f90:22-63 */
@@ -112,6 +115,8 @@ int or2d_field_fdtd_i(const or2d_params *p, or2d_fstate *st, double *uf,
                       const double *up, const double *gp, const int *cumcnt);
 /* field.f90:319-461 on st->df, st->gkl */
 int or2d_cgm(const or2d_params *p, or2d_fstate *st);
+/* field.f90:124-184 from the folded current already in st->uj */
+int or2d_field_from_current(const or2d_params *p, or2d_fstate *st, double *uf);
 /* boundary_periodic.f90:61-96 / boundary_reconnection.f90:61-99 */
 void or2d_bc_particle_x(const or2d_params *p, double *up, const int *np2);
 /* boundary_periodic.f90:99-248 (identical in boundary_reconnection.f90) */
@@ -177,6 +182,8 @@ void or3d_bc_dfield(const or3d_params *p, double *df);
 void or3d_bc_phi(const or3d_params *p, double *phi, int l);
 /* 3d/common/field.f90:409-560 */
 int or3d_cgm(const or3d_params *p, or3d_fstate *st);
+/* 3d/common/field.f90:127-206 from the folded current already in st->uj */
+int or3d_field_from_current(const or3d_params *p, or3d_fstate *st, double *uf);
 /* 3d/common/field.f90:70-208 */
 int or3d_field_fdtd_i(const or3d_params *p, or3d_fstate *st, double *uf, const double *up, const double *gp,
                       const int *cumcnt);

@@ -360,3 +360,27 @@ def test_shock_app_run():
     n_ion = meta["dataset"]["up01"]["shape"][1]
     n_init = n0 * (nx - 2) * ny
     assert n_init + steps * int(pflux) <= n_ion <= n_init + steps * (int(pflux) + 1)
+
+
+def test_shock_tracers_in_the_initial_state_and_orb_files():
+    """set_particle_ids (2d/proj/shock/app.f90:500-524): the ids of particles
+    starting at 450 <= x <= 500 are positive, so the orbit files carry them
+    (io__orb writes positive ids only)"""
+    nx, ny, n0, steps = 520, 2, 1, 2
+    par = {"num_process": 1, "n_ppc": n0, "n_x": nx, "n_x_ini": nx - 1, "n_y": ny, "u_inject": 0.3,
+           "mass_ratio": 25.0, "sigma_e": 0.1, "omega_pe": 0.1, "v_the": 0.05, "v_thi": 0.01, "theta_bn": 60.0,
+           "phi_bn": 90.0, "l_damp_ini": 6.0, "seed": 5}
+    cfg = {"config": {"verbose": 0, "datadir": ".", "max_elapsed": 1.0e6, "max_it": steps, "intvl_ptcl": 100,
+                      "intvl_mom": 100, "intvl_orb": 1, "intvl_expand": 1}, "parameter": par}
+    np2_row = n0 * (nx - 2)
+    ii = np.arange(1, np2_row + 1)
+    x = 2 + (nx - 1) * (ii - 0.5) / np2_row
+    want = int(((x >= 450.0) & (x <= 500.0)).sum()) * ny
+    assert want > 0
+    print("shock tracers", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        _run_app(cfg, d)
+        for it in (1, 2):
+            meta = json.load(open(os.path.join(d, "%07d_orb.json" % it)))
+            for sp in ("up01", "up02"):
+                assert meta["dataset"][sp]["shape"][1] == want, (it, sp, meta["dataset"][sp]["shape"])

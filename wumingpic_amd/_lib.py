@@ -41,7 +41,7 @@ EXPORTS3D = (
     "wpic_upload_fields3d", "wpic_download_fields3d", "wpic_download_current3d",
     "wpic_download_dfield3d", "wpic_upload_dfield3d",
     "wpic_particle_solv3d", "wpic_particle_solv_vay3d", "wpic_set_pusher3d", "wpic_field_fdtd_i3d", "wpic_bc_particle_x3d", "wpic_bc_particle_yz3d",
-    "wpic_sort_bucket3d", "wpic_cg_iters3d", "wpic_step3d",
+    "wpic_sort_bucket3d", "wpic_cg_iters3d", "wpic_set_cg_tol3d", "wpic_step3d",
     "wpic_energy3d", "wpic_energy_species3d", "wpic_gauss3d", "wpic_counts3d", "wpic_ic_beams3d",
     "wpic_ic_weibel3d",
     "wpic_set_timing3d", "wpic_timing3d",
@@ -181,6 +181,7 @@ def lib():
         "wpic_bc_particle_yz3d": [vp],
         "wpic_sort_bucket3d": [vp],
         "wpic_cg_iters3d": [vp, np.ctypeslib.ndpointer(np.int32)],
+        "wpic_set_cg_tol3d": [vp, C.c_double],
         "wpic_step3d": [vp, C.c_int],
         "wpic_energy3d": [vp, np.ctypeslib.ndpointer(np.float64)],
         "wpic_gauss3d": [vp, d],

@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) void k_cg3_fin0(Cg3Args a) {
     t = part_sum3(a, l, 0, sh);
   if (threadIdx.x == 0) {
     a.sc[l].sum_g = t;
-    a.sc[l].eps = sqrt(t) * 1e-6;
+    a.sc[l].eps = sqrt(t) * a.g.cg_err;
   }
 }
 

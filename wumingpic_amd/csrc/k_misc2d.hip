@@ -509,7 +509,10 @@ __global__ void k_ic_shock(IcShockArgs a) {
     a.B.ux[d] = ux;
     a.B.uy[d] = uy;
     a.B.uz[d] = uz;
-    a.B.id[d] = (uint64_t)(-(t + 1));
+    // set_particle_ids (app.f90:500-524): the id is -pid, made positive
+    // (a tracer written by io__orb) for x in [xrs, xre] = [450, 500]
+    const bool tracer = x >= 450.0 && x <= 500.0;
+    a.B.id[d] = tracer ? (uint64_t)(t + 1) : (uint64_t)(-(t + 1));
   }
 }
 

@@ -375,7 +375,12 @@ __device__ inline void push_reflect(const Grid2D& g, const double (*sf)[SFW * SF
     boris_push(sf, li, lj, i, j, g.d_delx, g.delt, g.c, unit_c, kc[0], kc[1], kc[2], x, y, ux, uy, uz, nxp, nyp, g3,
                g4, g5, gmv);
   reflect_x<BC>(g, nxp, g3, g4, g5);
-
+  // the injection edge sends ux to 2 u0 - ux, which changes gamma: vz of the
+  // deposit comes from the momenta after the boundary (field.f90:270-272)
+  if (BC == WPIC_BC_SHOCK) {
+    const double u2 = +g3 * g3 + g4 * g4 + g5 * g5;
+    gmv = 1.0 / sqrt(1.0 + (unit_c ? u2 : u2 / (g.c * g.c)));
+  }
 }
 
 // Fast mode, push and deposit in one pass over each particle.  The push is the

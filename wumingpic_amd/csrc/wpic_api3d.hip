@@ -447,6 +447,7 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   g.f4 = 6e0 + t * t;
   g.f5 = t * t;
   g.pi4dt = 4e0 * pi * p->delt;
+  g.cg_err = 1e-6;
   // (nxge-nxgs+1)*delx etc. (boundary_periodic.f90:89-91,150-163)
   g.spanx = (double)nx * p->delx;
   g.spany = (double)ny * p->delx;
@@ -726,6 +727,13 @@ int wpic_sort_bucket3d(wpic_ctx3d* c) {
 int wpic_cg_iters3d(wpic_ctx3d* c, int iters[3]) {
   if (!c || !iters) return set_error(WPIC_EINVAL, "null argument");
   for (int l = 0; l < 3; l++) iters[l] = c->cg_iters[l];
+  return 0;
+}
+
+int wpic_set_cg_tol3d(wpic_ctx3d* c, double err) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  if (!(err > 0.0 && err < 1.0)) return set_error(WPIC_EINVAL, "the CG tolerance must lie in (0, 1)");
+  c->g.cg_err = err;
   return 0;
 }
 

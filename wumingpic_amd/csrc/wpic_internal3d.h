@@ -19,6 +19,7 @@ struct Grid3D {
   double spanx, spany, spanz;        // (nxge-nxgs+1)*delx etc., round to nearest
   double q[WPIC_MAXSP], r[WPIC_MAXSP];
   long long ncap;
+  double cg_err;  // CG relative tolerance (3d/common/field.f90:430: 1e-6)
 };
 
 __host__ __device__ inline size_t fidx3(const Grid3D& g, int i, int j, int k) {

@@ -136,7 +136,10 @@ class Pic2D:
 
     def inject(self, seed, it, nl, v0, v_thi, v_the, l_damp, b0, theta_bn, phi_bn):
         """the shock app's inject (2d/proj/shock/app.f90:697-856); nl[r] particles per species join row r"""
-        check(self._L.wpic_inject2d(self._h, int(seed), int(it), np.ascontiguousarray(nl, dtype=np.int32),
+        nl = np.ascontiguousarray(nl, dtype=np.int32)
+        if nl.shape != (self.ny,):
+            raise ValueError(f"inject needs one count per row: nl of shape ({self.ny},), got {nl.shape}")
+        check(self._L.wpic_inject2d(self._h, int(seed), int(it), nl,
                                     float(v0), float(v_thi), float(v_the), float(l_damp), float(b0),
                                     float(theta_bn), float(phi_bn)))
 

@@ -145,6 +145,10 @@ class Pic3D:
         check(self._L.wpic_cg_iters3d(self._h, it))
         return tuple(int(v) for v in it)
 
+    def set_cg_tol(self, err):
+        """the CG's relative stopping tolerance (1e-6 as in field.f90 by default)"""
+        check(self._L.wpic_set_cg_tol3d(self._h, float(err)))
+
     def step(self, nsteps=1):
         """nsteps of the loop body of 3d/proj/weibel/app.f90:103-108"""
         check(self._L.wpic_step3d(self._h, int(nsteps)))
