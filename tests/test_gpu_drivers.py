@@ -366,7 +366,7 @@ def test_shock_tracers_in_the_initial_state_and_orb_files():
     """set_particle_ids (2d/proj/shock/app.f90:500-524): the ids of particles
     starting at 450 <= x <= 500 are positive, so the orbit files carry them
     (io__orb writes positive ids only)"""
-    nx, ny, n0, steps = 520, 2, 1, 2
+    nx, ny, n0, steps = 520, 4, 1, 2
     par = {"num_process": 1, "n_ppc": n0, "n_x": nx, "n_x_ini": nx - 1, "n_y": ny, "u_inject": 0.3,
            "mass_ratio": 25.0, "sigma_e": 0.1, "omega_pe": 0.1, "v_the": 0.05, "v_thi": 0.01, "theta_bn": 60.0,
            "phi_bn": 90.0, "l_damp_ini": 6.0, "seed": 5}

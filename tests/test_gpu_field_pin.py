@@ -206,6 +206,8 @@ def test_memory_over_paths():
         big[0, 0] += 1  # one particle more than the species capacity
         up2 = np.zeros((6, p.np, p.ny, 2), order="F")
         up2[:, :, :, :] = up
-        up2[:, int(np2[0, 0]), 0, 0] = up[:, 0, 0, 0]
+        up2[:, int(np2[0, 0]), 0, 0] = up[:, int(np2[0, 0]) - 1, 0, 0]  # a second copy in the row's last cell
+        cc2 = cc.copy(order="F")
+        cc2[-1, 0, 0] += 1  # keep cumcnt(nxe+1) = np2: the capacity check is the one that fires
         with pytest.raises(WpicError, match=r"memory over \(np2 > np\)"):
-            sim.upload_up(up2, big, cc)
+            sim.upload_up(up2, big, cc2)

@@ -54,6 +54,10 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/amdflang")
 FSRC = ["wpic_capi.f90", "wpic_shim_state.f90", "shim_modules.f90"]
 FDRIVER = "shim_step2d_driver.f90"
 FADJ = "adj_driver.f90"
+# the 3d/common modules repeat the 2D module names: their own module
+# directory, archive and driver
+FSRC3D = ["wpic_capi3d.f90", "wpic_shim_state3d.f90", "shim_modules3d.f90"]
+FDRIVER3D = "shim_step3d_driver.f90"
 
 
 def build_fortran():
@@ -89,6 +93,32 @@ def build_fortran():
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link of the Fortran adjoint driver failed:\n{r.stdout}\n{r.stderr}")
+    build_fortran3d()
+    return exe
+
+
+def build_fortran3d():
+    """The 3D shim modules (3d/common's module names) as
+    fortran/libwpic_fortran3d.a, and the 3D app-loop driver."""
+    mod = os.path.join(OBJ, "fmod3d")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for f in FSRC3D + [FDRIVER3D]:
+        obj = os.path.join(mod, f + ".o")
+        cmd = [FLANG, "-O2", "-fPIC", "-module-dir", mod, "-I", mod, "-c", os.path.join(FORT, f), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {f}:\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    archive = os.path.join(FORT, "libwpic_fortran3d.a")
+    if os.path.exists(archive):
+        os.remove(archive)
+    subprocess.run(["ar", "rcs", archive, *objs[:-1]], check=True)
+    exe = os.path.join(FORT, "shim_step3d_driver")
+    cmd = [FLANG, "-o", exe, objs[-1], archive, "-L" + HERE, "-lwpic", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link of the 3D Fortran driver failed:\n{r.stdout}\n{r.stderr}")
     return exe
 
 
