@@ -1,6 +1,6 @@
 """Benchmark of the MI355X PIC step loop.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload weibel1024]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--start S] [--workload weibel1024]
 
 Workload (default): 2D Weibel beam instability, 1024 x 1024 periodic cells,
 64 particles per cell per species (134,217,728 particles), mass ratio 25,
@@ -9,15 +9,24 @@ spread, ions 0.01; FP64 path.  One step = push + deposit + implicit field
 solve + particle boundaries + row exchange + sort, fused on the device.
 Inputs are resident in HBM before the timed region.
 
+The run advances S steps (default 880) untimed, then W warmup steps, then
+times exactly K steps, so with the defaults of the driver (K = 20, W = 5) the
+timed window is steps 885-905 of the workload's 1000: late in the
+instability, where the plasma is hotter and more particles change cell than
+at the start.
+
 Prints one JSON line: particle-pushes/s for the whole job, the roofline of the
-dominant kernel (push+deposit, HIP-event timed on the library's own stream)
-and a CPU baseline: the oracle (an OpenMP C restatement of the reference step)
-timed on this host's cores on a bounded sample of the same workload.  The
-"mixed" object is the mixed-precision form (int32 cell + FP32 offsets, FP32
-push and deposit, FP64 field solve) run from the same seeded state for the
-same steps, with its speed-up and its field and energy error against the FP64
-run.  The "traffic" of the roofline comes from the rocprofv3 FETCH_SIZE /
-WRITE_SIZE summary committed under profiles/ for this kernel, when present.
+dominant kernel (push+deposit, HIP-event timed on the library's own stream in
+the timed region) and a CPU baseline: the oracle (an OpenMP C restatement of
+the reference step) timed on this host's cores on a bounded sample of the same
+workload.  After the timed region the same number of steps runs again without
+any HIP event (ms_per_step_without_events) and once more with the events of
+every phase (phase_ms_per_step).  The "mixed" object is the mixed-precision
+form (int32 cell + FP32 offsets, FP32 push and deposit, FP64 field solve) run
+from the same seeded state through the same steps, with its speed-up and its
+field, moment and energy error against the FP64 run at the same step.  The
+"traffic" of the roofline comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+summary committed under profiles/ for this kernel, when present.
 """
 import argparse
 import json
@@ -65,8 +74,10 @@ MOM3D_BYTES = 52
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--start", type=int, default=880,
+                    help="untimed steps before the warmup (the timed window starts at step start + warmup)")
     ap.add_argument("--workload", default="weibel1024", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="fast", choices=["fast", "parity"])
     ap.add_argument("--cpu-rows", type=int, default=128, help="rows of the CPU baseline sample")
@@ -134,38 +145,73 @@ def measured_traffic(kernel):
     return t.get("traffic_bytes_per_launch")
 
 
-def timed_run(sim, warmup, steps):
+def timed_run(sim, start, warmup, steps):
+    """advance `start` + `warmup` steps, time exactly `steps` steps (HIP events
+    of the library on: the dominant kernel's launch times), then run `steps`
+    more without events and `steps` more with the events of every phase.
+    Returns (timed seconds, seconds of the event-free window, per-phase ms per
+    step, push_deposit (ms, launches) of the timed region)."""
     import torch
-    sim.step(warmup)
+    sim.step(start + warmup)
     torch.cuda.synchronize()
     sim.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.step(steps)
     torch.cuda.synchronize()
-    return time.perf_counter() - t0
+    elapsed = time.perf_counter() - t0
+    pd = sim.timing("push_deposit")
+    sim.set_timing(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    torch.cuda.synchronize()
+    plain = time.perf_counter() - t0
+    sim.set_timing(True)
+    sim.step(steps)
+    torch.cuda.synchronize()
+    phases = {name: sim.timing(name)[0] / steps
+              for name in ("push_deposit", "push", "deposit", "field", "cg", "exchange", "sort")
+              if sim.timing(name)[1] > 0}
+    field = sim.timing("field")
+    sim.set_timing(False)
+    return elapsed, plain, phases, pd, field
 
 
 def moments_run(sim, npart, nbytes, reps=5):
     """the output-cadence moments (mom_calc__accl + __nvt + bc__mom fused) on
-    the state the timed steps left, outside the timed region"""
+    the state the steps left, outside the timed region; returns the timing and
+    the moments"""
     import torch
+    sim.set_timing(True)
     for _ in range(reps):
         sim.moments()
     torch.cuda.synchronize()
     ms, n = sim.timing("moments")
+    sim.set_timing(False)
     avg = ms / max(n, 1)
     return {"ms": avg, "launches": n, "bytes_per_particle": nbytes,
             "achieved_GBs": nbytes * npart / (avg / 1e3) / 1e9,
-            "what": "mom_calc__accl, mom_calc__nvt and bc__mom in one pass (kick fused into the deposit)"}
+            "what": "mom_calc__accl, mom_calc__nvt and bc__mom in one pass (kick fused into the deposit)"}, \
+        sim.download_mom()
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def mixed_run(w, a, npart, elapsed64, uf64, energy64):
-    """the mixed-precision form over the same steps from the same seed"""
+MOMENT_NAMES = ("N", "Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz")
+FIELD_NAMES = ("Bx", "By", "Bz", "Ex", "Ey", "Ez")
+
+
+def rel_err(a, ref):
+    scale = float(abs(ref).max())
+    return float(abs(a - ref).max() / scale) if scale > 0 else None
+
+
+def mixed_run(w, a, npart, ref):
+    """the mixed-precision form through the same steps from the same seed;
+    its errors against the FP64 run at the same step"""
     import numpy as np
 
     from wumingpic_amd import MODE_MIXED
@@ -179,35 +225,40 @@ def mixed_run(w, a, npart, elapsed64, uf64, energy64):
         from wumingpic_amd.pic2d import weibel_beams_2d
         sim = weibel_beams_2d(w["nx"], w["ny"], w["n_ppc"], mode=MODE_MIXED)
         nbytes = PUSH_DEPOSIT_BYTES_MIXED
-    elapsed = timed_run(sim, a.warmup, a.steps)
-    log(f"mixed form: {a.warmup} warmup + {a.steps} timed steps: {elapsed:.3f} s")
-    pd_ms, pd_n = sim.timing("push_deposit")
-    phases = {name: sim.timing(name)[0] / a.steps for name in ("push_deposit", "field", "cg", "exchange", "sort")}
-    sim.set_timing(False)
+    elapsed, plain, phases, pd, _ = timed_run(sim, a.start, a.warmup, a.steps)
+    log(f"mixed form: {a.start} + {a.warmup} untimed + {a.steps} timed steps: {elapsed:.3f} s")
     energy = sim.energy()
     uf = sim.download_fields()
+    mom = None
+    if "nz" not in w:
+        from wumingpic_amd import WpicError
+        try:
+            sim.moments()
+            mom = sim.download_mom()
+        except WpicError as e:  # a library without the mixed-form moments
+            log(f"mixed-form moments unavailable: {e}")
     sim.close()
-    names = ("Bx", "By", "Bz", "Ex", "Ey", "Ez")
-    err = {}
-    for c, nm in enumerate(names):
-        ref = uf64[c]
-        scale = float(np.abs(ref).max())
-        err[nm] = float(np.abs(uf[c] - ref).max() / scale) if scale > 0 else None
-    pd_avg_s = pd_ms / max(pd_n, 1) / 1e3
-    return {
+    pd_avg_s = pd[0] / max(pd[1], 1) / 1e3
+    out = {
         "value": npart * a.steps / elapsed,
         "unit": "particle-pushes/s",
         "ms_per_step": elapsed / a.steps * 1e3,
-        "speedup_vs_f64": elapsed64 / elapsed,
+        "ms_per_step_without_events": plain / a.steps * 1e3,
+        "speedup_vs_f64": ref["elapsed"] / elapsed,
+        "speedup_vs_f64_without_events": ref["plain"] / plain,
         "dtype": "int32 cell + f32 offsets/momenta; f32 push/deposit; f64 field solve and current sum",
         "push_deposit": {"avg_launch_ms": pd_avg_s * 1e3, "bytes_per_particle": nbytes,
                          "achieved_GBs": nbytes * npart / pd_avg_s / 1e9},
         "phase_ms_per_step": phases,
-        "steps_compared": a.warmup + a.steps,
-        "field_rel_err_vs_f64": err,
-        "energy_rel_err_vs_f64": {k: float(abs(energy[i] - energy64[i]) / abs(energy64[i]))
+        "compared_at_step": ref["step"],
+        "field_rel_err_vs_f64": {nm: rel_err(uf[c], ref["uf"][c]) for c, nm in enumerate(FIELD_NAMES)},
+        "energy_rel_err_vs_f64": {k: float(abs(energy[i] - ref["energy"][i]) / abs(ref["energy"][i]))
                                   for i, k in enumerate(("kinetic", "E", "B"))},
     }
+    if mom is not None and ref.get("mom") is not None:
+        out["moment_rel_err_vs_f64"] = {f"{nm}[{sp}]": rel_err(mom[k, :, :, sp], ref["mom"][k, :, :, sp])
+                                        for sp in range(mom.shape[3]) for k, nm in enumerate(MOMENT_NAMES)}
+    return out
 
 
 def main3d(a, w):
@@ -220,36 +271,35 @@ def main3d(a, w):
                           capacity=int(w["n_ppc"] * w["nx"] * w["ny"] * w["nz"] * 1.05))
     npart = int(sim.counts().sum())
     log(f"initial state ready: {w['name']}, {npart} particles")
-    elapsed = timed_run(sim, a.warmup, a.steps)
-    log(f"{a.warmup} warmup + {a.steps} timed steps: {elapsed:.3f} s")
-    names = ("push_deposit", "push", "deposit", "field", "cg", "exchange", "sort")
-    ph = {name: sim.timing(name) for name in names}
-    ph = {k: v for k, v in ph.items() if v[1] > 0}
+    elapsed, plain, phases, pd, _ = timed_run(sim, a.start, a.warmup, a.steps)
+    log(f"{a.start} + {a.warmup} untimed + {a.steps} timed steps: {elapsed:.3f} s")
     cg = sim.cg_iters()
-    moments = moments_run(sim, npart, MOM3D_BYTES)
-    sim.set_timing(False)
+    moments, _ = moments_run(sim, npart, MOM3D_BYTES)
     energy64 = sim.energy()
     uf64 = sim.download_fields() if not a.no_mixed else None
     sim.close()
     rates = {}
     for name, nbytes in (("push_deposit", PUSH3D_BYTES), ("push", PUSH3D_BYTES), ("deposit", DEPOSIT3D_BYTES)):
-        if name in ph:
-            sec = ph[name][0] / ph[name][1] / 1e3
-            rates[name + "_GBs"] = nbytes * npart / sec / 1e9
+        if name in phases:
+            rates[name + "_GBs"] = nbytes * npart / (phases[name] / 1e3) / 1e9
     out = {
         "metric": "particle-pushes/s (push+deposit+sort) per GPU, 3D",
         "value": npart * a.steps / elapsed, "unit": "particle-pushes/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3,
+        "ms_per_step_without_events": plain / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded counter-based initial state, generated on the device)",
-        "config": {"workload": w["name"], "particles": npart, "mode": a.mode},
-        "phase_ms_per_step": {k: v[0] / a.steps for k, v in ph.items()},
+        "config": {"workload": w["name"], "particles": npart, "mode": a.mode,
+                   "timed_steps": [a.start + a.warmup, a.start + a.warmup + a.steps]},
+        "phase_ms_per_step": phases,
         **rates,
         "cg_iters_last_step": list(cg),
         "moments": moments,
     }
     if not a.no_mixed:
-        out["mixed"] = mixed_run(w, a, npart, elapsed, uf64, energy64)
+        ref = {"elapsed": elapsed, "plain": plain, "uf": uf64, "energy": energy64,
+               "step": a.start + a.warmup + 3 * a.steps}
+        out["mixed"] = mixed_run(w, a, npart, ref)
     print(json.dumps(out))
 
 
@@ -262,7 +312,7 @@ def main():
         print(f"bench.py: this build runs on exactly one GPU (asked for {a.gpus}); "
               "multi-GPU sharding is designed but not built", file=sys.stderr)
         sys.exit(2)
-    import torch
+    import torch  # noqa: F401
 
     from wumingpic_amd import MODE_FAST, MODE_PARITY
     from wumingpic_amd.pic2d import weibel_beams_2d
@@ -277,20 +327,18 @@ def main():
     npart = int(sim.counts().sum())
     ncell = nx * ny
     log(f"initial state ready: {w['name']}, {npart} particles")
-    elapsed = timed_run(sim, a.warmup, a.steps)
-    log(f"{a.warmup} warmup + {a.steps} timed steps: {elapsed:.3f} s")
-    pd_ms, pd_n = sim.timing("push_deposit")
-    fld_ms, fld_n = sim.timing("field")
-    phases = {name: sim.timing(name)[0] / a.steps for name in ("push_deposit", "field", "cg", "exchange", "sort")}
+    elapsed, plain, phases, (pd_ms, pd_n), (fld_ms, fld_n) = timed_run(sim, a.start, a.warmup, a.steps)
+    log(f"{a.start} + {a.warmup} untimed + {a.steps} timed steps: {elapsed:.3f} s")
     cg_iters = sim.cg_iters()
-    moments = moments_run(sim, npart, MOM_BYTES)
-    sim.set_timing(False)
+    moments, mom64 = moments_run(sim, npart, MOM_BYTES)
     energy = sim.energy()
     uf64 = sim.download_fields()
     sim.close()
     mixed = None
     if not a.no_mixed:
-        mixed = mixed_run(w, a, npart, elapsed, uf64, energy)
+        ref = {"elapsed": elapsed, "plain": plain, "uf": uf64, "energy": energy, "mom": mom64,
+               "step": a.start + a.warmup + 3 * a.steps}
+        mixed = mixed_run(w, a, npart, ref)
 
     pd_avg_s = pd_ms / max(pd_n, 1) / 1e3
     achieved = PUSH_DEPOSIT_BYTES * npart / pd_avg_s / 1e9
@@ -302,13 +350,15 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": elapsed / a.steps * 1e3,
+        "ms_per_step_without_events": plain / a.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded counter-based initial state, generated on the device)",
         "config": {"workload": w["name"], "nx": nx, "ny": ny, "particles_per_cell_per_species": n_ppc,
-                   "particles": npart, "mode": a.mode, "parallelism": "single GPU"},
+                   "particles": npart, "mode": a.mode, "parallelism": "single GPU",
+                   "timed_steps": [a.start + a.warmup, a.start + a.warmup + a.steps]},
         "roofline": {"kernel": "push_deposit", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(PUSH_DEPOSIT_KERNEL),

@@ -184,8 +184,11 @@ int wpic_counts2d(wpic_ctx *ctx, long long *counts);
  *                   (2d/proj/weibel/app.f90:121-123) in one pass over up
  *                   (FAST: the kick fused into the deposit); gp is left
  *                   undefined, as mom_calc__accl leaves it overwritten.
+ *                   WPIC_MODE_MIXED: the same sequence in FP32 from the mixed
+ *                   record (FP32 LDS node tiles, FP64 global sum); the
+ *                   particles are not touched.
  * Particles whose lower node falls outside their row or the grid give
- * WPIC_EDOMAIN.  FP64 modes only. */
+ * WPIC_EDOMAIN.  The single routines accl, nvt and bc need an FP64 mode. */
 int wpic_mom_accl2d(wpic_ctx *ctx);
 int wpic_mom_nvt2d(wpic_ctx *ctx);
 int wpic_bc_mom2d(wpic_ctx *ctx);

@@ -104,3 +104,54 @@ def test_mixed_rejects_single_routine_entry_points():
             mix.particle_solv()
         with pytest.raises(WpicError, match="fixed when the context is created"):
             mix.set_mode(MODE_FAST)
+
+
+MOMENTS = ("N", "Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz")
+
+
+def _moment_errors(mom, ref):
+    """max |mom - ref| / max |ref| per moment and species"""
+    err = {}
+    for s in range(ref.shape[3]):
+        for k, name in enumerate(MOMENTS):
+            scale = np.abs(ref[k, :, :, s]).max()
+            err[f"{name}[{s}]"] = float(np.abs(mom[k, :, :, s] - ref[k, :, :, s]).max() / scale)
+    return err
+
+
+@pytest.mark.parametrize("nsteps,tol", [(0, 2e-6), (10, 1e-4)])
+def test_mixed_fields_and_moments_against_the_fp64_oracle(nsteps, tol):
+    """The mixed form from its own initial state (exactly representable in
+    FP64) against the FP64 oracle from the same state: after nsteps the six
+    fields and the seven moments of each species (the output sequence
+    mom_calc__accl, mom_calc__nvt, bc__mom) agree to tol of each quantity's
+    maximum.  The errors are printed."""
+    from wumingpic_amd import MODE_MIXED
+    from wumingpic_amd.pic2d import Pic2D, weibel_charges
+    nx, ny, n_ppc = 32, 32, 16
+    q, r = weibel_charges(n_ppc, 0.1, 25.0)
+    p = po.make_params(nx, ny, np_cap=3 * n_ppc * nx, delt=0.5, q=q, r=r)
+    with Pic2D(nx, ny, q, r, delt=0.5, mode=MODE_MIXED, capacity=2 * n_ppc * nx * ny) as mix:
+        mix.ic_beams(20261202, n_ppc, 0.5, 0.1, 0.01, 0.0)
+        up, np2, cc = mix.download_up(p.np)
+        uf = mix.download_fields()
+        mix.step(nsteps)
+        mix.moments()
+        mom_mix = mix.download_mom()
+        uf_mix = mix.download_fields()
+    o = po.Oracle2D(p)
+    gp = np.zeros_like(up, order="F")
+    if nsteps:
+        o.step(up, gp, uf, cc, np2, nsteps)
+    o.mom_accl(gp, up, uf, cc)
+    mom = o.mom_nvt(gp, np2)
+    o.bc_mom(mom)
+    err = _moment_errors(mom_mix, mom)
+    ferr = {}
+    for c, name in enumerate(("Bx", "By", "Bz", "Ex", "Ey", "Ez")):
+        scale = np.abs(uf[c]).max()
+        if scale > 0:
+            ferr[name] = float(np.abs(uf_mix[c] - uf[c]).max() / scale)
+    print(f"mixed vs FP64 oracle after {nsteps} steps: moments {err} fields {ferr}")
+    assert max(err.values()) <= tol, err
+    assert not ferr or max(ferr.values()) <= max(tol, 1e-4), ferr

@@ -785,10 +785,10 @@ hipError_t launch_gauss(wpic_ctx* c, double* G, double* rho) {
   else
     hipLaunchKernelGGL(k_rho<false>, dim3(c->g.nsp * c->g.ny), dim3(256), 0, c->st, c->g, c->A, c->MA, c->rowbaseA,
                        c->np2A, c->aperm, rho);
-  if (c->g.bc == WPIC_BC_PERIODIC) {
-    hipLaunchKernelGGL(k_rho_fold_y, dim3((c->g.nx + 2 + 255) / 256), dim3(256), 0, c->st, c->g, rho);
+  // y is periodic under every boundary kind; x only in the periodic box
+  hipLaunchKernelGGL(k_rho_fold_y, dim3((c->g.nx + 2 + 255) / 256), dim3(256), 0, c->st, c->g, rho);
+  if (c->g.bc == WPIC_BC_PERIODIC)
     hipLaunchKernelGGL(k_rho_fold_x, dim3((c->g.ny + 255) / 256), dim3(256), 0, c->st, c->g, rho);
-  }
   hipLaunchKernelGGL(k_gauss, dim3((c->g.nx * c->g.ny + 255) / 256), dim3(256), 0, c->st, c->g, c->uf, rho, G);
   return hipGetLastError();
 }

@@ -32,10 +32,6 @@ struct MomArgs {
   double fac1[WPIC_MAXSP], txxx[WPIC_MAXSP], fac2[WPIC_MAXSP];  // mom_calc.f90:90-92 with delt/2
 };
 
-__host__ __device__ inline size_t midx(const Grid2D& g, int i, int j, int s) {
-  return (size_t)7 * ((size_t)(i - (g.nxgs - 1)) + (size_t)(g.nx + 2) * ((size_t)(j - (g.nys - 1)) + (size_t)(g.ny + 2) * s));
-}
-
 // stage the tile's cell-centred fields plus a one-cell halo (mom_calc.f90:68-81)
 __device__ inline void stage_fields(const Grid2D& g, const double* uf, double (*sf)[SFW * SFH], int i0, int j0) {
   for (int k = threadIdx.x; k < 6 * SFW * SFH; k += NT) {

@@ -81,6 +81,21 @@ __device__ inline void esirkepov_shapes(const Grid2D& g, double x, double y, dou
   }
 }
 
+// The shapes of a particle that stays in cell (i, j): inc = 0 makes smo =
+// (0, 1, 0), so ds(-2) = ds(2) = 0 and ds(k) = s1(k) - s0(k) for k = -1..1,
+// the values esirkepov_shapes gives there (x * 1 and + 0 are exact), with
+// three of its ten live at a time
+__device__ inline void stayer_shapes(double d_delx, double x, double y, double nxp, double nyp, int i, int j,
+                                     double s0x[3], double s0y[3], double dsx[3], double dsy[3]) {
+  shape3(x * d_delx - 0.5 - i, s0x[0], s0x[1], s0x[2]);
+  shape3(y * d_delx - 0.5 - j, s0y[0], s0y[1], s0y[2]);
+  double t0, t1, t2;
+  shape3(nxp * d_delx - 0.5 - i, t0, t1, t2);
+  dsx[0] = t0 - s0x[0]; dsx[1] = t1 - s0x[1]; dsx[2] = t2 - s0x[2];
+  shape3(nyp * d_delx - 0.5 - j, t0, t1, t2);
+  dsy[0] = t0 - s0y[0]; dsy[1] = t1 - s0y[1]; dsy[2] = t2 - s0y[2];
+}
+
 // One cell's N accumulators: parity mode stores them to the cell's block at
 // offset OFF; fast mode reduces them over the G lanes of the group and adds
 // them to the tile accumulator, spreading the atomics over the lanes.
@@ -396,21 +411,34 @@ __device__ inline void push_reflect(const Grid2D& g, const double (*sf)[SFW * SF
 // last up to rounding), so its current lives on 6 + 6 + 9 nodes: each lane
 // accumulates those 21 values in registers over the particles of the cell it
 // handles and the group reduces them once per cell.  A particle that changes
-// cell (about one in eight here) is queued in LDS with what its current needs
-// (old and unwrapped new position, q vz); after each round of cells (one per
-// group) all lanes add the queued particles' full blocks (deposit2_full), and
-// beyond the queue's capacity a particle adds its block at once.  With FUSED
-// the wraps and the exchange code follow in the same pass; otherwise gp keeps
-// the unwrapped positions as particle__solv leaves them.
+// cell (about one in eight here) is queued in its wave's LDS queue with what
+// its current needs (old and unwrapped new position, q vz); after each round
+// of the wave's cells all its lanes add the queued particles' full blocks
+// (deposit2_full), and beyond the queue's capacity a particle adds its block at
+// once.  The queues are per wave, so no barrier stops the workgroup between
+// the staging and the final flush.
+//
+// Work mapping: wave w walks row w of the 32 x 4 tile, four x-adjacent cells
+// per round (one per 16-lane group), so a cell's x-neighbours share its wave
+// and its y-neighbours run in the sibling waves at about the same time: the
+// records of particles that crossed a cell or row boundary are read while the
+// lines they share with their old neighbours are still in L2.  Each lane walks
+// its cell's particles (all species as one list) with a two-deep software
+// pipeline: the permutation entry of element t + 2G and the record of element
+// t + G are in flight while element t is pushed.  With FUSED the wraps and the
+// exchange code follow in the same pass; otherwise gp keeps the unwrapped
+// positions as particle__solv leaves them.
+constexpr int QW = 160;  // queued cell-changing particles per wave and round
 template <int G, bool FUSED, int BC, bool VAY>
 __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
+  constexpr int NW = NT / 64;
+  static_assert(NW == TY && 64 / G * (TX / (64 / G)) == TX, "one wave per tile row, whole rounds");
   __shared__ double sf[6][SFW * SFH];
   __shared__ double acc[3][ACW * ACH];
   __shared__ double spc[WPIC_MAXSP][5];  // fac1, txxx, fac2, qk, q
-  constexpr int NXQR = 512;        // queued cell-changing particles per round
-  __shared__ double xqd[NXQR][5];  // x, y, unwrapped new x, y, q vz
-  __shared__ int xqc[NXQR];        // cell in the tile | species << 8
-  __shared__ int nxq[2];           // this round's and the next round's counts
+  __shared__ double xqd[NW][QW][5];      // x, y, unwrapped new x, y, q vz
+  __shared__ int xqc[NW][QW];            // cell in the tile | species << 8
+  __shared__ int nxq[NW];
 
   const Grid2D& g = a.g;
   const int ntiles = a.ntx * a.nty;
@@ -426,7 +454,7 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
     if (i <= g.nxge + 1 && j <= g.nye + 1) sf[c][rem] = cell_field(g, a.uf, c, i, j);
   }
   for (int k = tid; k < 3 * ACW * ACH; k += NT) (&acc[0][0])[k] = 0.0;
-  if (tid == 0) nxq[0] = nxq[1] = 0;
+  if (tid < NW) nxq[tid] = 0;
   // per-species constants (particle.f90:92-94, field.f90:274)
 #pragma unroll
   for (int s = 0; s < WPIC_MAXSP; s++) {
@@ -442,16 +470,18 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
   }
   __syncthreads();
 
-  const int grp = tid / G, lg = tid % G;
-  constexpr int NGRP = NT / G;
+  constexpr int RX = 64 / G;  // cells per wave round
+  const int w = tid >> 6, lane = tid & 63;
+  const int lg = lane % G;
+  const int lj = w, j = j0 + lj;
   const bool unit_c = g.c * g.c == 1.0;
+  int* const nq = &nxq[w];
+  double(*const qd)[5] = xqd[w];
+  int* const qc = xqc[w];
 
-  // rounds of NGRP cells (one per group), each followed by the queue's drain
-  for (int rnd = 0; rnd < (TX * TY) / NGRP; rnd++) {
-    int* const nq_r = &nxq[rnd & 1];
-    const int cc = rnd * NGRP + grp;
-    const int li = cc % TX, lj = cc / TX;
-    const int i = i0 + li, j = j0 + lj;
+  for (int rnd = 0; rnd < TX / RX; rnd++) {
+    const int li = rnd * RX + lane / G;
+    const int i = i0 + li;
     if (i <= g.nxge && j <= g.nye) {  // uniform within a group
     // stayers: sx[m][u] Jx at x face m, y node u - 1; sy[m][u] Jy at y face
     // m, x node u - 1; sz[v][u] Jz at x node u - 1, y node v - 1
@@ -462,75 +492,119 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
       sz[0][u] = sz[1][u] = sz[2][u] = 0.0;
     }
 
-    for (int s = 0; s < g.nsp; s++) {
-      const int rw = s * g.ny + (j - g.nys);
-      const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
-      const int base = a.rowbase[rw];
-      const int pbeg = base + cr[i - g.nxgs], pend = base + cr[i - g.nxgs + 1];
-      const double* kc = spc[s];
-      const double qk = kc[3], q = kc[4];
-      for (int pi = pbeg + lg; pi < pend; pi += G) {
-        const int ia = a.ap ? a.ap[pi] : pi;
-        const double x = a.A.x[ia], y = a.A.y[ia];
-        double nxp, nyp, g3, g4, g5, gmv;
-        push_reflect<FUSED ? BC : WPIC_BC_PERIODIC, VAY>(g, sf, li, lj, i, j, unit_c, kc, x, y, a.A.ux[ia], a.A.uy[ia],
-                                                    a.A.uz[ia], nxp, nyp, g3, g4, g5, gmv);
-        a.B.ux[pi] = g3;
-        a.B.uy[pi] = g4;
-        a.B.uz[pi] = g5;
-        a.B.id[pi] = a.A.id[ia];  // particle.f90:173-177
-
-        double s0x[5], s0y[5], dsx[5], dsy[5];
-        esirkepov_shapes(g, x, y, nxp, nyp, i, j, s0x, s0y, dsx, dsy);
-        const double qvz = q * (g5 * gmv);
-        const int i2 = (int)(nxp * g.d_delx), j2 = (int)(nyp * g.d_delx);
-        if (i2 == i && j2 == j) {
-          const double cx1 = (0.0 - dsx[0]) - dsx[1], cx2 = cx1 - dsx[2];
-          const double cy1 = (0.0 - dsy[0]) - dsy[1], cy2 = cy1 - dsy[2];
+    // the cell's particles over all species as one list
+    CellList cl;
+    int tot = 0;
 #pragma unroll
-          for (int u = 0; u < 3; u++) {
-            const double wy = qk * fma(0.5, dsy[u + 1], s0y[u + 1]);
-            const double wx = qk * fma(0.5, dsx[u + 1], s0x[u + 1]);
-            sx[0][u] = fma(cx1, wy, sx[0][u]);
-            sx[1][u] = fma(cx2, wy, sx[1][u]);
-            sy[0][u] = fma(cy1, wx, sy[0][u]);
-            sy[1][u] = fma(cy2, wx, sy[1][u]);
-          }
-#pragma unroll
-          for (int v = 0; v < 3; v++) {
-            const double A = qvz * fma(0.5, dsy[v + 1], s0y[v + 1]);
-            const double B = qvz * fma(1.0 / 3.0, dsy[v + 1], 0.5 * s0y[v + 1]);
-#pragma unroll
-            for (int u = 0; u < 3; u++) sz[v][u] = fma(s0x[u + 1], A, fma(dsx[u + 1], B, sz[v][u]));
-          }
-        } else {
-          const int slot = wave_enqueue(nq_r);
-          if (slot < NXQR) {
-            xqd[slot][0] = x;
-            xqd[slot][1] = y;
-            xqd[slot][2] = nxp;
-            xqd[slot][3] = nyp;
-            xqd[slot][4] = qvz;
-            xqc[slot] = cc | (s << 8);
-          } else {
-            deposit2_full(acc, li, lj, s0x, s0y, dsx, dsy, qk, qvz);
-          }
-        }
-
-        if (FUSED) {
-          const int code = wrap_code<BC>(g, nxp, nyp, j, a.err);
-          a.code[pi] = code;
-        }
-        a.B.x[pi] = nxp;
-        a.B.y[pi] = nyp;
+    for (int s = 0; s < WPIC_MAXSP; s++) {
+      if (s < g.nsp) {
+        const int rw = s * g.ny + (j - g.nys);
+        const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
+        const int c0 = cr[i - g.nxgs];
+        cl.set(s, a.rowbase[rw] + c0, tot);
+        tot += cr[i - g.nxgs + 1] - c0;
+      } else {
+        cl.set(s, 0, 0x7fffffff);
       }
+    }
+    int ia_nn = 0;
+    double px = 0.0, py = 0.0, pux = 0.0, puy = 0.0, puz = 0.0;
+    uint64_t pid = 0;
+    {
+      int s_, p_;
+      int ia_n = 0;
+      if (lg < tot) {
+        cl.slot(lg, s_, p_);
+        ia_n = a.ap ? a.ap[p_] : p_;
+      }
+      if (lg + G < tot) {
+        cl.slot(lg + G, s_, p_);
+        ia_nn = a.ap ? a.ap[p_] : p_;
+      }
+      if (lg < tot) {
+        px = a.A.x[ia_n]; py = a.A.y[ia_n];
+        pux = a.A.ux[ia_n]; puy = a.A.uy[ia_n]; puz = a.A.uz[ia_n];
+        pid = a.A.id[ia_n];
+      }
+    }
+    for (int t = lg; t < tot; t += G) {
+      int s, pi;
+      cl.slot(t, s, pi);
+      const double x = px, y = py, ux = pux, uy = puy, uz = puz;
+      const uint64_t id = pid;
+      if (t + G < tot) {
+        const int ia = ia_nn;
+        px = a.A.x[ia]; py = a.A.y[ia];
+        pux = a.A.ux[ia]; puy = a.A.uy[ia]; puz = a.A.uz[ia];
+        pid = a.A.id[ia];
+      }
+      if (t + 2 * G < tot) {
+        int s_, p_;
+        cl.slot(t + 2 * G, s_, p_);
+        ia_nn = a.ap ? a.ap[p_] : p_;
+      }
+      const double* kc = spc[s];
+      double nxp, nyp, g3, g4, g5, gmv;
+      push_reflect<FUSED ? BC : WPIC_BC_PERIODIC, VAY>(g, sf, li, lj, i, j, unit_c, kc, x, y, ux, uy, uz, nxp, nyp,
+                                                       g3, g4, g5, gmv);
+      a.B.ux[pi] = g3;
+      a.B.uy[pi] = g4;
+      a.B.uz[pi] = g5;
+      a.B.id[pi] = id;  // particle.f90:173-177
+
+      const double qvz = kc[4] * (g5 * gmv);
+      const int i2 = (int)(nxp * g.d_delx), j2 = (int)(nyp * g.d_delx);
+      if (i2 == i && j2 == j) {
+        const double qk = kc[3];
+        double s0x[3], s0y[3], dsx[3], dsy[3];
+        stayer_shapes(g.d_delx, x, y, nxp, nyp, i, j, s0x, s0y, dsx, dsy);
+        const double cx1 = 0.0 - dsx[0], cx2 = cx1 - dsx[1];
+        const double cy1 = 0.0 - dsy[0], cy2 = cy1 - dsy[1];
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+          const double wy = qk * fma(0.5, dsy[u], s0y[u]);
+          const double wx = qk * fma(0.5, dsx[u], s0x[u]);
+          sx[0][u] = fma(cx1, wy, sx[0][u]);
+          sx[1][u] = fma(cx2, wy, sx[1][u]);
+          sy[0][u] = fma(cy1, wx, sy[0][u]);
+          sy[1][u] = fma(cy2, wx, sy[1][u]);
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+          const double A = qvz * fma(0.5, dsy[v], s0y[v]);
+          const double B = qvz * fma(1.0 / 3.0, dsy[v], 0.5 * s0y[v]);
+#pragma unroll
+          for (int u = 0; u < 3; u++) sz[v][u] = fma(s0x[u], A, fma(dsx[u], B, sz[v][u]));
+        }
+      } else {
+        const int slot = wave_enqueue(nq);
+        if (slot < QW) {
+          qd[slot][0] = x;
+          qd[slot][1] = y;
+          qd[slot][2] = nxp;
+          qd[slot][3] = nyp;
+          qd[slot][4] = qvz;
+          qc[slot] = (lj * TX + li) | (s << 8);
+        } else {
+          double s0x[5], s0y[5], dsx[5], dsy[5];
+          esirkepov_shapes(g, x, y, nxp, nyp, i, j, s0x, s0y, dsx, dsy);
+          deposit2_full(acc, li, lj, s0x, s0y, dsx, dsy, kc[3], qvz);
+        }
+      }
+
+      if (FUSED) {
+        const int code = wrap_code<BC>(g, nxp, nyp, j, a.err);
+        a.code[pi] = code;
+      }
+      a.B.x[pi] = nxp;
+      a.B.y[pi] = nyp;
     }
     // reduce the stayers' 21 values over the group, spread the LDS atomics
 #pragma unroll
     for (int q = 0; q < 21; q++) {
       double v = q < 6 ? sx[q / 3][q % 3] : (q < 12 ? sy[(q - 6) / 3][(q - 6) % 3] : sz[(q - 12) / 3][(q - 12) % 3]);
-#pragma unroll
-      for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, G);
+      static_assert(G == 16, "the group is one DPP row");
+      v = row16_sum(v);
       if ((q % G) == lg && v != 0.0) {
         int comp, xo, yo;
         if (q < 6) {
@@ -544,21 +618,30 @@ __global__ __launch_bounds__(NT, 2) void k_push_deposit_fast(PushArgs a) {
       }
     }
     }  // the cell
-    __syncthreads();
-    // this round's queued particles that changed cell
-    const int nq = *nq_r < NXQR ? *nq_r : NXQR;
-    for (int q = tid; q < nq; q += NT) {
-      const int e = xqc[q];
+    // the wave's queued particles that changed cell: the queue was written by
+    // this wave's own lanes, and LDS operations of one wave complete in order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int n = *nq < QW ? *nq : QW;
+    for (int q = lane; q < n; q += 64) {
+      const int e = qc[q];
       const int qcc = e & 255, s = e >> 8;
       const int qli = qcc % TX, qlj = qcc / TX;
       double s0x[5], s0y[5], dsx[5], dsy[5];
-      esirkepov_shapes(g, xqd[q][0], xqd[q][1], xqd[q][2], xqd[q][3], i0 + qli, j0 + qlj, s0x, s0y, dsx, dsy);
-      deposit2_full(acc, qli, qlj, s0x, s0y, dsx, dsy, spc[s][3], xqd[q][4]);
+      esirkepov_shapes(g, qd[q][0], qd[q][1], qd[q][2], qd[q][3], i0 + qli, j0 + qlj, s0x, s0y, dsx, dsy);
+      deposit2_full(acc, qli, qlj, s0x, s0y, dsx, dsy, spc[s][3], qd[q][4]);
     }
-    if (tid == 0) nxq[(rnd + 1) & 1] = 0;
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) *nq = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 
+  __syncthreads();
   const size_t pl = (size_t)g.fw * g.fh;
   for (int k = tid; k < 3 * ACW * ACH; k += NT) {
     const int comp = k / (ACW * ACH), rem = k - comp * (ACW * ACH);

@@ -528,12 +528,10 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
         const int rw = s * g.ncol + col;
         const int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
         const int c0 = cr[i - g.nxgs], c1 = cr[i - g.nxgs + 1];
-        cl.beg[s] = a.colbase[rw] + c0;
-        cl.cum[s] = ntot;
+        cl.set(s, a.colbase[rw] + c0, ntot);
         ntot += c1 - c0;
       } else {
-        cl.beg[s] = 0;
-        cl.cum[s] = 0x7fffffff;
+        cl.set(s, 0, 0x7fffffff);
       }
     }
     // stayers' current: [m][a][b] with m the face (offset m along the

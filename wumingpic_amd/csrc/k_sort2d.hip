@@ -415,6 +415,129 @@ __global__ __launch_bounds__(SB) void k_rowsort(SortArgs a, int* perm, int nbits
   }
 }
 
+// The fused step's exchange and sort in the fast and mixed modes, in one pass
+// per destination row and without the reference's hole filling: the row's
+// particles are its own stayers (row step 0) in slot order and the arrivals
+// of the rows below and above (the up/down lists of k_rowscan), each wave
+// taking a quarter of the stayer slots followed by a quarter of the
+// arrivals; a stable counting sort by cell of that sequence, written as a
+// permutation.  Each cell receives the particles the reference's exchange and
+// sort__bucket put there (cumcnt and np2 are the reference's), in a fixed
+// order of its own, so only the summation order of the deposit changes.
+__global__ __launch_bounds__(SB) void k_rowsort_fast(SortArgs a, int* perm, int nbits) {
+  extern __shared__ int hist[];  // [SB/64][nx]
+  __shared__ int wsum[SB / 64];
+  __shared__ int carry;
+  const Grid2D& g = a.g;
+  const int nx = g.nx, ny = g.ny;
+  const int rw = blockIdx.x;
+  const int s = rw / ny, r = rw % ny;
+  const int base = a.rowbaseB[rw], n = a.np2B[rw];
+  const int L = a.np2N[rw], nb = a.rowbaseN[rw];
+  const int rp = s * ny + (r == 0 ? ny - 1 : r - 1), rn = s * ny + (r == ny - 1 ? 0 : r + 1);
+  const int* ul = a.ulist + a.rowbaseB[rp];
+  const int nu = a.rowcnt[rp * 4 + 1];
+  const int* dl = a.dlist + a.rowbaseB[rn];
+  const int m = nu + a.rowcnt[rn * 4 + 2];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint64_t lt = lanemask_lt(lane);
+  constexpr int NW = SB / 64;
+  for (int k = tid; k < NW * nx; k += SB) hist[k] = 0;
+  __syncthreads();
+  // this wave's stayer slots [lo, hi) of the row and arrivals [alo, ahi)
+  const int lo = (int)(((long long)n * w) / NW), hi = (int)(((long long)n * (w + 1)) / NW);
+  const int alo = (int)(((long long)m * w) / NW), ahi = (int)(((long long)m * (w + 1)) / NW);
+  const int nsl = hi - lo;  // chunk index space: stayer slots then arrivals
+  const int tot = nsl + (ahi - alo);
+  int* hw = hist + w * nx;
+  // element e of this wave's sequence: its source slot, or -1 for a leaver
+  auto source = [&](int e) -> int {
+    if (e < nsl) {
+      const int p = base + lo + e;
+      return ((a.code[p] & 3) == 1) ? p : -1;
+    }
+    const int t = alo + (e - nsl);
+    return t < nu ? ul[t] : dl[t - nu];
+  };
+  const unsigned ntot = (unsigned)(g.ncap * g.nsp);
+  for (int pass = 0; pass < 2; pass++) {
+    for (int e0 = 0; e0 < tot; e0 += 64 * SU) {
+      int src[SU], key[SU];
+      bool act[SU];
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int e = e0 + u * 64 + lane;
+        src[u] = e < tot ? source(e) : -1;
+        if (src[u] >= 0 && (unsigned)src[u] >= ntot) {
+          if (pass == 1) atomicOr(a.err, ERR_ROW);
+          src[u] = -1;
+        }
+        act[u] = src[u] >= 0;
+      }
+#pragma unroll
+      for (int u = 0; u < SU; u++) key[u] = act[u] ? (a.code[src[u]] >> 2) : 0;
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const uint64_t peers = key_peers(key[u], act[u], nbits);
+        const int rank = __popcll(peers & lt);
+        if (pass == 0) {
+          if (act[u] && rank == 0) hw[key[u]] += __popcll(peers);
+        } else {
+          int pos = 0;
+          if (act[u]) pos = hw[key[u]] + rank;
+          if (act[u] && rank == 0) hw[key[u]] = pos + __popcll(peers);
+          if (act[u]) {
+            if (pos < L)
+              perm[nb + pos] = src[u];
+            else
+              atomicOr(a.err, ERR_ROW);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (pass == 1) break;
+    // exclusive scan over keys of the row totals -> cumcnt; per-wave cursors
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    int* cc = a.cumcnt + (size_t)rw * (nx + 1);
+    for (int k0 = 0; k0 < nx; k0 += SB) {
+      const int key = k0 + tid;
+      int t = 0;
+      if (key < nx)
+        for (int v = 0; v < NW; v++) t += hist[v * nx + key];
+      int v = t;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
+      }
+      if (lane == 63) wsum[w] = v;
+      __syncthreads();
+      int pre = carry;
+      for (int q = 0; q < w; q++) pre += wsum[q];
+      const int excl = pre + v - t;
+      if (key < nx) {
+        cc[key] = excl;
+        int st = excl;
+        for (int q = 0; q < NW; q++) {
+          const int h = hist[q * nx + key];
+          hist[q * nx + key] = st;
+          st += h;
+        }
+      }
+      __syncthreads();
+      if (tid == SB - 1) carry = excl + t;
+      __syncthreads();
+    }
+    if (tid == 0) {
+      cc[nx] = L;
+      if (carry != L) atomicOr(a.err, ERR_ROW);
+    }
+    __syncthreads();
+  }
+}
+
 // sort__bucket on a gp that did not go through bc__particle_y (the restart
 // path, app.f90:351-352): identity permutation, key int(x)
 __global__ __launch_bounds__(SB) void k_codes_identity(SortArgs a) {
@@ -484,6 +607,16 @@ hipError_t launch_rowcount(wpic_ctx* c) {
 hipError_t launch_rowbuild(wpic_ctx* c) {
   SortArgs a = sort_args(c);
   hipLaunchKernelGGL(k_rowbuild, dim3(c->g.nsp * c->g.ny), dim3(SB), 0, c->st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowsort_fast(wpic_ctx* c) {
+  SortArgs a = sort_args(c);
+  int nbits = 0;
+  while ((1 << nbits) < c->g.nx) nbits++;
+  const size_t shm = sizeof(int) * (SB / 64) * (size_t)c->g.nx;
+  int* perm = c->perm_out ? c->perm_out : c->permbuf;
+  hipLaunchKernelGGL(k_rowsort_fast, dim3(c->g.nsp * c->g.ny), dim3(SB), shm, c->st, a, perm, nbits);
   return hipGetLastError();
 }
 

@@ -155,16 +155,20 @@ int ensure_pjbuf(wpic_ctx* c) {
 
 // field__fdtd_i after the deposit: current BCs, RHS, CG, E, BCs, update
 int cg_loop_impl(wpic_ctx* c) {
-  // iterations are launched NB at a time; the three components stop on their
-  // own device flags, the host polls once per batch
+  // iterations are launched a batch at a time; the three components stop on
+  // their own device flags (an iteration past convergence changes nothing),
+  // the host polls once per batch.  The first batch is the last solve's
+  // count plus one, so a step whose count does not grow polls once.
   const int NB = 4;
   int it = 0;
   struct {
     CgScal s[3];
     int err;
   } h;
+  int nb = c->cg_hint > 0 ? c->cg_hint + 1 : NB;
   for (;;) {
-    for (int b = 0; b < NB && it < 100; b++, it++) HIPCHK(launch_cg_iter(c, it));
+    for (int b = 0; b < nb && it < 100; b++, it++) HIPCHK(launch_cg_iter(c, it));
+    nb = NB;
     HIPCHK(hipMemcpyAsync(h.s, c->scal, sizeof(h.s), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(&h.err, c->err, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -172,7 +176,11 @@ int cg_loop_impl(wpic_ctx* c) {
     if (!h.s[0].active && !h.s[1].active && !h.s[2].active) break;
     if (it >= 100) return set_error(WPIC_ECG, "********** stop at cgm after ite_max **********");
   }
-  for (int l = 0; l < 3; l++) c->cg_iters[l] = h.s[l].ite;
+  c->cg_hint = 0;
+  for (int l = 0; l < 3; l++) {
+    c->cg_iters[l] = h.s[l].ite;
+    if (h.s[l].ite > c->cg_hint) c->cg_hint = h.s[l].ite;
+  }
   return 0;
 }
 
@@ -216,12 +224,14 @@ int deposit(wpic_ctx* c, bool push, bool fused) {
   return 0;
 }
 
-// bc__particle_y's exchange on B (rows B -> rows N, permutation R)
-int exchange(wpic_ctx* c, bool ywrap) {
+// bc__particle_y's exchange on B (rows B -> rows N, permutation R); with
+// `lists_only` the row counts and up/down lists alone, for the fused fast
+// sort (k_rowsort_fast builds each new row from them)
+int exchange(wpic_ctx* c, bool ywrap, bool lists_only = false) {
   Timed t(c, "exchange");
   HIPCHK(launch_rowscan(c, ywrap));
   HIPCHK(launch_rowcount(c));
-  HIPCHK(launch_rowbuild(c));
+  if (!lists_only) HIPCHK(launch_rowbuild(c));
   c->h_rows_ok[c->rn] = false;
   c->exch_pending = true;
   return 0;
@@ -232,11 +242,12 @@ int mixed_only_step() {
                    "the mixed-precision form runs the fused loop (wpic_step2d); single routines need an FP64 mode");
 }
 
-// sort__bucket: gather B through R into A, by cell within rows
-int sort_into_a(wpic_ctx* c) {
+// sort__bucket: gather B through R into A, by cell within rows (fast: the
+// fused step's one-pass exchange and sort, see k_rowsort_fast)
+int sort_into_a(wpic_ctx* c, bool fast = false) {
   {
     Timed t(c, "sort");
-    HIPCHK(launch_rowsort(c));
+    HIPCHK(fast ? launch_rowsort_fast(c) : launch_rowsort(c));
   }
   const int old_a = c->ra;
   c->ra = c->rn;
@@ -271,9 +282,12 @@ int step_once(wpic_ctx* c) {
   c->b_aligned = true;
   rc = field_after_deposit(c);
   if (rc) return rc;
-  rc = exchange(c, false);
+  // the parity mode keeps the reference's hole filling and bucket order; the
+  // fast and mixed modes put each cell's particles in an order of their own
+  const bool fast = c->mode != WPIC_MODE_PARITY;
+  rc = exchange(c, false, fast);
   if (rc) return rc;
-  rc = sort_into_a(c);
+  rc = sort_into_a(c, fast);
   if (rc) return rc;
   c->h_rows_ok[c->ra] = false;
   return 0;
@@ -1121,9 +1135,18 @@ int wpic_bc_mom2d(wpic_ctx* c) {
 
 int wpic_moments2d(wpic_ctx* c) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
-  if (c->mixed) return mixed_only_step();
   int rc = ensure_mom(c);
   if (rc) return rc;
+  if (c->mixed) {
+    // the fused output sequence in the mixed form; the state is not touched
+    {
+      Timed t(c, "moments");
+      set_rows(c);
+      HIPCHK(launch_mom_mixed(c));
+      HIPCHK(launch_mom_bc(c));
+    }
+    return check_err(c);
+  }
   {
     Timed t(c, "moments");
     if (c->mode == WPIC_MODE_PARITY) {

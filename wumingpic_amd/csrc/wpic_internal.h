@@ -44,6 +44,12 @@ __host__ __device__ inline int cidx(const Grid2D& g, int i, int j) {
   return (i - g.nxgs) + g.nx * (j - g.nys);
 }
 
+// moments mom(7, nxgs-1:nxge+1, nys-1:nye+1, nsp): the seven moments of a
+// node contiguous, as the reference's array
+__host__ __device__ inline size_t midx(const Grid2D& g, int i, int j, int s) {
+  return (size_t)7 * ((size_t)(i - (g.nxgs - 1)) + (size_t)(g.nx + 2) * ((size_t)(j - (g.nys - 1)) + (size_t)(g.ny + 2) * s));
+}
+
 // a particle buffer: the record and the reference's id (raw 64 bits) travel
 // together
 struct Parts {
@@ -154,22 +160,62 @@ __host__ __device__ inline void shape3(double dh, double& s0, double& s1, double
 
 // The particles of one cell over all species, as one list: element t is slot
 // beg[s] + t - cum[s] of species s, the last s with cum[s] <= t (species
-// beyond nsp carry cum = INT_MAX).
+// beyond nsp carry cum = INT_MAX).  Kept as named scalars and resolved with
+// masks: a chain of selects over an array is turned by the compiler into an
+// indexed table in scratch memory.
 struct CellList {
-  int beg[WPIC_MAXSP], cum[WPIC_MAXSP];
+  int o0, d1, d2, d3, c1, c2, c3;  // slot offset of species 0, its steps to 1..3, their first elements
+  int ob;                          // the offset of the species set last (building d1..d3)
+  __device__ void set(int s, int beg, int cum) {
+    const int o = beg - cum;
+    if (s == 0) { o0 = o; }
+    if (s == 1) { d1 = o - ob; c1 = cum; }
+    if (s == 2) { d2 = o - ob; c2 = cum; }
+    if (s == 3) { d3 = o - ob; c3 = cum; }
+    ob = o;
+  }
   __device__ void slot(int t, int& s, int& p) const {
-    int ss = 0, o = t, b = beg[0];
-#pragma unroll
-    for (int k = 1; k < WPIC_MAXSP; k++)
-      if (t >= cum[k]) {
-        ss = k;
-        o = t - cum[k];
-        b = beg[k];
-      }
-    s = ss;
-    p = b + o;
+    static_assert(WPIC_MAXSP == 4, "CellList holds four species");
+    const int m1 = -(int)(t >= c1), m2 = -(int)(t >= c2), m3 = -(int)(t >= c3);
+    s = -(m1 + m2 + m3);
+    p = t + o0 + (m1 & d1) + (m2 & d2) + (m3 & d3);
   }
 };
+
+// Sum over the 16 lanes of a DPP row (lanes 16r .. 16r+15), every lane of the
+// row receiving it: quad permutes for the partners 1 and 2 apart, then row
+// rotations by 4 and 8.  Cross-lane moves on the VALU (DPP) instead of
+// ds_bpermute through the LDS crossbar.  Each row must be wholly active or
+// wholly inactive (rows never exchange data).
+template <int CTRL>
+__device__ inline float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+constexpr int DPP_QUAD_XOR1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+constexpr int DPP_ROW_ROR4 = 0x124;
+constexpr int DPP_ROW_ROR8 = 0x128;
+template <class T>
+__device__ inline T row16_sum(T v) {
+  if constexpr (sizeof(T) == 8) {
+    v += dpp_f64<DPP_QUAD_XOR1>(v);
+    v += dpp_f64<DPP_QUAD_XOR2>(v);
+    v += dpp_f64<DPP_ROW_ROR4>(v);
+    v += dpp_f64<DPP_ROW_ROR8>(v);
+  } else {
+    v += dpp_f32<DPP_QUAD_XOR1>(v);
+    v += dpp_f32<DPP_QUAD_XOR2>(v);
+    v += dpp_f32<DPP_ROW_ROR4>(v);
+    v += dpp_f32<DPP_ROW_ROR8>(v);
+  }
+  return v;
+}
 
 // Slots in an LDS queue for the active lanes of a wave with one atomic on the
 // counter *n: the active lanes get consecutive slots in lane order.
@@ -252,6 +298,7 @@ struct wpic_ctx {
   int* err;                              // device error bits
   int* hflags;                           // pinned host mirror: err, active[3], ite[3]
   int cg_iters[3];
+  int cg_hint;  // the largest iteration count of the last solve (the first batch's size)
 
   double* staging;  // transfer staging (lazy)
   size_t staging_n;
@@ -302,6 +349,7 @@ hipError_t launch_rowscan(wpic_ctx* c, bool ywrap);
 hipError_t launch_rowcount(wpic_ctx* c);
 hipError_t launch_rowbuild(wpic_ctx* c);
 hipError_t launch_rowsort(wpic_ctx* c);
+hipError_t launch_rowsort_fast(wpic_ctx* c);
 // transfers and initial state
 hipError_t launch_up_to_soa(wpic_ctx* c, const double* aos, long long n, int which);
 hipError_t launch_soa_to_aos(wpic_ctx* c, double* aos, int which);
@@ -326,6 +374,7 @@ hipError_t launch_mixed_to_aos(wpic_ctx* c, double* aos);
 hipError_t launch_ic_beams_mixed(wpic_ctx* c, unsigned long long seed, int n_ppc, double udrift, double uth_e,
                                  double uth_i);
 hipError_t launch_energy_mixed(wpic_ctx* c, double* dout);
+hipError_t launch_mom_mixed(wpic_ctx* c);
 
 // one fused step (wpic_step2d's body) and the CG iteration loop of field__fdtd_i
 int step_once(wpic_ctx* c);
