@@ -164,6 +164,16 @@ int wpic_energy2d(wpic_ctx *ctx, double out[3]);
 /* Gauss residual div E - 4 pi rho at cell centres, rho with the deposit's
  * quadratic shape; G(nx, ny) column-major.  No reference counterpart. */
 int wpic_gauss2d(wpic_ctx *ctx, double *G);
+/* wpic_gauss2d without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The charge plane
+ * and the residual live in a fixed device array of the library (5 * 2^20
+ * doubles, 40 MiB, held on every GPU the library runs on until the process
+ * ends), so (nx+4)*(ny+4) + nx*ny may not exceed 5 * 2^20: a larger grid is
+ * refused with WPIC_EINVAL and a message.  Every call on one GPU uses that
+ * GPU's array: calls that overlap on one GPU (from several threads or
+ * contexts) race. */
+int wpic_gauss2d_noalloc(wpic_ctx *ctx, double *G);
 /* particle counts per species on the device */
 int wpic_counts2d(wpic_ctx *ctx, long long *counts);
 

@@ -28,7 +28,7 @@ EXPORTS = (
     "wpic_bc_particle_x2d", "wpic_bc_particle_y2d", "wpic_bc_injection2d", "wpic_set_injection2d",
     "wpic_inject2d", "wpic_ic_shock2d",
     "wpic_sort_bucket2d", "wpic_cg_iters2d", "wpic_step2d",
-    "wpic_energy2d", "wpic_gauss2d", "wpic_counts2d", "wpic_ic_beams2d",
+    "wpic_energy2d", "wpic_gauss2d", "wpic_gauss2d_noalloc", "wpic_counts2d", "wpic_ic_beams2d",
     "wpic_set_timing", "wpic_timing",
     "wpic_mom_accl2d", "wpic_mom_nvt2d", "wpic_bc_mom2d", "wpic_moments2d",
     "wpic_download_mom2d", "wpic_upload_mom2d", "wpic_ic_weibel2d", "wpic_ic_harris2d",
@@ -142,6 +142,7 @@ def lib():
         "wpic_step2d": [vp, C.c_int],
         "wpic_energy2d": [vp, np.ctypeslib.ndpointer(np.float64)],
         "wpic_gauss2d": [vp, d],
+        "wpic_gauss2d_noalloc": [vp, d],
         "wpic_counts2d": [vp, np.ctypeslib.ndpointer(np.int64)],
         "wpic_ic_beams2d": [vp, C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
         "wpic_set_timing": [vp, C.c_int],
