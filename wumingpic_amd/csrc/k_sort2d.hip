@@ -450,43 +450,49 @@ __global__ __launch_bounds__(SB) void k_rowsort_fast(SortArgs a, int* perm, int 
   const int nsl = hi - lo;  // chunk index space: stayer slots then arrivals
   const int tot = nsl + (ahi - alo);
   int* hw = hist + w * nx;
-  // element e of this wave's sequence: its source slot, or -1 for a leaver
-  auto source = [&](int e) -> int {
-    if (e < nsl) {
-      const int p = base + lo + e;
-      return ((a.code[p] & 3) == 1) ? p : -1;
-    }
-    const int t = alo + (e - nsl);
-    return t < nu ? ul[t] : dl[t - nu];
-  };
   const unsigned ntot = (unsigned)(g.ncap * g.nsp);
+  constexpr int SF = 16;  // chunks of 64 in flight per wave: the row is walked in few dependent rounds
   for (int pass = 0; pass < 2; pass++) {
-    for (int e0 = 0; e0 < tot; e0 += 64 * SU) {
-      int src[SU], key[SU];
-      bool act[SU];
+    for (int e0 = 0; e0 < tot; e0 += 64 * SF) {
+      int src[SF], key[SF];
+      // stayer slots: the code gives the row step and the key in one load;
+      // arrivals: the list entry, then its code
 #pragma unroll
-      for (int u = 0; u < SU; u++) {
+      for (int u = 0; u < SF; u++) {
         const int e = e0 + u * 64 + lane;
-        src[u] = e < tot ? source(e) : -1;
-        if (src[u] >= 0 && (unsigned)src[u] >= ntot) {
-          if (pass == 1) atomicOr(a.err, ERR_ROW);
-          src[u] = -1;
+        src[u] = -1;
+        key[u] = 0;
+        if (e < nsl) {
+          const int p = base + lo + e;
+          const int c = a.code[p];
+          if ((c & 3) == 1) {
+            src[u] = p;
+            key[u] = c >> 2;
+          }
+        } else if (e < tot) {
+          const int t = alo + (e - nsl);
+          src[u] = t < nu ? ul[t] : dl[t - nu];
+          if ((unsigned)src[u] >= ntot) {
+            if (pass == 1) atomicOr(a.err, ERR_ROW);
+            src[u] = -1;
+          }
         }
-        act[u] = src[u] >= 0;
       }
 #pragma unroll
-      for (int u = 0; u < SU; u++) key[u] = act[u] ? (a.code[src[u]] >> 2) : 0;
+      for (int u = 0; u < SF; u++)
+        if (e0 + u * 64 + lane >= nsl && src[u] >= 0) key[u] = a.code[src[u]] >> 2;
 #pragma unroll
-      for (int u = 0; u < SU; u++) {
-        const uint64_t peers = key_peers(key[u], act[u], nbits);
+      for (int u = 0; u < SF; u++) {
+        const bool act = src[u] >= 0;
+        const uint64_t peers = key_peers(key[u], act, nbits);
         const int rank = __popcll(peers & lt);
         if (pass == 0) {
-          if (act[u] && rank == 0) hw[key[u]] += __popcll(peers);
+          if (act && rank == 0) hw[key[u]] += __popcll(peers);
         } else {
           int pos = 0;
-          if (act[u]) pos = hw[key[u]] + rank;
-          if (act[u] && rank == 0) hw[key[u]] = pos + __popcll(peers);
-          if (act[u]) {
+          if (act) pos = hw[key[u]] + rank;
+          if (act && rank == 0) hw[key[u]] = pos + __popcll(peers);
+          if (act) {
             if (pos < L)
               perm[nb + pos] = src[u];
             else
