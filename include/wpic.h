@@ -28,10 +28,10 @@ extern "C" {
 /* boundary kinds */
 #define WPIC_BC_PERIODIC 0 /* 2d/common/boundary_periodic.f90 */
 #define WPIC_BC_WALLX 1    /* 2d/proj/reconnection/boundary_reconnection.f90 */
-/* 2d/proj/shock/boundary_shock.f90 with the box fully expanded (nxe = nxge):
- * the reconnection's left wall, the injection edge at the right
- * (boundary_shock__injection as the step's x boundary, zero field and phi
- * ghosts beyond nxe); FP64 modes */
+/* 2d/proj/shock/boundary_shock.f90 at the active edge nxe (wpic_set_nxe2d,
+ * wpic_relocate2d; nxe = nxge unless set): the reconnection's left wall, the
+ * injection edge at the right (boundary_shock__injection as the step's x
+ * boundary, zero field and phi ghosts beyond nxe); FP64 modes */
 #define WPIC_BC_SHOCK 2
 
 /* particle kicks of the FP64 modes (wpic_set_pusher2d) */
@@ -126,18 +126,20 @@ int wpic_bc_injection2d(wpic_ctx *ctx, double u0);
 /* the injection 4-velocity u0 (< 0: inflow from the right, the shock app's
  * -|u_inject|, 2d/proj/shock/app.f90:319) of the step's injection edge */
 int wpic_set_injection2d(wpic_ctx *ctx, double u0);
-/* inject (2d/proj/shock/app.f90:697-856) on the sorted state, box fully
- * expanded: nl[r] particles per species join row r at x = nxe dx + (m-1/2)/nl
+/* inject (2d/proj/shock/app.f90:697-856) on the sorted state at the active
+ * edge nxe: nl[r] particles per species join row r at x = nxe dx + (m-1/2)/nl
  * |v0| dt moved by (v0 + ux) dt, y uniform in the row (shared by the
  * species), Maxwellian momenta (v_thi ions, v_the electrons) boosted by the
  * inflow profile vprofile(x) = v0/2 (1 + tanh((x - l_damp - nxgs dx) /
  * (l_damp/10))), ids continuing each species' count; they close cell nxe-1
- * (cumcnt(nxe) and cumcnt(nxe+1) grow by nl[r]); the upstream B and E are set
- * at nxe-1 and nxe.  The split of the injected number over rows (random in the
- * reference) is the caller's; `it` keys the draws of this call.  nl must
- * hold ny = nyge - nygs + 1 entries (one per row); the call reads that many. */
-/* The shock app's initial state (2d/proj/shock/app.f90:405-470), box fully
- * expanded: n0 (nxe - nxs - 1) particles per row per species on a regular x
+ * (cumcnt(nxe) and every entry after it grow by nl[r]); the upstream B and E
+ * are set at nxe-1 and nxe.  The split of the injected number over rows
+ * (random in the reference) is the caller's; `it` keys the draws of this
+ * call.  nl must hold ny = nyge - nygs + 1 entries (one per row); the call
+ * reads that many. */
+/* The shock app's initial state (2d/proj/shock/app.f90:405-470) over the
+ * active box nxs..nxe (wpic_set_nxe2d; nxe = nxge unless set):
+ * n0 (nxe - nxs - 1) particles per row per species on a regular x
  * lattice over [nxs, nxe) dx, y uniform in the row and shared, Maxwellian
  * momenta (v_thi, v_the) boosted by vprofile(x); B = b0 (cos theta,
  * sin theta cos phi, sin theta sin phi) and E = -vprofile(i dx) x B / c at
@@ -146,6 +148,25 @@ int wpic_ic_shock2d(wpic_ctx *ctx, unsigned long long seed, int n0, double v0, d
                     double l_damp, double b0, double theta_bn, double phi_bn);
 int wpic_inject2d(wpic_ctx *ctx, unsigned long long seed, int it, const int *nl, double v0, double v_thi,
                   double v_the, double l_damp, double b0, double theta_bn, double phi_bn);
+/* The shock's expanding box.  The app keeps the active right edge nxe
+ * (app.f90:276-277: nxe = nxgs + n_x_ini) and passes nxs, nxe to every
+ * routine; here the context holds it: the injection edge (xend = nxe dx +
+ * v0 dt), the shock's particle_x wall at nxe-1, the field solve (its right
+ * hand side, CG, E update and boundaries over nodes nxs..nxe, zeros at
+ * nxe+1, uf += df over nxs-2..nxe+2), the initial state and inject follow it.
+ * The cells beyond nxe hold no particles; their cumcnt entries carry the
+ * row's total.  WPIC_BC_SHOCK only; nxgs+3 <= nxe <= nxge. */
+int wpic_set_nxe2d(wpic_ctx *ctx, int nxe);
+int wpic_get_nxe2d(wpic_ctx *ctx, int *nxe);
+/* relocate (2d/proj/shock/app.f90:615-692) on the sorted state: nothing when
+ * nxe = nxge; else nxe grows by one and each row of each species gains n0
+ * particles at its end on the lattice x = (nxe-1) dx + (m-1/2)/n0 dx, y
+ * uniform in the row (shared by the species), momenta as inject's, ids
+ * -(m + (j - nygs) n0 + nptotal) continuing each species' count; cumcnt(nxe)
+ * = cumcnt(nxe-1) + n0 as the reference sets it; the upstream B and E are set
+ * at nxe-1 and nxe.  `it` keys the draws of this call. */
+int wpic_relocate2d(wpic_ctx *ctx, unsigned long long seed, int it, int n0, double v0, double v_thi, double v_the,
+                    double l_damp, double b0, double theta_bn, double phi_bn);
 /* boundary_periodic__particle_y(gp, np2): boundary_periodic.f90:99 */
 int wpic_bc_particle_y2d(wpic_ctx *ctx);
 /* sort__bucket(up, gp, cumcnt, np2, nxs, nxe): 2d/common/sort.f90:36 */

@@ -82,6 +82,11 @@ double or_div_rd(double a, double b) {
  * ------------------------------------------------------------------------- */
 #define NX (p->nxge - p->nxgs + 1)
 #define NY (p->nye - p->nys + 1)
+/* the active right edge nxe the reference passes as an argument (the
+ * shock's expanding box, 2d/proj/shock/app.f90:276-277,625); nxge when 0.
+ * Array layouts keep nxge; the loops that run over nxs..nxe in the reference
+ * stop here. */
+#define NXE(p) ((p)->nxe > 0 ? (p)->nxe : (p)->nxge)
 
 static inline size_t iUP(const or2d_params *p, int idim, int ii, int j, int isp) {
   return (size_t)(idim - 1) +
@@ -187,7 +192,7 @@ void or2d_particle_solv_vay(const or2d_params *p, double *gp, const double *up, 
 
 static void particle_solv(const or2d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt,
                           int vay) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = p->nxgs, nxe = NXE(p), nys = p->nys, nye = p->nye;
   const int tw = NX + 2;
   const double d_delx = 1. / p->delx;
   const double delt = p->delt, c = p->c;
@@ -361,7 +366,7 @@ void or2d_ele_cur(const or2d_params *p, double *uj, const double *up, const doub
 #endif
   if (nth <= 1) {
     for (int j = p->nys; j <= p->nye; j++)
-      for (int i = p->nxgs; i <= p->nxge; i++) ele_cur_cell(p, uj, up, gp, cumcnt, i, j);
+      for (int i = p->nxgs; i <= NXE(p); i++) ele_cur_cell(p, uj, up, gp, cumcnt, i, j);
     return;
   }
   /* OMP REDUCTION(+:uj): private copies summed at the end */
@@ -370,7 +375,7 @@ void or2d_ele_cur(const or2d_params *p, double *uj, const double *up, const doub
     double *priv = (double *)calloc(n3, sizeof(double));
 #pragma omp for schedule(static)
     for (int j = p->nys; j <= p->nye; j++)
-      for (int i = p->nxgs; i <= p->nxge; i++) ele_cur_cell(p, priv, up, gp, cumcnt, i, j);
+      for (int i = p->nxgs; i <= NXE(p); i++) ele_cur_cell(p, priv, up, gp, cumcnt, i, j);
 #pragma omp critical
     for (size_t k = 0; k < n3; k++) uj[k] = uj[k] + priv[k];
     free(priv);
@@ -384,7 +389,7 @@ void or2d_ele_cur(const or2d_params *p, double *uj, const double *up, const doub
 /* boundary_periodic__curre: boundary_periodic.f90:357-508 (y part shared with
  * boundary_reconnection__curre, which has no x fold: :364-502) */
 void or2d_bc_curre(const or2d_params *p, double *uj) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = p->nxgs, nxe = NXE(p), nys = p->nys, nye = p->nye;
   const int w = nxe - nxs + 5;
   double *snd = (double *)malloc(sizeof(double) * 6u * (size_t)w);
 #define UJ(cc, i, j) uj[iF3(p, cc, i, j)]
@@ -460,7 +465,7 @@ void or2d_bc_curre(const or2d_params *p, double *uj) {
 /* boundary_periodic__dfield: boundary_periodic.f90:251-354;
  * boundary_reconnection__dfield x walls: boundary_reconnection.f90:350-359 */
 void or2d_bc_dfield(const or2d_params *p, double *df) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = p->nxgs, nxe = NXE(p), nys = p->nys, nye = p->nye;
   const int w = nxe - nxs + 1;
   double *snd = (double *)malloc(sizeof(double) * 12u * (size_t)w);
 #define DF(cc, i, j) df[iF6(p, cc, i, j)]
@@ -524,7 +529,7 @@ void or2d_bc_dfield(const or2d_params *p, double *df) {
  * boundary_reconnection__phi walls: boundary_reconnection.f90:557-577.
  * phi(nxs-1:nxe+1, nys-1:nye+1) */
 void or2d_bc_phi(const or2d_params *p, double *phi, int l) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = p->nxgs, nxe = NXE(p), nys = p->nys, nye = p->nye;
   const int pw = NX + 2;
 #define PH(i, j) phi[(size_t)((i) - (nxs - 1)) + (size_t)pw * (size_t)((j) - (nys - 1))]
   double *snd = (double *)malloc(sizeof(double) * (size_t)NX);
@@ -565,7 +570,7 @@ void or2d_bc_phi(const or2d_params *p, double *phi, int l) {
  * the reference computes them at one thread.
  * ------------------------------------------------------------------------- */
 int or2d_cgm(const or2d_params *p, or2d_fstate *st) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = p->nxgs, nxe = NXE(p), nys = p->nys, nye = p->nye;
   const int ite_max = 100;
   const double err = g_cg_err;
   double f[5];
@@ -667,7 +672,7 @@ int or2d_field_fdtd_i(const or2d_params *p, or2d_fstate *st, double *uf, const d
 /* field.f90:124-184: the RHS, the three solves, E, the ghost fills and the
  * update from the current in st->uj (after its boundary fold) */
 int or2d_field_from_current(const or2d_params *p, or2d_fstate *st, double *uf) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = p->nxgs, nxe = NXE(p), nys = p->nys, nye = p->nye;
   const double pi = M_PI, delt = p->delt, gfac = p->gfac;
   double f[5];
   or2d_field_consts(p, f);
@@ -736,7 +741,7 @@ void or2d_bc_particle_x(const or2d_params *p, double *up, const int *np2) {
       for (int ii = 1; ii <= np2[iNP(p, j, isp)]; ii++) {
         double *u = &up[iUP(p, 1, ii, j, isp)];
         if (p->bc != OR_BC_PERIODIC) { /* boundary_shock__particle_x is the same wall pair */
-          const int nxs = nxgs, nxe = nxge;
+          const int nxs = nxgs, nxe = NXE(p);
           const int ipos = (int)(u[0] / delx);
           if (ipos < nxs + 1) {
             u[0] = (double)(2.0f * (float)(nxs + 1)) * delx - u[0];
@@ -761,12 +766,12 @@ void or2d_bc_particle_x(const or2d_params *p, double *up, const int *np2) {
   }
 }
 
-/* boundary_shock__injection (2d/proj/shock/boundary_shock.f90:255-297) on
- * the whole box (nxs = nxgs, nxe = nxge): the reconnection's left wall, and a
- * mirror moving with the inflow at xend = nxe dx + v0 dt that sends ux to
+/* boundary_shock__injection (2d/proj/shock/boundary_shock.f90:255-297) at
+ * the active edge (nxs = nxgs, nxe = NXE): the reconnection's left wall, and
+ * a mirror moving with the inflow at xend = nxe dx + v0 dt that sends ux to
  * 2 u0 - ux. */
 void or2d_bc_injection(const or2d_params *p, double *up, const int *np2) {
-  const int nxs = p->nxgs, nxe = p->nxge;
+  const int nxs = p->nxgs, nxe = NXE(p);
   const double delx = p->delx, c = p->c, u0 = p->u0;
   const double xend = nxe * delx + u0 / sqrt(1 + (u0 * u0) / (c * c)) * p->delt;
   for (int isp = 1; isp <= p->nsp; isp++)
@@ -904,12 +909,15 @@ done:
  * sort__bucket: 2d/common/sort.f90:36-82 (cell = int(x), no 1/delx)
  * ------------------------------------------------------------------------- */
 void or2d_sort_bucket(const or2d_params *p, double *gp, const double *up, int *cumcnt, const int *np2) {
-  const int nxs = p->nxgs, nxe = p->nxge;
+  const int nxs = p->nxgs, nxe = NXE(p);
   for (int isp = 1; isp <= p->nsp; isp++) {
 #pragma omp parallel for
     for (int j = p->nys; j <= p->nye; j++) {
-      int *cnt = (int *)calloc((size_t)(nxe - nxs + 1), sizeof(int));
-      int *sum_cnt = (int *)calloc((size_t)(nxe - nxs + 2), sizeof(int));
+      /* cnt(nxs:nxe), sum_cnt(nxs:nxe+1) of the reference, held over the
+       * whole x range (a particle beyond nxe writes out of the reference's
+       * bounds; here it stays in memory but its slot is undefined there) */
+      int *cnt = (int *)calloc((size_t)(p->nxge - nxs + 1), sizeof(int));
+      int *sum_cnt = (int *)calloc((size_t)(p->nxge - nxs + 2), sizeof(int));
       const int n = np2[iNP(p, j, isp)];
       for (int ii = 1; ii <= n; ii++) {
         const int i = (int)up[iUP(p, 1, ii, j, isp)];
@@ -944,7 +952,7 @@ int or2d_step(const or2d_params *p, or2d_fstate *st, double *up, double *gp, dou
       or2d_bc_particle_x(p, gp, np2);
       if (or2d_field_fdtd_i(p, st, uf, up, gp, cumcnt)) return -1;
     } else if (p->bc == OR_BC_SHOCK) {
-      /* 2d/proj/shock/app.f90:111-116 (the box fully expanded) */
+      /* 2d/proj/shock/app.f90:111-116 */
       or2d_bc_injection(p, gp, np2);
       if (or2d_field_fdtd_i(p, st, uf, up, gp, cumcnt)) return -1;
     } else {
@@ -1037,7 +1045,7 @@ static inline size_t iMOM(const or2d_params *p, int k, int i, int j, int isp) {
  * no move: gp(1:2) = up(1:2), gp(3:5) = the kicked momenta.  gp(6) is not
  * written (the reference's gp is intent(out) there). */
 void or2d_mom_accl(const or2d_params *p, double *gp, const double *up, const double *uf, const int *cumcnt) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = p->nxgs, nxe = NXE(p), nys = p->nys, nye = p->nye;
   const int tw = NX + 2;
   const double d_delx = 1. / p->delx;
   const double delt = p->delt * 0.5, c = p->c;

@@ -31,7 +31,7 @@ class Params2D(C.Structure):
         ("nys", C.c_int), ("nye", C.c_int),
         ("delx", C.c_double), ("delt", C.c_double), ("c", C.c_double), ("gfac", C.c_double),
         ("q", C.c_double * MAXSP), ("r", C.c_double * MAXSP),
-        ("bc", C.c_int), ("u0", C.c_double),
+        ("bc", C.c_int), ("u0", C.c_double), ("nxe", C.c_int),
     ]
 
     @property

@@ -73,6 +73,7 @@ typedef struct {
   double q[OR_MAXSP], r[OR_MAXSP];
   int bc;
   double u0; /* OR_BC_SHOCK: the injection 4-velocity (negative: inflow from the right) */
+  int nxe;   /* OR_BC_SHOCK: the active right edge while the box expands (0: nxge) */
 } or2d_params;
 
 /* persistent module state of field.f90 (df, gkl, uj are `save`d) */

@@ -28,3 +28,19 @@ def shock_state(nx, ny, n_ppc, seed, u0=-0.3, theta=np.pi / 3, phi=np.pi / 2, vt
     uf[5] = -v0 * uf[1]
     uf += 0.01 * b0 * rng.standard_normal(uf.shape)
     return p, np.asfortranarray(up), np.asfortranarray(uf), cc, np2
+
+
+def wider_box(p, uf, cc, extra, rng):
+    """The same state in a box `extra` cells wider whose active edge nxe is the narrow box's
+    nxge (the shock's expanding box, 2d/proj/shock/app.f90:276-277): the narrow planes fill the
+    first columns, the columns beyond get unrelated values (a field there must not reach the
+    active box), the cumcnt entries beyond nxe+1 carry each row's total."""
+    pb = po.make_params(p.nx + extra, p.ny, np_cap=p.np, delt=p.delt, q=[p.q[k] for k in range(p.nsp)],
+                        r=[p.r[k] for k in range(p.nsp)], bc=p.bc, u0=p.u0)
+    pb.nxe = p.nxge
+    ufb = np.asfortranarray(rng.standard_normal((6, pb.nx + 4, p.ny + 4)) * np.abs(uf).max())
+    ufb[:, :p.nx + 4, :] = uf
+    ccb = np.zeros((pb.nx + 1, p.ny, p.nsp), dtype=np.int32, order="F")
+    ccb[:p.nx + 1] = cc
+    ccb[p.nx + 1:] = cc[-1][None]
+    return pb, ufb, ccb

@@ -305,7 +305,7 @@ def test_driver_restart_at_the_elapsed_limit():
 
 
 def test_shock_initial_state_matches_its_formulas():
-    """2d/proj/shock/app.f90:405-470 with the box fully expanded"""
+    """2d/proj/shock/app.f90:405-470 with the box fully expanded (n_x_ini = n_x - 1)"""
     from wumingpic_amd import MODE_PARITY
     nx, ny, n0 = 32, 6, 4
     u0 = -0.3
@@ -384,3 +384,42 @@ def test_shock_tracers_in_the_initial_state_and_orb_files():
             meta = json.load(open(os.path.join(d, "%07d_orb.json" % it)))
             for sp in ("up01", "up02"):
                 assert meta["dataset"][sp]["shape"][1] == want, (it, sp, meta["dataset"][sp]["shape"])
+
+
+def test_shock_app_expanding_box():
+    """relocate in the shock app's loop (2d/proj/shock/app.f90:120-125, :615-692): the box starts
+    with nxe = nxgs + n_x_ini and grows by one cell every intvl_expand steps; the restart records
+    the grown edge; the particles stay left of it; the fields beyond nxe+2 keep their initial
+    values; each relocation adds n0 particles per row per species"""
+    nx, ny, n0, steps, ini, every = 40, 8, 4, 6, 20, 2
+    par = {"num_process": 1, "n_ppc": n0, "n_x": nx, "n_x_ini": ini, "n_y": ny, "u_inject": 0.3,
+           "mass_ratio": 25.0, "sigma_e": 0.1, "omega_pe": 0.1, "v_the": 0.05, "v_thi": 0.01, "theta_bn": 60.0,
+           "phi_bn": 90.0, "l_damp_ini": 6.0, "seed": 3}
+    np_cap = n0 * nx * 5
+    print("shock app expanding box", flush=True)
+    out = {}
+    for max_it in (0, steps):
+        cfg = {"config": {"verbose": 0, "datadir": ".", "max_elapsed": 1.0e6, "max_it": max_it,
+                          "intvl_ptcl": 1000, "intvl_mom": 1000, "intvl_orb": 1000, "intvl_expand": every},
+               "parameter": par}
+        with tempfile.TemporaryDirectory() as d:
+            _run_app(cfg, d)
+            out[max_it] = _read_restart(d, "%07d_restart" % (max_it + 1), np_cap, ny, nx)
+    meta0, up0, np20, uf0 = out[0]
+    meta, up, np2, uf = out[steps]
+    assert meta0["attribute"]["nxe"]["data"] == 2 + ini
+    nxe = meta["attribute"]["nxe"]["data"]
+    assert nxe == 2 + ini + steps // every
+    assert np.all(np20 == n0 * (ini - 1))
+    v0 = 0.3 / np.sqrt(1 + 0.09)
+    pflux = n0 * v0 * 1.0 * 1.0 * ny
+    for s in range(2):
+        tot = int(np2[:, s].sum())
+        base = n0 * (ini - 1) * ny + (steps // every) * n0 * ny
+        assert base + steps * int(pflux) <= tot <= base + steps * (int(pflux) + 1)
+        for j in range(ny):
+            x = up[0, :np2[j, s], j, s]
+            assert np.all(x >= 3.0) and np.all(x < nxe + 1.0)
+    i_beyond = nxe + 3 - 0  # plane column of node nxe+3 (columns start at nxgs-2 = 0)
+    assert np.array_equal(uf[:, i_beyond:, :].view(np.uint64), uf0[:, i_beyond:, :].view(np.uint64))
+    assert not np.array_equal(uf[:, :i_beyond, :], uf0[:, :i_beyond, :])

@@ -117,3 +117,52 @@ def test_fortran_app_loop_on_shim_matches_oracle(bc, resident):
                         assert np.array_equal(a[:, :np2[j, s], j, s], b[:, :np2[j, s], j, s]), (s, j)
                 continue
             assert mine == data, name
+
+
+@pytest.mark.parametrize("resident", [0, 1])
+def test_fortran_shock_loop_with_active_edge_on_shim_matches_oracle(resident):
+    """the shock app's step (2d/proj/shock/app.f90:112-117) through the shim's particle, field,
+    sort and boundary_shock modules while the box is expanding (nxe < nxge passed to every
+    call): particles, cell offsets up to nxe+1, fields and the moment sequence equal the oracle's
+    bit for bit in parity mode.  The input cumcnt holds zeros beyond nxe+1, as the reference's
+    sort__bucket leaves them."""
+    from shock_state import shock_state, wider_box
+    p0, up, uf0, cc0, np2 = shock_state(24, 16, 8, 41, u0=-0.35)
+    p, uf, cc = wider_box(p0, uf0, cc0, 10, np.random.default_rng(42))
+    cc[p0.nx + 1:] = 0
+    nsteps = 5
+    with tempfile.TemporaryDirectory() as tmp:
+        with open(os.path.join(tmp, "params.txt"), "w") as f:
+            f.write(f"{nsteps} 2 1 {resident} {p.ndim} {p.np} {p.nsp} {p.nxgs} {p.nxge} {p.nygs} {p.nyge}\n")
+            f.write(f"{p.delx!r} {p.delt!r} {p.c!r} {p.gfac!r}\n")
+            f.write(" ".join(repr(float(p.q[k])) for k in range(p.nsp)) + "\n")
+            f.write(" ".join(repr(float(p.r[k])) for k in range(p.nsp)) + "\n")
+            f.write(f"{p.nxe} {p.u0!r}\n")
+        up.ravel(order="F").tofile(os.path.join(tmp, "up.bin"))
+        uf.ravel(order="F").tofile(os.path.join(tmp, "uf.bin"))
+        cc.ravel(order="F").tofile(os.path.join(tmp, "cumcnt.bin"))
+        np2.ravel(order="F").tofile(os.path.join(tmp, "np2.bin"))
+        print(f"fortran shock shim, nxe {p.nxe} of {p.nxge}, resident {resident}", flush=True)
+        r = subprocess.run([DRIVER, tmp], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        u_dev = np.fromfile(os.path.join(tmp, "up.out")).reshape(up.shape, order="F")
+        f_dev = np.fromfile(os.path.join(tmp, "uf.out")).reshape(uf.shape, order="F")
+        c_dev = np.fromfile(os.path.join(tmp, "cumcnt.out"), dtype=np.int32).reshape(cc.shape, order="F")
+        n_dev = np.fromfile(os.path.join(tmp, "np2.out"), dtype=np.int32).reshape(np2.shape, order="F")
+        m_dev = np.fromfile(os.path.join(tmp, "mom.out")).reshape((7, p.nx + 2, p.ny + 2, p.nsp), order="F")
+        meta = json.load(open(os.path.join(tmp, "restart.json")))
+    assert meta["attribute"]["nxe"]["data"] == p.nxe
+    o = po.Oracle2D(p)
+    gp = np.zeros_like(up, order="F")
+    o.step(up, gp, uf, cc, np2, nsteps)
+    assert np.array_equal(n_dev, np2)
+    assert np.array_equal(c_dev[:p.nxe - p.nxgs + 2], cc[:p.nxe - p.nxgs + 2])
+    for s in range(2):
+        for j in range(p.ny):
+            n = np2[j, s]
+            assert np.array_equal(_bits(u_dev[:, :n, j, s]), _bits(up[:, :n, j, s])), (s, j)
+    assert np.array_equal(_bits(f_dev), _bits(uf))
+    o.mom_accl(gp, up, uf, cc)
+    mom = o.mom_nvt(gp, np2)
+    o.bc_mom(mom)
+    assert np.array_equal(_bits(m_dev), _bits(mom))

@@ -1,8 +1,9 @@
-"""The oracle's shock boundaries (2d/proj/shock/boundary_shock.f90 with nxs = nxgs, nxe = nxge).
+"""The oracle's shock boundaries (2d/proj/shock/boundary_shock.f90 with nxs = nxgs).
 boundary_shock.f90 needs MPI, so no reference build pins them; these tests check what the
 mathematics fixes: the injection mirror is an involution, with u0 = 0 it is the static mirror
 x -> 2 nxe dx - x, u -> -u; the left wall is the reconnection's; after a field solve the field
-and CG ghosts beyond nxe are zero and the left ghosts are the reconnection's."""
+and CG ghosts beyond nxe are zero and the left ghosts are the reconnection's; with an active edge
+nxe below nxge (the expanding box) the steps equal those of the box cut at nxe."""
 import numpy as np
 
 from oracle import pyoracle as po
@@ -92,3 +93,36 @@ def test_field_solve_ghosts():
     os_.step(up1.copy(order="F"), np.zeros_like(up1, order="F"), ufs, cc1.copy(order="F"), np21.copy(order="F"), 1)
     assert not np.array_equal(ufs[:, -4:, :], ufw[:, -4:, :])
     assert np.allclose(ufs[:, 2:8, :], ufw[:, 2:8, :], rtol=0, atol=1e-3 * np.abs(ufs).max())
+
+
+def test_active_edge_equals_the_box_cut_there():
+    """The expanding box (nxe < nxge, 2d/proj/shock/app.f90:112-117 with nxs, nxe passed to every
+    routine): steps of a box whose active edge is nxe equal, bit for bit, the steps of a box that
+    ends at nxe (its particles, cell offsets and fields up to nxe+2), and leave the fields beyond
+    nxe+2 untouched.  Pins the active-edge loops of the push, deposit, current fold, field solve
+    (right-hand side, CG sums, E update, ghosts) and sort against the full-box code."""
+    from shock_state import wider_box
+    p, up, uf, cc, np2 = shock_state(24, 8, 8, 7)
+    pb, ufb, ccb = wider_box(p, uf, cc, 9, np.random.default_rng(8))
+    beyond = ufb[:, p.nx + 4:, :].copy()
+    upb, np2b = up.copy(order="F"), np2.copy(order="F")
+    o = po.Oracle2D(p)
+    o.step(up, np.zeros_like(up, order="F"), uf, cc, np2, 5)
+    ob = po.Oracle2D(pb)
+    ob.step(upb, np.zeros_like(upb, order="F"), ufb, ccb, np2b, 5)
+    assert np.array_equal(np2b, np2)
+    assert np.array_equal(ccb[:p.nx + 1], cc)
+    for s in range(2):
+        for j in range(p.ny):
+            n = np2[j, s]
+            assert np.array_equal(_bits(upb[:, :n, j, s]), _bits(up[:, :n, j, s])), (s, j)
+    assert np.array_equal(_bits(ufb[:, :p.nx + 4, :]), _bits(uf))
+    assert np.array_equal(_bits(ufb[:, p.nx + 4:, :]), _bits(beyond))
+    assert tuple(o.st.cg_iters) == tuple(ob.st.cg_iters)
+    # the same state with the whole wider box active differs (the edge is where the walls are)
+    pf = po.make_params(pb.nx, p.ny, np_cap=p.np, delt=p.delt, q=[p.q[0], p.q[1]], r=[p.r[0], p.r[1]],
+                        bc=p.bc, u0=p.u0)
+    p2, up2, uf2, cc2, np22 = shock_state(24, 8, 8, 7)
+    _, uff, ccf = wider_box(p2, uf2, cc2, 9, np.random.default_rng(8))
+    po.Oracle2D(pf).step(up2, np.zeros_like(up2, order="F"), uff, ccf, np22, 1)
+    assert not np.array_equal(uff[:, :p.nx + 4, :], ufb[:, :p.nx + 4, :])

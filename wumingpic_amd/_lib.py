@@ -26,7 +26,7 @@ EXPORTS = (
     "wpic_download_dfield2d", "wpic_upload_dfield2d",
     "wpic_particle_solv2d", "wpic_particle_solv_vay2d", "wpic_set_pusher2d", "wpic_field_fdtd_i2d",
     "wpic_bc_particle_x2d", "wpic_bc_particle_y2d", "wpic_bc_injection2d", "wpic_set_injection2d",
-    "wpic_inject2d", "wpic_ic_shock2d",
+    "wpic_inject2d", "wpic_ic_shock2d", "wpic_relocate2d", "wpic_set_nxe2d", "wpic_get_nxe2d",
     "wpic_sort_bucket2d", "wpic_cg_iters2d", "wpic_step2d",
     "wpic_energy2d", "wpic_gauss2d", "wpic_gauss2d_noalloc", "wpic_counts2d", "wpic_ic_beams2d",
     "wpic_set_timing", "wpic_timing",
@@ -136,6 +136,10 @@ def lib():
                             C.c_double, C.c_double],
         "wpic_inject2d": [vp, C.c_ulonglong, C.c_int, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                           C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double],
+        "wpic_relocate2d": [vp, C.c_ulonglong, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                            C.c_double, C.c_double, C.c_double],
+        "wpic_set_nxe2d": [vp, C.c_int],
+        "wpic_get_nxe2d": [vp, C.POINTER(C.c_int)],
         "wpic_bc_particle_y2d": [vp],
         "wpic_sort_bucket2d": [vp],
         "wpic_cg_iters2d": [vp, np.ctypeslib.ndpointer(np.int32)],

@@ -439,6 +439,19 @@ __global__ void k_uf_add(size_t n, double* __restrict__ uf, const double* __rest
     uf[k] = uf[k] + df[k];
 }
 
+// uf += df over columns nxgs-2..nxge+2 of a cut grid (field.f90:176-184 with
+// nxe below the planes' edge); the columns beyond keep their values
+__global__ void k_uf_add_cols(Grid2D g, double* __restrict__ uf, const double* __restrict__ df) {
+  const int w = g.nx + 4;
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= (size_t)6 * w * g.fh) return;
+  const int i = (int)(t % w);
+  const size_t rest = t / w;
+  const int J = (int)(rest % g.fh), comp = (int)(rest / g.fh);
+  const size_t k = (size_t)g.fw * g.fh * comp + (size_t)g.fw * J + i;
+  uf[k] = uf[k] + df[k];
+}
+
 __global__ void k_fill(double* p, size_t n, double v) {
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
 }
@@ -446,9 +459,23 @@ __global__ void k_fill(double* p, size_t n, double v) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// The field solve's grid.  While the shock's box expands the solve spans
+// nodes nxgs..nxa (field.f90 runs every loop over nxs..nxe, the CG's vectors
+// and sums included, and its boundary routines act at nxe+1): the same grid
+// with nx and nxge cut there, over the full-size planes (fidx keeps the
+// storage width fw).
+static Grid2D field_grid(const wpic_ctx* c) {
+  Grid2D g = c->g;
+  if (g.nxa != g.nxge) {
+    g.nx = g.nxa - g.nxgs + 1;
+    g.nxge = g.nxa;
+  }
+  return g;
+}
+
 static CgArgs cg_args(wpic_ctx* c) {
   CgArgs a;
-  a.g = c->g;
+  a.g = field_grid(c);
   a.uf = c->uf;
   a.uj = c->uj;
   a.df = c->df;
@@ -478,6 +505,8 @@ hipError_t launch_fill(wpic_ctx* c, double* p, size_t n, double v) {
 
 hipError_t launch_zero(wpic_ctx* c, double* p, size_t n) { return launch_fill(c, p, n, 0.0); }
 
+// (the y fold runs over every column of the plane: beyond nxa+2 the current
+// is zero while the shock's box expands)
 hipError_t launch_curre_bc(wpic_ctx* c) {
   hipLaunchKernelGGL(k_curre_y, dim3(grid_for(3 * c->g.fw)), dim3(256), 0, c->st, c->g, c->uj);
   if (c->g.bc == WPIC_BC_PERIODIC)
@@ -486,8 +515,9 @@ hipError_t launch_curre_bc(wpic_ctx* c) {
 }
 
 hipError_t launch_dfield_bc(wpic_ctx* c) {
-  hipLaunchKernelGGL(k_dfield_y, dim3(grid_for(6 * c->g.nx)), dim3(256), 0, c->st, c->g, c->df);
-  hipLaunchKernelGGL(k_dfield_x, dim3(grid_for(c->g.fh)), dim3(256), 0, c->st, c->g, c->df);
+  const Grid2D g = field_grid(c);
+  hipLaunchKernelGGL(k_dfield_y, dim3(grid_for(6 * g.nx)), dim3(256), 0, c->st, g, c->df);
+  hipLaunchKernelGGL(k_dfield_x, dim3(grid_for(g.fh)), dim3(256), 0, c->st, g, c->df);
   return hipGetLastError();
 }
 
@@ -525,17 +555,23 @@ hipError_t launch_cg_iter(wpic_ctx* c, int it) {
 
 hipError_t launch_cg_store(wpic_ctx* c) {
   CgArgs a = cg_args(c);
-  hipLaunchKernelGGL(k_cg_store, dim3(grid_for((size_t)c->g.nx * c->g.ny)), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg_store, dim3(grid_for((size_t)a.g.nx * a.g.ny)), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_e_update(wpic_ctx* c) {
-  hipLaunchKernelGGL(k_e_update, dim3(grid_for((size_t)c->g.nx * c->g.ny)), dim3(256), 0, c->st, c->g, c->uf, c->uj,
-                     c->df);
+  const Grid2D g = field_grid(c);
+  hipLaunchKernelGGL(k_e_update, dim3(grid_for((size_t)g.nx * g.ny)), dim3(256), 0, c->st, g, c->uf, c->uj, c->df);
   return hipGetLastError();
 }
 
 hipError_t launch_uf_add(wpic_ctx* c) {
+  if (c->g.nxa != c->g.nxge) {
+    const Grid2D g = field_grid(c);
+    const size_t n = (size_t)6 * (g.nx + 4) * g.fh;
+    hipLaunchKernelGGL(k_uf_add_cols, dim3(grid_for(n)), dim3(256), 0, c->st, g, c->uf, c->df);
+    return hipGetLastError();
+  }
   const size_t n = (size_t)6 * c->g.fw * c->g.fh;
   hipLaunchKernelGGL(k_uf_add, dim3(grid_for(n) < 8192 ? grid_for(n) : 8192), dim3(256), 0, c->st, n, c->uf, c->df);
   return hipGetLastError();

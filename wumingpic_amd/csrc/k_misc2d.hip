@@ -464,8 +464,8 @@ hipError_t launch_ic_harris(wpic_ctx* c, unsigned long long seed, int np2, int i
   return hipGetLastError();
 }
 
-// The shock app's initial state (2d/proj/shock/app.f90:405-470) with the box
-// fully expanded (nxs = nxgs, nxe = nxge): per row np2 particles on a regular
+// The shock app's initial state (2d/proj/shock/app.f90:405-470) over the
+// active box (nxs = nxgs, nxe = nxa): per row np2 particles on a regular
 // x lattice over [nxs, nxe) dx, y uniform in the row and shared by the
 // species, Maxwellian momenta boosted by the inflow profile; written unsorted
 // into B (the reference sorts them next); the upstream field with
@@ -491,7 +491,7 @@ __global__ void k_ic_shock(IcShockArgs a) {
   const int ii = (int)(t - (long long)r * a.np2) + 1;
   const int j = g.nys + r;
   const double c = g.c;
-  const double x = (g.nxgs + (g.nxge - g.nxgs) * (ii - 0.5) / a.np2) * g.delx;
+  const double x = (g.nxgs + (g.nxa - g.nxgs) * (ii - 0.5) / a.np2) * g.delx;
   const double y = (j + u01(rkey(a.seed, 7, r, 0, ii))) * g.delx;
   for (int s = 0; s < 2; s++) {
     double n0, n1, n2, n3;
@@ -550,15 +550,23 @@ hipError_t launch_ic_shock(wpic_ctx* c, unsigned long long seed, int np2, double
   return hipGetLastError();
 }
 
-// The shock app's inject (2d/proj/shock/app.f90:697-856) with the box fully
-// expanded (nxe = nxge), fused with the re-layout it needs: one block per
-// destination row copies the row's sorted particles (read through the
-// permutation) into B at the packed new base, then draws the row's nl
-// injected particles after them: positions spread over |v0| dt beyond nxe,
-// moved by (v0 + ux) dt, y uniform in the row and shared by the species,
-// Maxwellian momenta boosted by the inflow profile v1(x), ids continuing the
-// species' count.  cumcnt(nxe) and cumcnt(nxe+1) grow by nl: the injected
-// particles close cell nxe-1's range, where the reference counts them.
+// The shock app's inject (2d/proj/shock/app.f90:697-856) at the active edge
+// nxe = nxa, fused with the re-layout it needs: one block per destination
+// row copies the row's sorted particles (read through the permutation) into
+// B at the packed new base, then draws the row's nl injected particles after
+// them: positions spread over |v0| dt beyond nxe, moved by (v0 + ux) dt, y
+// uniform in the row and shared by the species, Maxwellian momenta boosted
+// by the inflow profile v1(x), ids continuing the species' count.
+// cumcnt(nxe) and every entry after it grow by nl: the injected particles
+// close cell nxe-1's range, where the reference counts them, and the cells
+// from nxe on keep their (empty) ranges.
+//
+// With `reloc` the same kernel is relocate (app.f90:615-692), called after
+// nxa has grown by one: nl = n0 particles per row on a regular x lattice over
+// cell nxe-1 (not moved), y uniform in the row and shared by the species,
+// the same momenta and ids (pid = ii + (j - nygs) n0 + nptotal); cumcnt(nxe)
+// = cumcnt(nxe-1) + n0 as the reference sets it, and the entries after it
+// the row's new total (the cells beyond the new edge stay empty).
 struct InjArgs {
   Grid2D g;
   Parts A, B;
@@ -568,7 +576,7 @@ struct InjArgs {
   const long long* idoff;    // [nsp*ny]
   int* cumcnt;
   unsigned long long seed;
-  int it;
+  int it, reloc;
   double v0, x0, sd[2], xc, xs;
 };
 
@@ -590,16 +598,19 @@ __global__ __launch_bounds__(256) void k_inject(InjArgs a) {
   }
   const double delx = g.delx, delt = g.delt, c = g.c;
   const int j = g.nys + r;
+  const int tag = a.reloc ? 70 : 40;  // relocate draws from streams of its own
   for (int m = 1 + threadIdx.x; m <= nl; m += 256) {
     const int d = bn + n_old + m - 1;
-    double x = g.nxge * delx + (m - 0.5) / nl * a.x0;
-    const double y = (j + u01(rkey(a.seed, 40, r, (uint64_t)a.it, m))) * delx;
+    double x = a.reloc ? (g.nxa - 1) * delx + (m - 0.5) / nl * delx : g.nxa * delx + (m - 0.5) / nl * a.x0;
+    const double y = (j + u01(rkey(a.seed, tag, r, (uint64_t)a.it, m))) * delx;
     double n0, n1, n2, n3;
-    normal2(rkey(a.seed, 41 + 2 * s, r, (uint64_t)a.it, m), rkey(a.seed, 42 + 2 * s, r, (uint64_t)a.it, m), n0, n1);
-    normal2(rkey(a.seed, 45 + 2 * s, r, (uint64_t)a.it, m), rkey(a.seed, 46 + 2 * s, r, (uint64_t)a.it, m), n2, n3);
+    normal2(rkey(a.seed, tag + 1 + 2 * s, r, (uint64_t)a.it, m), rkey(a.seed, tag + 2 + 2 * s, r, (uint64_t)a.it, m),
+            n0, n1);
+    normal2(rkey(a.seed, tag + 5 + 2 * s, r, (uint64_t)a.it, m), rkey(a.seed, tag + 6 + 2 * s, r, (uint64_t)a.it, m),
+            n2, n3);
     double ux = a.sd[s] * n0;
     const double uy = a.sd[s] * n1, uz = a.sd[s] * n2;
-    x = x + (a.v0 + ux) * delt;
+    if (!a.reloc) x = x + (a.v0 + ux) * delt;
     const double v1 = 0.5 * a.v0 * (1 + tanh((x - a.xc) / a.xs));
     const double gam1 = 1 / sqrt(1 - (v1 / c) * (v1 / c));
     const double gamp = sqrt(1 + (ux * ux + uy * uy + uz * uz) / (c * c));
@@ -613,18 +624,24 @@ __global__ __launch_bounds__(256) void k_inject(InjArgs a) {
   }
   if (threadIdx.x == 0 && nl > 0) {
     int* cr = a.cumcnt + (size_t)rw * (g.nx + 1);
-    cr[g.nx - 1] += nl;
-    cr[g.nx] += nl;
+    const int e = g.nxa - g.nxgs;  // cumcnt(nxe)
+    if (a.reloc) {
+      cr[e] = cr[e - 1] + nl;
+      for (int k = e + 1; k <= g.nx; k++) cr[k] = n_old + nl;
+    } else {
+      for (int k = e; k <= g.nx; k++) cr[k] += nl;
+    }
   }
 }
 
-// the upstream field at i = nxe-1 and nxe over every row (app.f90:840-850)
+// the upstream field at i = nxe-1 and nxe over every row (app.f90:840-850,
+// relocate's :681-690)
 __global__ void k_inject_fields(Grid2D g, double* uf, double by, double bz, double v0) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.fh) return;
   const int J = g.nys - 2 + t;
   const size_t pl = (size_t)g.fw * g.fh;
-  const int f1 = fidx(g, g.nxge - 1, J), f2 = fidx(g, g.nxge, J);
+  const int f1 = fidx(g, g.nxa - 1, J), f2 = fidx(g, g.nxa, J);
   uf[pl + f1] = by;
   uf[2 * pl + f1] = bz;
   uf[4 * pl + f1] = +v0 * uf[2 * pl + f1] / g.c;
@@ -634,7 +651,8 @@ __global__ void k_inject_fields(Grid2D g, double* uf, double by, double bz, doub
 }
 
 hipError_t launch_inject(wpic_ctx* c, unsigned long long seed, int it, const int* base_new, const int* nl,
-                         const long long* idoff, double v0, const double* sd, double l_damp, double by, double bz) {
+                         const long long* idoff, double v0, const double* sd, double l_damp, double by, double bz,
+                         bool reloc) {
   InjArgs a;
   a.g = c->g;
   a.A = c->A;
@@ -648,6 +666,7 @@ hipError_t launch_inject(wpic_ctx* c, unsigned long long seed, int it, const int
   a.cumcnt = c->cumcnt;
   a.seed = seed;
   a.it = it;
+  a.reloc = reloc ? 1 : 0;
   a.v0 = v0;
   a.x0 = fabs(v0) * c->g.delt;
   a.sd[0] = sd[0];

@@ -335,6 +335,7 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
   g.ny = ny;
   g.nxgs = p->nxgs;
   g.nxge = p->nxge;
+  g.nxa = p->nxge;
   g.nygs = p->nygs;
   g.nyge = p->nyge;
   g.nys = p->nygs;
@@ -724,7 +725,15 @@ namespace {
 void set_u0(wpic_ctx* c, double u0) {
   const double delx = c->g.delx, cc = c->g.c;
   c->g.u0 = u0;
-  c->g.xend = c->g.nxge * delx + u0 / sqrt(1 + (u0 * u0) / (cc * cc)) * c->g.delt;
+  c->g.xend = c->g.nxa * delx + u0 / sqrt(1 + (u0 * u0) / (cc * cc)) * c->g.delt;
+}
+
+// the active right edge nxe: the injection edge and the shock's particle_x
+// wall move with it (boundary_shock.f90:87-88,269)
+void set_nxa(wpic_ctx* c, int nxe) {
+  c->g.nxa = nxe;
+  c->g.wall_hi = (double)(2.0f * (float)(nxe - 1)) * c->g.delx;
+  set_u0(c, c->g.u0);
 }
 }  // namespace
 
@@ -745,7 +754,7 @@ int wpic_ic_shock2d(wpic_ctx* c, unsigned long long seed, int n0, double v0, dou
   if (g.nsp != 2) return set_error(WPIC_EINVAL, "the shock state has two species (ions, electrons)");
   if (!(std::fabs(v0) < g.c) || !(l_damp > 0.0)) return set_error(WPIC_EINVAL, "need |v0| < c and l_damp_ini > 0");
   // np2 = n0 (nxe - nxs - 1) per row (app.f90:334)
-  const long long np2l = (long long)n0 * (g.nxge - g.nxgs - 1);
+  const long long np2l = (long long)n0 * (g.nxa - g.nxgs - 1);
   if (n0 <= 0 || np2l * g.ny > g.ncap) return set_error(WPIC_EMEM, "Error: Too large number of particles");
   const int np2 = (int)np2l;
   const int nr = g.nsp * g.ny;
@@ -772,15 +781,23 @@ int wpic_ic_shock2d(wpic_ctx* c, unsigned long long seed, int n0, double v0, dou
   return wpic_sort_bucket2d(c);
 }
 
-int wpic_inject2d(wpic_ctx* c, unsigned long long seed, int it, const int* nl, double v0, double v_thi, double v_the,
-                  double l_damp, double b0, double theta_bn, double phi_bn) {
-  if (!c || !nl) return set_error(WPIC_EINVAL, "null argument");
+namespace {
+// checks shared by inject and relocate
+int shock_rows_check(wpic_ctx* c, const char* what, double v0, double l_damp) {
   if (c->mixed) return mixed_only_step();
-  if (c->g.bc != WPIC_BC_SHOCK) return set_error(WPIC_EINVAL, "inject needs WPIC_BC_SHOCK");
+  if (c->g.bc != WPIC_BC_SHOCK) return set_error(WPIC_EINVAL, std::string(what) + " needs WPIC_BC_SHOCK");
+  if (c->g.nsp != 2) return set_error(WPIC_EINVAL, std::string(what) + " loads two species (ions, electrons)");
+  if (c->exch_pending)
+    return set_error(WPIC_EINVAL, std::string(what) + " needs the sorted state (after sort__bucket)");
+  if (!(std::fabs(v0) < c->g.c) || !(l_damp > 0.0)) return set_error(WPIC_EINVAL, "need |v0| < c and l_damp_ini > 0");
+  return 0;
+}
+
+// inject and relocate: row r of each species grows by nl[r] particles at its
+// end, the rows re-laid at their new packed bases (k_inject)
+int grow_rows(wpic_ctx* c, unsigned long long seed, int it, const int* nl, double v0, double v_thi, double v_the,
+              double l_damp, double b0, double theta_bn, double phi_bn, bool reloc) {
   const Grid2D& g = c->g;
-  if (g.nsp != 2) return set_error(WPIC_EINVAL, "inject loads two species (ions, electrons)");
-  if (c->exch_pending) return set_error(WPIC_EINVAL, "inject needs the sorted state (after sort__bucket)");
-  if (!(std::fabs(v0) < g.c) || !(l_damp > 0.0)) return set_error(WPIC_EINVAL, "need |v0| < c and l_damp_ini > 0");
   const int nr = g.nsp * g.ny;
   if (fetch_rows(c, c->ra)) return WPIC_EHIP;
   const std::vector<int> old = c->h_rows[c->ra];
@@ -811,7 +828,7 @@ int wpic_inject2d(wpic_ctx* c, unsigned long long seed, int it, const int* nl, d
   HIPCHK(hipMemcpyAsync(c->rows[c->rb], hrows.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice, c->st));
   const double sd[2] = {v_thi, v_the};
   const double by = b0 * sin(theta_bn) * cos(phi_bn), bz = b0 * sin(theta_bn) * sin(phi_bn);
-  HIPCHK(launch_inject(c, seed, it, c->rows[c->rb], d_nl, d_idoff, v0, sd, l_damp, by, bz));
+  HIPCHK(launch_inject(c, seed, it, c->rows[c->rb], d_nl, d_idoff, v0, sd, l_damp, by, bz, reloc));
   HIPCHK(hipStreamSynchronize(c->st));
   // B holds the new sorted state in identity order
   std::swap(c->A, c->B);
@@ -827,6 +844,47 @@ int wpic_inject2d(wpic_ctx* c, unsigned long long seed, int it, const int* nl, d
   c->b_aligned = false;
   c->exch_pending = false;
   return check_err(c);
+}
+}  // namespace
+
+int wpic_inject2d(wpic_ctx* c, unsigned long long seed, int it, const int* nl, double v0, double v_thi, double v_the,
+                  double l_damp, double b0, double theta_bn, double phi_bn) {
+  if (!c || !nl) return set_error(WPIC_EINVAL, "null argument");
+  int rc = shock_rows_check(c, "inject", v0, l_damp);
+  if (rc) return rc;
+  return grow_rows(c, seed, it, nl, v0, v_thi, v_the, l_damp, b0, theta_bn, phi_bn, false);
+}
+
+int wpic_relocate2d(wpic_ctx* c, unsigned long long seed, int it, int n0, double v0, double v_thi, double v_the,
+                    double l_damp, double b0, double theta_bn, double phi_bn) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  int rc = shock_rows_check(c, "relocate", v0, l_damp);
+  if (rc) return rc;
+  if (n0 < 0) return set_error(WPIC_EINVAL, "negative n0");
+  if (c->g.nxa == c->g.nxge) return 0;  // app.f90:622
+  const int old = c->g.nxa;
+  set_nxa(c, old + 1);
+  const std::vector<int> nl(c->g.ny, n0);
+  rc = grow_rows(c, seed, it, nl.data(), v0, v_thi, v_the, l_damp, b0, theta_bn, phi_bn, true);
+  if (rc) set_nxa(c, old);
+  return rc;
+}
+
+int wpic_set_nxe2d(wpic_ctx* c, int nxe) {
+  if (!c) return set_error(WPIC_EINVAL, "null context");
+  const Grid2D& g = c->g;
+  if (nxe == g.nxa) return 0;
+  if (g.bc != WPIC_BC_SHOCK) return set_error(WPIC_EINVAL, "only the shock's box has an active edge below nxge");
+  if (nxe < g.nxgs + 3 || nxe > g.nxge)
+    return set_error(WPIC_EINVAL, "nxe must lie in [nxgs+3, nxge], got " + std::to_string(nxe));
+  set_nxa(c, nxe);
+  return 0;
+}
+
+int wpic_get_nxe2d(wpic_ctx* c, int* nxe) {
+  if (!c || !nxe) return set_error(WPIC_EINVAL, "null argument");
+  *nxe = c->g.nxa;
+  return 0;
 }
 
 int wpic_bc_injection2d(wpic_ctx* c, double u0) {

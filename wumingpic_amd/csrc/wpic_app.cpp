@@ -11,8 +11,8 @@
 // Weibel app, "nbg"/"ncs" for the reconnection app, "udrift" for the
 // counter-streaming electron beams of the headline benchmark (an extension:
 // the reference's Weibel app loads an anisotropic Maxwellian instead),
-// "u_inject" for the shock app (2d/proj/shock/app.f90) with its box fully
-// expanded (n_x_ini = n_x - 1: relocate returns at once there).
+// "u_inject" for the shock app (2d/proj/shock/app.f90), its box growing from
+// n_x_ini cells by relocate every intvl_expand steps.
 // Optional extension keys in "parameter": "seed" (the counter-based draws
 // replacing the reference's OS-seeded ones), "e1" (the X-point perturbation
 // the reconnection app hard-codes as 0.12).
@@ -47,7 +47,7 @@ struct Config {
   int verbose = 0;
   std::string datadir;
   double max_elapsed = 0;
-  int max_it = 0, intvl_ptcl = 0, intvl_mom = 0, intvl_orb = 0;
+  int max_it = 0, intvl_ptcl = 0, intvl_mom = 0, intvl_orb = 0, intvl_expand = 0;
   std::string restart_file;
 };
 
@@ -202,7 +202,9 @@ struct App {
   int save_restart(int it) {
     const std::string name = restart_name(it);
     if (pull_particles()) return -1;
-    if (chk(wpic_paraio_output(io, up.data(), uf.data(), np2.data(), P.nxgs, P.nxge, it, name.c_str()))) return -1;
+    int nxe = P.nxge;  // the shock's active edge (io__output(up, uf, np2, nxs, nxe, ...))
+    if (chk(wpic_get_nxe2d(ctx, &nxe))) return -1;
+    if (chk(wpic_paraio_output(io, up.data(), uf.data(), np2.data(), P.nxgs, nxe, it, name.c_str()))) return -1;
     rewrite_config(cfg, name);
     return 0;
   }
@@ -236,15 +238,14 @@ struct App {
     P.r[1] = 1.0;
     double b0 = 0, x0 = 0, y0 = 0, lcs = 0, rtemp = 0, t_ani = 0, udrift = 0, e1 = 0.12;
     double u0 = 0, v0 = 0, theta = 0, phi = 0, l_damp = 0, pflux = 0;
-    int nbg = 0, ncs = 0;
+    int nbg = 0, ncs = 0, n_x_ini = 0;
     if (shock) {
-      // 2d/proj/shock/app.f90:240-345 with the box fully expanded
-      int n_x_ini = 0;
+      // 2d/proj/shock/app.f90:240-345; the box spans nxs..nxe = nxgs + n_x_ini
+      // and grows by relocate every intvl_expand steps
       double sigma_e = 0, omega_pe = 0, u_inject = 0;
       get_int(par, "n_ppc", n0);
-      get_int(par, "n_x_ini", n_x_ini);
-      if (n_x_ini != nx - 1)
-        return fail("Error: this driver runs the fully expanded box (n_x_ini = n_x - 1); relocate is not built");
+      if (!get_int(par, "n_x_ini", n_x_ini)) return fail("Error: n_x_ini is required");
+      if (n_x_ini < 3 || n_x_ini > nx - 1) return fail("Error: n_x_ini must lie in [3, n_x - 1]");
       get_num(par, "u_inject", u_inject);
       get_num(par, "sigma_e", sigma_e);
       get_num(par, "omega_pe", omega_pe);
@@ -342,6 +343,8 @@ struct App {
       int nxs, nxe;
       if (chk(wpic_paraio_input(io, up.data(), uf.data(), np2.data(), &nxs, &nxe, &it0, cfg.restart_file.c_str())))
         return -1;
+      if (shock && (nxs != P.nxgs || nxe < P.nxgs + 3 || nxe > P.nxge))
+        return fail("Error: the restart file's nxs, nxe lie outside the box");
       long long most = 0;
       for (int s = 0; s < P.nsp; s++) {
         long long t = 0;
@@ -350,6 +353,7 @@ struct App {
       }
       P.capacity = std::max(P.capacity, most);
       if (chk(wpic_create2d(&ctx, &P))) return -1;
+      if (shock && chk(wpic_set_nxe2d(ctx, nxe))) return -1;
       if (chk(wpic_upload_gp2d(ctx, up.data(), np2.data(), np)) || chk(wpic_upload_fields2d(ctx, uf.data())))
         return -1;
       // io__input into gp, then sort__bucket(up, gp, ...) (app.f90:346-348)
@@ -365,7 +369,9 @@ struct App {
       else if (beams)
         rc = wpic_ic_beams2d(ctx, seed, n0, udrift, vte, vti, b0);
       else if (shock)
-        rc = wpic_ic_shock2d(ctx, seed, n0, v0, vti, vte, l_damp, b0, theta, phi);
+        rc = wpic_set_nxe2d(ctx, P.nxgs + n_x_ini)
+                 ? -1
+                 : wpic_ic_shock2d(ctx, seed, n0, v0, vti, vte, l_damp, b0, theta, phi);
       else
         rc = wpic_ic_harris2d(ctx, seed, nbg, ncs, lcs, x0, y0, b0, e1, rtemp, vti, vte);
       if (chk(rc)) return -1;
@@ -396,6 +402,10 @@ struct App {
         }
         for (int r = 0; r < nginj % ny; r++) nl[idx[r]]++;
         if (chk(wpic_inject2d(ctx, seed, it, nl.data(), v0, vti, vte, l_damp, b0, theta, phi))) return -1;
+        // expand box (app.f90:122-125)
+        if (cfg.intvl_expand > 0 && it % cfg.intvl_expand == 0 &&
+            chk(wpic_relocate2d(ctx, seed, it, n0, v0, vti, vte, l_damp, b0, theta, phi)))
+          return -1;
       }
       if (cfg.intvl_ptcl > 0 && it % cfg.intvl_ptcl == 0) {
         if (pull_particles() || chk(wpic_paraio_ptcl(io, up.data(), uf.data(), np2.data(), it))) return -1;
@@ -608,6 +618,7 @@ int load_config(const char* config, Config& cfg) {
   get_int(c, "intvl_ptcl", cfg.intvl_ptcl);
   get_int(c, "intvl_mom", cfg.intvl_mom);
   get_int(c, "intvl_orb", cfg.intvl_orb);
+  get_int(c, "intvl_expand", cfg.intvl_expand);
   const JVal* rf = c ? c->get("restart_file") : nullptr;
   if (rf && rf->k == JVal::STR && !rf->s.empty()) cfg.restart_file = rf->s;
   return 0;

@@ -20,12 +20,13 @@ namespace wpic {
 // ---------------------------------------------------------------------------
 struct Grid2D {
   int nx, ny, nxgs, nxge, nygs, nyge, nys, nye;
+  int nxa;     // the active right edge node nxe (= nxge; below it while the shock's box expands)
   int fw, fh;  // plane width nx+4 and height ny+4
   int bc, nsp;
   double delx, delt, c, gfac, d_delx, d_delt;
   double f1, f2, f3, f4, f5, pi4dt;  // field.f90:53-57; 4*pi*delt
   double spanx, spany;               // nx*delx, ny*delx rounded down
-  double wall_lo, wall_hi;           // 2*(nxs+1)*delx, 2*(nxe-1)*delx
+  double wall_lo, wall_hi;           // 2*(nxs+1)*delx, 2*(nxa-1)*delx
   double u0, xend;                   // WPIC_BC_SHOCK: injection 4-velocity, moving mirror position
   double q[WPIC_MAXSP], r[WPIC_MAXSP];
   long long ncap;                    // particle capacity per species
@@ -144,7 +145,7 @@ __host__ __device__ inline void reflect_x(const Grid2D& g, double& x, double& ux
       ux = +2. * g.u0 - ux;
       uy = -uy; uz = -uz;
     }
-  } else if (ipos >= g.nxge - 1) {
+  } else if (ipos >= g.nxa - 1) {
     x = g.wall_hi - x;
     ux = -ux; uy = -uy; uz = -uz;
   }
@@ -344,7 +345,8 @@ hipError_t launch_wrapx(wpic_ctx* c, bool injection);
 hipError_t launch_ic_shock(wpic_ctx* c, unsigned long long seed, int np2, double v0, const double* sd, double l_damp,
                            const double* b);
 hipError_t launch_inject(wpic_ctx* c, unsigned long long seed, int it, const int* base_new, const int* nl,
-                         const long long* idoff, double v0, const double* sd, double l_damp, double by, double bz);
+                         const long long* idoff, double v0, const double* sd, double l_damp, double by, double bz,
+                         bool reloc);
 hipError_t launch_rowscan(wpic_ctx* c, bool ywrap);
 hipError_t launch_rowcount(wpic_ctx* c);
 hipError_t launch_rowbuild(wpic_ctx* c);

@@ -70,7 +70,7 @@ contains
     real(8), intent(in)  :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
     real(8), intent(out) :: gp(s_ndim, s_np, s_nys:s_nye, s_nsp)
     logical, intent(in)  :: vay
-    integer(c_int) :: np2(s_nys:s_nye, s_nsp)
+    integer(c_int) :: np2(s_nys:s_nye, s_nsp), cc(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp)
     if (.not. have_particle) then
        write(6,*) 'Initialize first by calling particle__init()'
        stop
@@ -78,8 +78,10 @@ contains
     ! rows are sorted: the last cell offset is the row count (sort.f90:64-69)
     np2 = cumcnt(nxe+1, :, :)
     call shim_ctx(np2)
+    call shim_edge(nxe)
     if (.not. resident) then
-       call shim_check(wpic_upload_up2d(ctx, up, np2, cumcnt, s_np))
+       call shim_cumcnt(cumcnt, nxe, cc)
+       call shim_check(wpic_upload_up2d(ctx, up, np2, cc, s_np))
        call shim_check(wpic_upload_fields2d(ctx, uf))
     end if
     if (vay) then
@@ -141,15 +143,17 @@ contains
     real(8), intent(in)    :: gp(s_ndim, s_np, s_nys:s_nye, s_nsp)
     real(8), intent(in)    :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
     real(8), intent(inout) :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
-    integer(c_int) :: np2(s_nys:s_nye, s_nsp)
+    integer(c_int) :: np2(s_nys:s_nye, s_nsp), cc(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp)
     if (.not. have_field) then
        write(6,*) 'Initialize first by calling field__init()'
        stop
     end if
     np2 = cumcnt(nxe+1, :, :)
     call shim_ctx(np2)
+    call shim_edge(nxe)
     if (.not. resident) then
-       call shim_check(wpic_upload_up2d(ctx, up, np2, cumcnt, s_np))
+       call shim_cumcnt(cumcnt, nxe, cc)
+       call shim_check(wpic_upload_up2d(ctx, up, np2, cc, s_np))
        call shim_check(wpic_upload_gp2d(ctx, gp, np2, s_np))
        call shim_check(wpic_upload_fields2d(ctx, uf))
     end if
@@ -184,6 +188,7 @@ contains
     real(8), intent(out) :: gp(s_ndim, s_np, s_nys:s_nye, s_nsp)
     integer(c_int) :: n2(s_nys:s_nye, s_nsp)
     call shim_ctx(np2)
+    call shim_edge(nxe)
     if (.not. resident) call shim_check(wpic_upload_gp2d(ctx, up, np2, s_np))
     call shim_check(wpic_sort_bucket2d(ctx))
     if (.not. resident) then
@@ -227,15 +232,17 @@ contains
     real(8), intent(in)  :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
     real(8), intent(in)  :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
     real(8), intent(out) :: gp(s_ndim, s_np, s_nys:s_nye, s_nsp)
-    integer(c_int) :: np2(s_nys:s_nye, s_nsp)
+    integer(c_int) :: np2(s_nys:s_nye, s_nsp), cc(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp)
     if (.not. have_mom) then
        write(6,*) 'Initialize first by calling mom_calc__init()'
        stop
     end if
     np2 = cumcnt(nxe+1, :, :)
     call shim_ctx(np2)
+    call shim_edge(nxe)
     if (.not. resident) then
-       call shim_check(wpic_upload_up2d(ctx, up, np2, cumcnt, s_np))
+       call shim_cumcnt(cumcnt, nxe, cc)
+       call shim_check(wpic_upload_up2d(ctx, up, np2, cc, s_np))
        call shim_check(wpic_upload_fields2d(ctx, uf))
     end if
     call shim_check(wpic_mom_accl2d(ctx))
@@ -374,6 +381,7 @@ contains
     real(8), intent(inout) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
     integer(c_int) :: n2(s_nys:s_nye, s_nsp)
     call shim_ctx(np2)
+    call shim_edge(nxe)
     if (.not. resident) call shim_check(wpic_upload_gp2d(ctx, up, np2, s_np))
     call shim_check(wpic_bc_particle_x2d(ctx))
     if (.not. resident) call shim_check(wpic_download_gp2d(ctx, up, n2, s_np))
@@ -429,8 +437,8 @@ contains
 end module boundary_reconnection
 
 
-! the shock app's boundaries (2d/proj/shock/boundary_shock.f90) with the box
-! fully expanded; the field boundaries are applied on the device
+! the shock app's boundaries (2d/proj/shock/boundary_shock.f90) at the active
+! edge nxe each call passes; the field boundaries are applied on the device
 module boundary_shock
   use iso_c_binding
   use wpic_capi
@@ -461,13 +469,14 @@ contains
     real(8), intent(inout) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
     integer(c_int) :: n2(s_nys:s_nye, s_nsp)
     call shim_ctx(np2)
+    call shim_edge(nxe)
     if (.not. resident) call shim_check(wpic_upload_gp2d(ctx, up, np2, s_np))
     call shim_check(wpic_bc_particle_x2d(ctx))
     if (.not. resident) call shim_check(wpic_download_gp2d(ctx, up, n2, s_np))
   end subroutine boundary_shock__particle_x
 
   ! the injection edge moving with the inflow (boundary_shock.f90:255-297),
-  ! with nxs = nxgs, nxe = nxge
+  ! with nxs = nxgs
   subroutine boundary_shock__injection(up, np2, nxs, nxe, u0)
     integer, intent(in)    :: nxs, nxe
     integer, intent(in)    :: np2(s_nys:s_nye, s_nsp)
@@ -475,6 +484,7 @@ contains
     real(8), intent(in)    :: u0
     integer(c_int) :: n2(s_nys:s_nye, s_nsp)
     call shim_ctx(np2)
+    call shim_edge(nxe)
     if (.not. resident) call shim_check(wpic_upload_gp2d(ctx, up, np2, s_np))
     call shim_check(wpic_bc_injection2d(ctx, u0))
     if (.not. resident) call shim_check(wpic_download_gp2d(ctx, up, n2, s_np))
@@ -548,14 +558,24 @@ contains
     resident = on
   end subroutine wpic_shim_resident
 
-  ! copy the app's sorted particles (with cumcnt) and fields to the device
-  subroutine wpic_shim_push(up, uf, np2, cumcnt)
+  ! copy the app's sorted particles (with cumcnt) and fields to the device;
+  ! nxe: the shock's active edge while its box expands (cumcnt is read up to
+  ! nxe+1, as the reference's sort__bucket writes it)
+  subroutine wpic_shim_push(up, uf, np2, cumcnt, nxe)
     real(8), intent(in) :: up(s_ndim, s_np, s_nys:s_nye, s_nsp)
     real(8), intent(in) :: uf(6, s_nxgs-2:s_nxge+2, s_nys-2:s_nye+2)
     integer, intent(in) :: np2(s_nys:s_nye, s_nsp)
     integer, intent(in) :: cumcnt(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp)
+    integer, intent(in), optional :: nxe
+    integer(c_int) :: cc(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp)
     call shim_ctx(np2)
-    call shim_check(wpic_upload_up2d(ctx, up, np2, cumcnt, s_np))
+    if (present(nxe)) then
+       call shim_edge(nxe)
+       call shim_cumcnt(cumcnt, nxe, cc)
+    else
+       cc = cumcnt
+    end if
+    call shim_check(wpic_upload_up2d(ctx, up, np2, cc, s_np))
     call shim_check(wpic_upload_fields2d(ctx, uf))
   end subroutine wpic_shim_push
 

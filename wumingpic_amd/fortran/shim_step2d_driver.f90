@@ -1,13 +1,15 @@
 ! shim_step2d_driver.f90 -- the reference app's per-step call sequence
 ! (2d/proj/weibel/app.f90:100-107 for periodic boxes,
-! 2d/proj/reconnection/app.f90:100-106 for x walls) written against the shim
-! modules, reading and writing the app's arrays as raw files.
+! 2d/proj/reconnection/app.f90:100-106 for x walls, 2d/proj/shock/app.f90:
+! 111-117 for the shock's box with its active edge nxe) written against the
+! shim modules, reading and writing the app's arrays as raw files.
 !
 ! Usage: shim_step2d_driver <case-dir>
 !   params.txt  nsteps bc mode resident ndim np nsp nxgs nxge nygs nyge
 !               delx delt c gfac
 !               q(1:nsp)
 !               r(1:nsp)
+!               nxe u0          (bc = 2 only)
 !   in:  up.bin uf.bin cumcnt.bin np2.bin     out: up.out uf.out cumcnt.out np2.out
 !   and, after the loop, the app's moment output sequence
 !   (2d/proj/weibel/app.f90:121-124): mom.out, then io__mom and save_restart's
@@ -29,11 +31,14 @@ program shim_step2d_driver
        bcr_x => boundary_reconnection__particle_x, bcr_y => boundary_reconnection__particle_y, &
        bcr_dfield => boundary_reconnection__dfield, bcr_curre => boundary_reconnection__curre, &
        bcr_phi => boundary_reconnection__phi, bcr_mom => boundary_reconnection__mom
+  use boundary_shock, only: bcs_init => boundary_shock__init, bcs_inj => boundary_shock__injection, &
+       bcs_y => boundary_shock__particle_y, bcs_dfield => boundary_shock__dfield, &
+       bcs_curre => boundary_shock__curre, bcs_phi => boundary_shock__phi, bcs_mom => boundary_shock__mom
   implicit none
   character(len=512) :: dir
   integer :: nsteps, bc, mode, res, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, nxs, nxe, u, it
   integer :: nstat(1)
-  real(8) :: delx, delt, c, gfac
+  real(8) :: delx, delt, c, gfac, u0
   real(8), allocatable :: q(:), r(:), up(:,:,:,:), gp(:,:,:,:), uf(:,:,:), mom(:,:,:,:)
   integer, allocatable :: cumcnt(:,:,:), np2(:,:)
 
@@ -44,11 +49,13 @@ program shim_step2d_driver
   read(u,*) delx, delt, c, gfac
   read(u,*) q
   read(u,*) r
-  close(u)
   nys = nygs
   nye = nyge
   nxs = nxgs
   nxe = nxge
+  u0 = 0.0d0
+  if (bc == 2) read(u,*) nxe, u0
+  close(u)
   nstat = 0
 
   allocate(up(ndim,np,nys:nye,nsp), gp(ndim,np,nys:nye,nsp))
@@ -64,8 +71,10 @@ program shim_step2d_driver
   if (mode == 1) call wpic_shim_mode(WPIC_MODE_PARITY)
   if (bc == 0) then
      call bcp_init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, 0, 0, 0, 0, 0, 0, nstat, delx, delt, c)
-  else
+  else if (bc == 1) then
      call bcr_init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, 0, 0, 0, 0, 0, 0, nstat, delx, delt, c)
+  else
+     call bcs_init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, 0, 0, 0, 0, 0, 0, nstat, delx, delt, c)
   end if
   call particle__init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c, q, r)
   call field__init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, 0, 0, 0, 0, delx, delt, c, q, r, gfac)
@@ -73,7 +82,7 @@ program shim_step2d_driver
   call mom_calc__init(ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c, q, r)
   if (res == 1) then
      call wpic_shim_resident(.true.)
-     call wpic_shim_push(up, uf, np2, cumcnt)
+     call wpic_shim_push(up, uf, np2, cumcnt, nxe)
   end if
 
   do it = 1, nsteps
@@ -82,10 +91,14 @@ program shim_step2d_driver
         call field__fdtd_i(uf, up, gp, cumcnt, nxs, nxe, bcp_dfield, bcp_curre, bcp_phi)
         call bcp_x(gp, np2)
         call bcp_y(gp, np2)
-     else
+     else if (bc == 1) then
         call bcr_x(gp, np2, nxs, nxe)
         call field__fdtd_i(uf, up, gp, cumcnt, nxs, nxe, bcr_dfield, bcr_curre, bcr_phi)
         call bcr_y(gp, np2)
+     else
+        call bcs_inj(gp, np2, nxs, nxe, u0)
+        call field__fdtd_i(uf, up, gp, cumcnt, nxs, nxe, bcs_dfield, bcs_curre, bcs_phi)
+        call bcs_y(gp, np2)
      end if
      call sort__bucket(up, gp, cumcnt, np2, nxs, nxe)
   end do
@@ -93,8 +106,10 @@ program shim_step2d_driver
   call mom_calc__nvt(mom, gp, np2)
   if (bc == 0) then
      call bcp_mom(mom)
-  else
+  else if (bc == 1) then
      call bcr_mom(mom)
+  else
+     call bcs_mom(mom)
   end if
   if (res == 1) call wpic_shim_pull(up, uf, np2, cumcnt)
   call wpic_shim_finalize()

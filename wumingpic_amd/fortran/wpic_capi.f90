@@ -187,6 +187,21 @@ module wpic_capi
        real(c_double), value :: u0
      end function wpic_bc_injection2d
 
+     integer(c_int) function wpic_set_nxe2d(ctx, nxe) bind(C, name='wpic_set_nxe2d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nxe
+     end function wpic_set_nxe2d
+
+     integer(c_int) function wpic_relocate2d(ctx, seed, it, n0, v0, v_thi, v_the, l_damp, b0, theta_bn, phi_bn) &
+          bind(C, name='wpic_relocate2d')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       integer(c_long_long), value :: seed
+       integer(c_int), value :: it, n0
+       real(c_double), value :: v0, v_thi, v_the, l_damp, b0, theta_bn, phi_bn
+     end function wpic_relocate2d
+
      integer(c_int) function wpic_particle_solv_vay2d(ctx) bind(C, name='wpic_particle_solv_vay2d')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -56,6 +56,25 @@ contains
     prm%nyge = nyge_in
   end subroutine shim_set_grid
 
+  ! the active right edge nxe an entry point is passed (the shock's box grows
+  ! from nxgs + n_x_ini); the context takes it before the call
+  subroutine shim_edge(nxe)
+    integer, intent(in) :: nxe
+    call shim_check(wpic_set_nxe2d(ctx, int(nxe, c_int)))
+  end subroutine shim_edge
+
+  ! cumcnt as the device takes it: the entries beyond nxe+1, which the
+  ! reference's sort__bucket leaves as they were, carry the row's total
+  subroutine shim_cumcnt(cumcnt, nxe, cc)
+    integer, intent(in) :: cumcnt(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp), nxe
+    integer(c_int), intent(out) :: cc(s_nxgs:s_nxge+1, s_nys:s_nye, s_nsp)
+    integer :: i
+    cc = cumcnt
+    do i = nxe + 2, s_nxge + 1
+       cc(i, :, :) = cumcnt(nxe + 1, :, :)
+    end do
+  end subroutine shim_cumcnt
+
   ! create the context on first use; capacity = largest species count seen
   ! in np2 (particle counts are conserved by the periodic and wall boundaries)
   subroutine shim_ctx(np2)

@@ -143,8 +143,25 @@ class Pic2D:
                                     float(v0), float(v_thi), float(v_the), float(l_damp), float(b0),
                                     float(theta_bn), float(phi_bn)))
 
+    def relocate(self, seed, it, n0, v0, v_thi, v_the, l_damp, b0, theta_bn, phi_bn):
+        """the shock app's relocate (2d/proj/shock/app.f90:615-692): the box grows by one cell at
+        its active edge nxe, each row of each species gains n0 particles in the new cell nxe-1"""
+        check(self._L.wpic_relocate2d(self._h, int(seed), int(it), int(n0), float(v0), float(v_thi),
+                                      float(v_the), float(l_damp), float(b0), float(theta_bn), float(phi_bn)))
+
+    @property
+    def nxe(self):
+        """the active right edge nxe of the shock's box (nxge unless set or grown)"""
+        v = C.c_int()
+        check(self._L.wpic_get_nxe2d(self._h, C.byref(v)))
+        return v.value
+
+    @nxe.setter
+    def nxe(self, nxe):
+        check(self._L.wpic_set_nxe2d(self._h, int(nxe)))
+
     def ic_shock(self, seed, n0, v0, v_thi, v_the, l_damp, b0, theta_bn, phi_bn):
-        """the shock app's initial state (2d/proj/shock/app.f90:405-470), box fully expanded"""
+        """the shock app's initial state (2d/proj/shock/app.f90:405-470) over the active box"""
         check(self._L.wpic_ic_shock2d(self._h, int(seed), int(n0), float(v0), float(v_thi), float(v_the),
                                       float(l_damp), float(b0), float(theta_bn), float(phi_bn)))
 
