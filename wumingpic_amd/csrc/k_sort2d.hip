@@ -451,53 +451,72 @@ __global__ __launch_bounds__(SB) void k_rowsort_fast(SortArgs a, int* perm, int 
   const int tot = nsl + (ahi - alo);
   int* hw = hist + w * nx;
   const unsigned ntot = (unsigned)(g.ncap * g.nsp);
-  constexpr int SF = 16;  // chunks of 64 in flight per wave: the row is walked in few dependent rounds
+  const int* codes = a.code;
+  // SF chunks of 64 per wave and round.  Every load of a round is issued
+  // before the first is used (addresses are selected, not branched on), and
+  // the per-key cursors move by LDS atomics, whose returns the LDS serves in
+  // issue order: a round waits for memory twice and for the LDS once.
+  constexpr int SF = 8;
+  bool bad = false;
   for (int pass = 0; pass < 2; pass++) {
     for (int e0 = 0; e0 < tot; e0 += 64 * SF) {
-      int src[SF], key[SF];
-      // stayer slots: the code gives the row step and the key in one load;
-      // arrivals: the list entry, then its code
+      // a stayer slot's code (row step and key in one word), or an arrival's
+      // list entry (its source slot)
+      int v[SF];
 #pragma unroll
       for (int u = 0; u < SF; u++) {
         const int e = e0 + u * 64 + lane;
-        src[u] = -1;
-        key[u] = 0;
-        if (e < nsl) {
-          const int p = base + lo + e;
-          const int c = a.code[p];
-          if ((c & 3) == 1) {
-            src[u] = p;
-            key[u] = c >> 2;
-          }
-        } else if (e < tot) {
-          const int t = alo + (e - nsl);
-          src[u] = t < nu ? ul[t] : dl[t - nu];
-          if ((unsigned)src[u] >= ntot) {
-            if (pass == 1) atomicOr(a.err, ERR_ROW);
-            src[u] = -1;
-          }
-        }
+        const int t = alo + (e - nsl);
+        const int* ptr = e < nsl ? codes + (base + lo + e) : (e < tot ? (t < nu ? ul + t : dl + (t - nu)) : codes);
+        v[u] = *ptr;
       }
-#pragma unroll
-      for (int u = 0; u < SF; u++)
-        if (e0 + u * 64 + lane >= nsl && src[u] >= 0) key[u] = a.code[src[u]] >> 2;
+      int src[SF];
 #pragma unroll
       for (int u = 0; u < SF; u++) {
-        const bool act = src[u] >= 0;
-        const uint64_t peers = key_peers(key[u], act, nbits);
-        const int rank = __popcll(peers & lt);
-        if (pass == 0) {
-          if (act && rank == 0) hw[key[u]] += __popcll(peers);
-        } else {
-          int pos = 0;
-          if (act) pos = hw[key[u]] + rank;
-          if (act && rank == 0) hw[key[u]] = pos + __popcll(peers);
-          if (act) {
-            if (pos < L)
-              perm[nb + pos] = src[u];
-            else
-              atomicOr(a.err, ERR_ROW);
-          }
+        const int e = e0 + u * 64 + lane;
+        const bool arr = e >= nsl && e < tot;
+        const bool okarr = arr && (unsigned)v[u] < ntot;
+        bad |= arr && !okarr;
+        src[u] = e < nsl ? (((v[u] & 3) == 1) ? base + lo + e : -1) : (okarr ? v[u] : -1);
+      }
+      // an arrival's code (the stayers already hold theirs)
+      int c2[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        const int e = e0 + u * 64 + lane;
+        c2[u] = codes[(e >= nsl && src[u] >= 0) ? src[u] : 0];
+      }
+      int key[SF];
+      uint64_t peers[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        const int e = e0 + u * 64 + lane;
+        key[u] = src[u] < 0 ? 0 : ((e < nsl ? v[u] : c2[u]) >> 2);
+        peers[u] = key_peers(key[u], src[u] >= 0, nbits);
+      }
+      if (pass == 0) {
+#pragma unroll
+        for (int u = 0; u < SF; u++)
+          if (src[u] >= 0 && __popcll(peers[u] & lt) == 0) atomicAdd(&hw[key[u]], __popcll(peers[u]));
+      } else {
+        // the lowest lane of each key takes the key's cursor for all of them
+        int old[SF];
+#pragma unroll
+        for (int u = 0; u < SF; u++) {
+          old[u] = 0;
+          if (src[u] >= 0 && __popcll(peers[u] & lt) == 0) old[u] = atomicAdd(&hw[key[u]], __popcll(peers[u]));
+        }
+        int pos[SF];
+#pragma unroll
+        for (int u = 0; u < SF; u++) {
+          const int lead = src[u] >= 0 ? __builtin_ctzll(peers[u]) : lane;
+          pos[u] = __shfl(old[u], lead, 64) + __popcll(peers[u] & lt);
+        }
+#pragma unroll
+        for (int u = 0; u < SF; u++) {
+          const bool ok = src[u] >= 0 && pos[u] < L;
+          bad |= src[u] >= 0 && !ok;
+          if (ok) perm[nb + pos[u]] = src[u];
         }
       }
     }
@@ -542,6 +561,7 @@ __global__ __launch_bounds__(SB) void k_rowsort_fast(SortArgs a, int* perm, int 
     }
     __syncthreads();
   }
+  if (bad) atomicOr(a.err, ERR_ROW);
 }
 
 // sort__bucket on a gp that did not go through bc__particle_y (the restart
