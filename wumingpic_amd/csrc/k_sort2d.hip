@@ -296,10 +296,30 @@ __global__ __launch_bounds__(SB) void k_rowbuild(SortArgs a) {
 }
 
 // peers of this lane among the active lanes: lanes holding the same key
-__device__ inline uint64_t key_peers(int key, bool act, int nbits) {
+__device__ inline uint64_t key_peers_bits(int key, bool act, int nbits) {
   uint64_t peers = __ballot(act);
   for (int b = 0; b < nbits; b++) {
     const bool bit = (key >> b) & 1;
+    const uint64_t m = __ballot(act && bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+// The same mask, cheaper for the usual chunk: slots in cell order carry keys
+// within a few cells of each other, so when every active key lies within
+// [k0 - 8, k0 + 7] of the first active lane's k0, four ballots on key - k0
+// + 8 decide equality; otherwise all nbits.
+__device__ inline uint64_t key_peers(int key, bool act, int nbits) {
+  const uint64_t am = __ballot(act);
+  if (am == 0) return 0;
+  const int k0 = __builtin_amdgcn_readlane(key, __builtin_ctzll(am));
+  const int rel = key - k0 + 8;
+  if (__ballot(act && (rel < 0 || rel > 15)) != 0) return key_peers_bits(key, act, nbits);
+  uint64_t peers = am;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const bool bit = (rel >> b) & 1;
     const uint64_t m = __ballot(act && bit);
     peers &= bit ? m : ~m;
   }
