@@ -326,7 +326,7 @@ __global__ __launch_bounds__(SB) void k_colbuild3d(Sort3Args a) {
   }
 }
 
-__device__ inline uint64_t peers_of(int key, bool act, int nbits) {
+__device__ inline uint64_t peers_bits(int key, bool act, int nbits) {
   uint64_t peers = __ballot(act);
   for (int b = 0; b < nbits; b++) {
     const bool bit = (key >> b) & 1;
@@ -336,12 +336,37 @@ __device__ inline uint64_t peers_of(int key, bool act, int nbits) {
   return peers;
 }
 
-// stable counting sort of one column by x cell (sort.f90:56-81), B -> A
+// lanes whose key equals this lane's.  Slots in cell order carry keys within
+// a few cells of each other: when every active key lies within [k0 - 8,
+// k0 + 7] of the first active lane's k0, four ballots on key - k0 + 8 decide
+// equality; otherwise all nbits.
+__device__ inline uint64_t peers_of(int key, bool act, int nbits) {
+  const uint64_t am = __ballot(act);
+  if (am == 0) return 0;
+  const int k0 = __builtin_amdgcn_readlane(key, __builtin_ctzll(am));
+  const int rel = key - k0 + 8;
+  if (__ballot(act && (rel < 0 || rel > 15)) != 0) return peers_bits(key, act, nbits);
+  uint64_t peers = am;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const bool bit = (rel >> b) & 1;
+    const uint64_t msk = __ballot(act && bit);
+    peers &= bit ? msk : ~msk;
+  }
+  return peers;
+}
+
+// stable counting sort of one column by x cell (sort.f90:56-81), B -> A.
+// Each wave owns a contiguous quarter of the column: its per-key histogram,
+// then (after a scan over keys and waves) its per-key write cursors.  A round
+// takes SF chunks of 64: their loads are issued before the first is used, the
+// cursors move by LDS atomics (served in issue order), and the lowest lane of
+// each key broadcasts the key's base by one permute.
 __global__ __launch_bounds__(SB) void k_colsort3d(Sort3Args a, int nbits) {
   extern __shared__ int hist[];  // [SB/64][nx]
   __shared__ int wsum[SB / 64];
   __shared__ int carry;
-  constexpr int SU = 4, NW = SB / 64;
+  constexpr int SF = 8, NW = SB / 64;
   const Grid3D& g = a.g;
   const int nx = g.nx;
   const int rw = blockIdx.x;
@@ -353,84 +378,91 @@ __global__ __launch_bounds__(SB) void k_colsort3d(Sort3Args a, int nbits) {
   __syncthreads();
   const int lo = (int)(((long long)n * w) / NW), hi = (int)(((long long)n * (w + 1)) / NW);
   int* hw = hist + w * nx;
-  for (int k0 = lo; k0 < hi; k0 += 64 * SU) {
-    int key[SU];
-    bool act[SU];
-#pragma unroll
-    for (int u = 0; u < SU; u++) {
-      const int q = k0 + u * 64 + lane;
-      act[u] = q < hi;
-      int src = act[u] ? a.R[nb + q] : 0;
-      if ((unsigned)src >= ntot) src = 0;
-      key[u] = act[u] ? (a.code[src] >> CODE_SHIFT3) : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < SU; u++) {
-      const uint64_t peers = peers_of(key[u], act[u], nbits);
-      if (act[u] && __popcll(peers & lt) == 0) hw[key[u]] += __popcll(peers);
-    }
-  }
-  __syncthreads();
-  if (tid == 0) carry = 0;
-  __syncthreads();
   int* cc = a.cumcnt + (size_t)rw * (nx + 1);
-  for (int k0 = 0; k0 < nx; k0 += SB) {
-    const int key = k0 + tid;
-    int tot = 0;
-    if (key < nx)
-      for (int v = 0; v < NW; v++) tot += hist[v * nx + key];
-    int v = tot;
+  bool bad = false;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int k0 = lo; k0 < hi; k0 += 64 * SF) {
+      int src[SF];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int t = __shfl_up(v, off, 64);
-      if (lane >= off) v += t;
-    }
-    if (lane == 63) wsum[w] = v;
-    __syncthreads();
-    int pre = carry;
-    for (int q = 0; q < w; q++) pre += wsum[q];
-    const int excl = pre + v - tot;
-    if (key < nx) {
-      cc[key] = excl;
-      int st = excl;
-      for (int q = 0; q < NW; q++) {
-        const int h = hist[q * nx + key];
-        hist[q * nx + key] = st;
-        st += h;
+      for (int u = 0; u < SF; u++) {
+        const int q = k0 + u * 64 + lane;
+        src[u] = a.R[nb + (q < hi ? q : lo)];
+      }
+      bool act[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        const bool in = k0 + u * 64 + lane < hi;
+        act[u] = in && (unsigned)src[u] < ntot;
+        bad |= in && !act[u];
+      }
+      int key[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) key[u] = a.code[act[u] ? src[u] : 0];
+      uint64_t peers[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        key[u] = act[u] ? (key[u] >> CODE_SHIFT3) : 0;
+        peers[u] = peers_of(key[u], act[u], nbits);
+      }
+      if (pass == 0) {
+#pragma unroll
+        for (int u = 0; u < SF; u++)
+          if (act[u] && __popcll(peers[u] & lt) == 0) atomicAdd(&hw[key[u]], __popcll(peers[u]));
+      } else {
+        int old[SF];
+#pragma unroll
+        for (int u = 0; u < SF; u++) {
+          old[u] = 0;
+          if (act[u] && __popcll(peers[u] & lt) == 0) old[u] = atomicAdd(&hw[key[u]], __popcll(peers[u]));
+        }
+        int pos[SF];
+#pragma unroll
+        for (int u = 0; u < SF; u++) {
+          const int lead = act[u] ? __builtin_ctzll(peers[u]) : lane;
+          pos[u] = __shfl(old[u], lead, 64) + __popcll(peers[u] & lt);
+        }
+#pragma unroll
+        for (int u = 0; u < SF; u++)
+          if (act[u]) a.perm[nb + pos[u]] = src[u];
       }
     }
     __syncthreads();
-    if (tid == SB - 1) carry = excl + tot;
+    if (pass == 1) break;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int k0 = 0; k0 < nx; k0 += SB) {
+      const int key = k0 + tid;
+      int tot = 0;
+      if (key < nx)
+        for (int v = 0; v < NW; v++) tot += hist[v * nx + key];
+      int v = tot;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+      }
+      if (lane == 63) wsum[w] = v;
+      __syncthreads();
+      int pre = carry;
+      for (int q = 0; q < w; q++) pre += wsum[q];
+      const int excl = pre + v - tot;
+      if (key < nx) {
+        cc[key] = excl;
+        int st = excl;
+        for (int q = 0; q < NW; q++) {
+          const int h = hist[q * nx + key];
+          hist[q * nx + key] = st;
+          st += h;
+        }
+      }
+      __syncthreads();
+      if (tid == SB - 1) carry = excl + tot;
+      __syncthreads();
+    }
+    if (tid == 0) cc[nx] = n;
     __syncthreads();
   }
-  if (tid == 0) cc[nx] = n;
-  for (int k0 = lo; k0 < hi; k0 += 64 * SU) {
-    int src[SU], key[SU];
-    bool act[SU];
-#pragma unroll
-    for (int u = 0; u < SU; u++) {
-      const int q = k0 + u * 64 + lane;
-      act[u] = q < hi;
-      src[u] = act[u] ? a.R[nb + q] : 0;
-      if ((unsigned)src[u] >= ntot) {
-        atomicOr(a.err, ERR_ROW);
-        act[u] = false;
-        src[u] = 0;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < SU; u++) key[u] = act[u] ? (a.code[src[u]] >> CODE_SHIFT3) : 0;
-#pragma unroll
-    for (int u = 0; u < SU; u++) {
-      const uint64_t peers = peers_of(key[u], act[u], nbits);
-      const int rank = __popcll(peers & lt);
-      const int cnt = __popcll(peers);
-      int pos = 0;
-      if (act[u]) pos = hw[key[u]] + rank;
-      if (act[u] && rank == 0) hw[key[u]] = pos + cnt;
-      if (act[u]) a.perm[nb + pos] = src[u];
-    }
-  }
+  if (bad) atomicOr(a.err, ERR_ROW);
 }
 
 // sort__bucket on a gp that did not go through particle_yz: identity R
