@@ -255,5 +255,5 @@ def test_config4_adjoint_weibel256_200_steps_against_central_differences():
         fd.append((magnetic_energy(case, *tp) - magnetic_energy(case, *tm)) / (2 * h))
     fd = np.array(fd)
     rel = np.abs(grad - fd) / np.abs(fd)
-    _say(f"config 4: W_B {float(wb):.6g}, adjoint {grad}, central differences {fd}, relative {rel}")
+    _say(f"config 4: W_B {float(wb.detach()):.6g}, adjoint {grad}, central differences {fd}, relative {rel}")
     assert np.all(rel <= 1e-3)
