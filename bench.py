@@ -67,6 +67,9 @@ PUSH3D_BYTES_MIXED = 72
 # moments (fused kick + CIC deposit): the 40-byte record without its id read
 # once through the 4-byte permutation entry; nothing per particle is written
 MOM_BYTES = 44
+# the mixed form's: cell, two FP32 offsets and three FP32 momenta (24 bytes)
+# and the permutation entry
+MOM_BYTES_MIXED = 28
 # 3D: the 48-byte record and the permutation entry
 MOM3D_BYTES = 52
 
@@ -229,12 +232,11 @@ def mixed_run(w, a, npart, ref):
     log(f"mixed form: {a.start} + {a.warmup} untimed + {a.steps} timed steps: {elapsed:.3f} s")
     energy = sim.energy()
     uf = sim.download_fields()
-    mom = None
+    mom = mom_t = None
     if "nz" not in w:
         from wumingpic_amd import WpicError
         try:
-            sim.moments()
-            mom = sim.download_mom()
+            mom_t, mom = moments_run(sim, npart, MOM_BYTES_MIXED)
         except WpicError as e:  # a library without the mixed-form moments
             log(f"mixed-form moments unavailable: {e}")
     sim.close()
@@ -255,6 +257,8 @@ def mixed_run(w, a, npart, ref):
         "energy_rel_err_vs_f64": {k: float(abs(energy[i] - ref["energy"][i]) / abs(ref["energy"][i]))
                                   for i, k in enumerate(("kinetic", "E", "B"))},
     }
+    if mom_t is not None:
+        out["moments"] = mom_t
     if mom is not None and ref.get("mom") is not None:
         out["moment_rel_err_vs_f64"] = {f"{nm}[{sp}]": rel_err(mom[k, :, :, sp], ref["mom"][k, :, :, sp])
                                         for sp in range(mom.shape[3]) for k, nm in enumerate(MOMENT_NAMES)}

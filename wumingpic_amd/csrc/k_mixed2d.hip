@@ -328,8 +328,10 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
 // half-step kick (2d/common/mom_calc.f90:84-155, delt/2, no move) and
 // mom_calc__nvt's CIC deposit of the seven moments (:167-249) fused, FP32 per
 // particle from the int32 cell + FP32 offset record, accumulated per species
-// in an FP32 LDS node tile (one-node halo) and added to the FP64 mom array
-// with one FP64 atomic per node and moment.  The lower CIC node of a particle
+// in an FP64 LDS node tile (one-node halo; the lanes of a group share their
+// cell's nodes, and FP32 LDS atomics that meet on an address serialize far
+// worse than FP64 ones) and added to the FP64 mom array with one FP64 atomic
+// per node and moment.  The lower CIC node of a particle
 // at cell i + offset o is i when o >= 1/2 (weight o - 1/2 on i + 1), else
 // i - 1 (weight o + 1/2 on i): int(x/dx - 1/2) and x/dx - 1/2 - ih of the
 // reference, with delx = 1.
@@ -339,7 +341,7 @@ __global__ __launch_bounds__(mx::NT) void k_mom_mixed(MixArgs a, double* mom) {
   constexpr int GM = 8, NGRP = NT / GM;
   constexpr int MTW = TX + 2, MTH = TY + 2;
   __shared__ float sf[6][SFW * SFH];
-  __shared__ float mt[7][MTW * MTH];
+  __shared__ double mt[7][MTW * MTH];
   const Grid2D& g = a.g;
   const int ntiles = a.ntx * a.nty;
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(mx::NT) void k_mom_mixed(MixArgs a, double* mom) {
   const int rot = tid % 7;  // lanes meeting on a node add to different planes first
 
   for (int s = 0; s < g.nsp; s++) {
-    for (int k = tid; k < 7 * MTW * MTH; k += NT) (&mt[0][0])[k] = 0.0f;
+    for (int k = tid; k < 7 * MTW * MTH; k += NT) (&mt[0][0])[k] = 0.0;
     __syncthreads();
     const double hdelt = g.delt * 0.5;
     const float fac1 = (float)(g.q[s] / g.r[s] * 0.5 * hdelt);
@@ -410,27 +412,27 @@ __global__ __launch_bounds__(mx::NT) void k_mom_mixed(MixArgs a, double* mom) {
         const int lx = li + (hx ? 1 : 0), ly = lj + (hy ? 1 : 0);  // lower node in the tile (halo at 0)
         float mv[7] = {1.0f, ux * gi, uy * gi, uz * gi, ux * ux * gi, uy * uy * gi, uz * uz * gi};
         const float w00 = dxm * dym, w10 = dx * dym, w01 = dxm * dy, w11 = dx * dy;
-        float* node = &mt[0][ly * MTW + lx];
+        double* node = &mt[0][ly * MTW + lx];
 #pragma unroll
         for (int q = 0; q < 7; q++) {
           const int plane = q + rot < 7 ? q + rot : q + rot - 7;
           const float v = mv[plane];
-          float* p = node + plane * (MTW * MTH);
-          atomicAdd(p, v * w00);
-          atomicAdd(p + 1, v * w10);
-          atomicAdd(p + MTW, v * w01);
-          atomicAdd(p + MTW + 1, v * w11);
+          double* p = node + plane * (MTW * MTH);
+          atomicAdd(p, (double)(v * w00));
+          atomicAdd(p + 1, (double)(v * w10));
+          atomicAdd(p + MTW, (double)(v * w01));
+          atomicAdd(p + MTW + 1, (double)(v * w11));
         }
       }
     }
     __syncthreads();
     for (int k = tid; k < 7 * MTW * MTH; k += NT) {
       const int node = k / 7, comp = k % 7;
-      const float v = mt[comp][node];
-      if (v == 0.0f) continue;
+      const double v = mt[comp][node];
+      if (v == 0.0) continue;
       const int I = i0 - 1 + node % MTW, J = j0 - 1 + node / MTW;
       if (I > g.nxge + 1 || J > g.nye + 1) continue;
-      unsafeAtomicAdd(mom + midx(g, I, J, s) + comp, (double)v);
+      unsafeAtomicAdd(mom + midx(g, I, J, s) + comp, v);
     }
     __syncthreads();
   }
