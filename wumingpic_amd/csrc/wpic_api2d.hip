@@ -270,6 +270,14 @@ int sort_into_a(wpic_ctx* c, bool fast = false) {
 
 namespace wpic {
 
+// launches made while it lives go to the context's second stream
+struct OnSecondStream {
+  wpic_ctx* c;
+  hipStream_t saved;
+  explicit OnSecondStream(wpic_ctx* c_) : c(c_), saved(c_->st) { c->st = c->st2; }
+  ~OnSecondStream() { c->st = saved; }
+};
+
 int step_once(wpic_ctx* c) {
   // B shares A's slots and rows; the exchange writes rows N
   c->rb = c->ra;
@@ -280,14 +288,25 @@ int step_once(wpic_ctx* c) {
   c->b_valid = true;
   c->b_accl = false;
   c->b_aligned = true;
+  // the exchange and sort read the push's codes and rows, the field solve its
+  // current and the fields: independent, so the exchange and sort run on the
+  // second stream beside the field solve (whose CG polls only the first);
+  // the first stream waits for them before the next step
+  HIPCHK(hipEventRecord(c->ev_push, c->st));
+  HIPCHK(hipStreamWaitEvent(c->st2, c->ev_push, 0));
+  {
+    OnSecondStream on2(c);
+    // the parity mode keeps the reference's hole filling and bucket order; the
+    // fast and mixed modes put each cell's particles in an order of their own
+    const bool fast = c->mode != WPIC_MODE_PARITY;
+    rc = exchange(c, false, fast);
+    if (rc) return rc;
+    rc = sort_into_a(c, fast);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->ev_sort, c->st));
+  }
   rc = field_after_deposit(c);
-  if (rc) return rc;
-  // the parity mode keeps the reference's hole filling and bucket order; the
-  // fast and mixed modes put each cell's particles in an order of their own
-  const bool fast = c->mode != WPIC_MODE_PARITY;
-  rc = exchange(c, false, fast);
-  if (rc) return rc;
-  rc = sort_into_a(c, fast);
+  HIPCHK(hipStreamWaitEvent(c->st, c->ev_sort, 0));
   if (rc) return rc;
   c->h_rows_ok[c->ra] = false;
   return 0;
@@ -374,6 +393,9 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
 
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_push, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_sort, hipEventDisableTiming);
   const size_t N = (size_t)g.ncap * g.nsp;
   const int nr = g.nsp * ny;
   const size_t pl = (size_t)g.fw * g.fh, nc = (size_t)nx * ny;
@@ -449,6 +471,9 @@ int wpic_destroy(wpic_ctx* c) {
                 c->MB.ix, c->MB.ox, c->MB.oy, c->MB.ux, c->MB.uy, c->MB.uz, c->permbuf};
   for (void* q : ps)
     if (q) (void)hipFree(q);
+  if (c->ev_push) (void)hipEventDestroy(c->ev_push);
+  if (c->ev_sort) (void)hipEventDestroy(c->ev_sort);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return 0;

@@ -463,6 +463,9 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
 
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_push, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_sort, hipEventDisableTiming);
   const size_t N = (size_t)g.ncap * g.nsp;
   const int nr = g.nsp * g.ncol;
   const size_t pl = plane3(g), nc = (size_t)nx * ny * nz;
@@ -547,6 +550,9 @@ int wpic_destroy3d(wpic_ctx3d* c) {
                 c->MB.ix,  c->MB.ox, c->MB.oy,    c->MB.oz,   c->MB.ux,  c->MB.uy,  c->MB.uz};
   for (void* q : ps)
     if (q) (void)hipFree(q);
+  if (c->ev_push) (void)hipEventDestroy(c->ev_push);
+  if (c->ev_sort) (void)hipEventDestroy(c->ev_sort);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return 0;
@@ -737,6 +743,16 @@ int wpic_set_cg_tol3d(wpic_ctx3d* c, double err) {
   return 0;
 }
 
+namespace {
+// launches made while it lives go to the context's second stream
+struct OnSecondStream3 {
+  wpic_ctx3d* c;
+  hipStream_t saved;
+  explicit OnSecondStream3(wpic_ctx3d* c_) : c(c_), saved(c_->st) { c->st = c->st2; }
+  ~OnSecondStream3() { c->st = saved; }
+};
+}  // namespace
+
 int wpic_step3d(wpic_ctx3d* c, int nsteps) {
   if (!c) return set_error(WPIC_EINVAL, "null context");
   for (int it = 0; it < nsteps; it++) {
@@ -753,15 +769,25 @@ int wpic_step3d(wpic_ctx3d* c, int nsteps) {
       rc = deposit3(c);
       if (rc) return rc;
     }
-    rc = field_after_deposit3(c);
-    if (rc) return rc;
-    if (!c->mixed) {
-      Timed3 t(c, "exchange");
-      HIPCHK(launch_wrapx3d(c));
+    // the x wrap, exchange and sort work on gp and its columns, the field
+    // solve on the current and the fields: the former run on the second
+    // stream beside the latter, and the first stream waits for them
+    HIPCHK(hipEventRecord(c->ev_push, c->st));
+    HIPCHK(hipStreamWaitEvent(c->st2, c->ev_push, 0));
+    {
+      OnSecondStream3 on2(c);
+      if (!c->mixed) {
+        Timed3 t(c, "exchange");
+        HIPCHK(launch_wrapx3d(c));
+      }
+      rc = exchange3(c);
+      if (rc) return rc;
+      rc = sort_into_a3(c);
+      if (rc) return rc;
+      HIPCHK(hipEventRecord(c->ev_sort, c->st));
     }
-    rc = exchange3(c);
-    if (rc) return rc;
-    rc = sort_into_a3(c);
+    rc = field_after_deposit3(c);
+    HIPCHK(hipStreamWaitEvent(c->st, c->ev_sort, 0));
     if (rc) return rc;
   }
   return check_err3(c);

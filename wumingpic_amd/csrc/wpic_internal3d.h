@@ -60,6 +60,10 @@ struct wpic_ctx3d {
   wpic::Grid3D g;
   int mode;
   hipStream_t st;
+  // the step's exchange and sort run on st2 while the field solve runs on st
+  // (they share no buffer); ev_push / ev_sort order them around the push
+  hipStream_t st2;
+  hipEvent_t ev_push, ev_sort;
   int device;
 
   // particles: species s occupies [s*ncap, (s+1)*ncap); columns compact within.
