@@ -237,8 +237,17 @@ def test_fast3d_cell_changers_beyond_the_tile_queue():
         sim.step(1)
         uj = sim.download_current()
         uf_dev = sim.download_fields()
+        u_dev, n2_dev, cc_dev = sim.download_up(p.np)
     assert np.abs(uj - o.uj).max() <= 1e-13 * np.abs(o.uj).max()
     assert np.abs(uf_dev - uf_ref).max() <= 1e-12 * np.abs(uf_ref).max()
+    # the particles of the whole step (the overflowed ones wrapped after their
+    # deposit) equal the oracle's step bit for bit
+    o2 = po.Oracle3D(p)
+    up2, uf2, cc2, np22 = up.copy(order="F"), uf.copy(order="F"), cc.copy(order="F"), np2.copy(order="F")
+    o2.step(up2, np.zeros_like(up2, order="F"), uf2, cc2, np22, 1)
+    assert np.array_equal(n2_dev, np22) and np.array_equal(cc_dev, cc2)
+    ok, info = _active_equal(u_dev, up2, np22)
+    assert ok, info
 
 
 def test_weibel3d_fast_mode_close_and_conserving():

@@ -721,7 +721,7 @@ hipError_t launch_push_deposit(wpic_ctx* c, bool push, bool deposit, bool fused)
                                     : launch_t<1, true, WPIC_BC_PERIODIC>(c, push, deposit, fused, a, nblk);
   }
   if (push && deposit && fused) {
-    constexpr int GF = 16;  // a group's 16 lanes read and write whole 128-byte lines
+    constexpr int GF = 16;  // a group's 16 lanes move 128 bytes of each record array per pass (one DPP row)
     const bool wall = c->g.bc == WPIC_BC_WALLX;
     if (c->g.bc == WPIC_BC_SHOCK && a.vay)
       hipLaunchKernelGGL((k_push_deposit_fast<GF, true, WPIC_BC_SHOCK, true>), dim3(nblk), dim3(NT), 0, c->st, a);

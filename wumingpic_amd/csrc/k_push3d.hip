@@ -37,6 +37,8 @@ struct Push3Args {
   double* pjbuf;
   int ntx, nty, ntz;
   int vay;  // the Vay kick instead of Buneman-Boris (particle__solv_vay)
+  int* code;  // fused fast kernel: the exchange codes of gp
+  int* err;
 };
 
 __global__ __launch_bounds__(p3::NT) void k_push3d(Push3Args a) {
@@ -557,9 +559,6 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
       else
         boris_push3(sf, so, i, j, k, d_delx, delt, c, kc[0], kc[1], kc[2], x, y, z, ux, uy, uz, nw[0], nw[1], nw[2],
                     g4, g5, g6, sh);
-      a.B.x[pc] = nw[0];
-      a.B.y[pc] = nw[1];
-      a.B.z[pc] = nw[2];
       a.B.ux[pc] = g4;
       a.B.uy[pc] = g5;
       a.B.uz[pc] = g6;
@@ -577,6 +576,7 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
         inc[d] = i2 - cell[d];
         shape3(dh1[d], s1[d][0], s1[d][1], s1[d][2]);
       }
+      bool over = false;
       if ((inc[0] | inc[1] | inc[2]) == 0) {
         // nodes -1..1 of each axis: s0, ds = s1 - s0, a = s0 + ds/2, b = s0/2 + ds/3
         double s0[3][3], ds[3][3], A[3][3], B[3][3], cq[3][2];
@@ -619,12 +619,24 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
           }
           xqc[slot] = cc | (sc << 7) | ((inc[0] + 1) << 9) | ((inc[1] + 1) << 11) | ((inc[2] + 1) << 13);
         } else {
-          // beyond the LDS queue: listed for k_deposit3d_overflow
+          // beyond the LDS queue: listed for k_deposit3d_overflow, which
+          // reads its new position unwrapped and wraps it after the deposit
           const int o = atomicAdd(a.nov, 1);
           a.ovslot[o] = pc;
           a.ovcell[o] = (int)cidx3(g, i, j, k) | (sc << 28);
+          over = true;
         }
       }
+      // the x wrap, the y-z wraps and the exchange code: particle_x and
+      // particle_yz act on gp after the deposit, which uses nw unwrapped
+      double wx = nw[0], wy = nw[1], wz = nw[2];
+      if (!over) {
+        bool moved;
+        a.code[pc] = wrap3_code<true>(g, wx, wy, wz, j, k, a.err, moved);
+      }
+      a.B.x[pc] = wx;
+      a.B.y[pc] = wy;
+      a.B.z[pc] = wz;
 
     }
     // reduce the stayers' current over the group, spread the LDS atomics
@@ -694,6 +706,13 @@ __global__ __launch_bounds__(256) void k_deposit3d_overflow(Push3Args a) {
     deposit3_window(g, a.A, a.B, a.ap, pc, cell, qd, [&](int comp, int xo, int yo, int zo, double v) {
       unsafeAtomicAdd(a.uj + pl * comp + fidx3(g, cell[0] + xo, cell[1] + yo, cell[2] + zo), v);
     });
+    // the wraps and the code the fused kernel left to this deposit
+    double x = a.B.x[pc], y = a.B.y[pc], z = a.B.z[pc];
+    bool moved;
+    a.code[pc] = wrap3_code<true>(g, x, y, z, cell[1], cell[2], a.err, moved);
+    a.B.x[pc] = x;
+    a.B.y[pc] = y;
+    a.B.z[pc] = z;
   }
 }
 
@@ -714,6 +733,8 @@ static Push3Args push_args(wpic_ctx3d* c) {
   a.nty = (c->g.ny + p3::TY - 1) / p3::TY;
   a.ntz = (c->g.nz + p3::TZ - 1) / p3::TZ;
   a.vay = c->pusher == WPIC_PUSHER_VAY;
+  a.code = c->code;
+  a.err = c->err;
   return a;
 }
 

@@ -76,28 +76,15 @@ __global__ __launch_bounds__(SB) void k_colscan3d(Sort3Args a, int wrap) {
     if (q < n) {
       const int p = base + q;
       if (wrap) {
-        const double y = a.B.y[p], z = a.B.z[p];
-        const int jpos = (int)(y * g.d_delx), kpos = (int)(z * g.d_delx);
-        int dj = jpos - j, dk = kpos - k;
-        if (!(jpos == j && kpos == k)) {
-          if (jpos <= g.nygs - 1)
-            a.B.y[p] = y + g.spany;
-          else if (jpos >= g.nyge + 1)
-            a.B.y[p] = y - g.spany;
-          if (kpos <= g.nzgs - 1)
-            a.B.z[p] = z + g.spanz;
-          else if (kpos >= g.nzge + 1)
-            a.B.z[p] = z - g.spanz;
+        double x = a.B.x[p], y = a.B.y[p], z = a.B.z[p];
+        bool moved;
+        const int code = wrap3_code<false>(g, x, y, z, j, k, a.err, moved);
+        if (moved) {
+          a.B.y[p] = y;
+          a.B.z[p] = z;
         }
-        int ni = (int)a.B.x[p];
-        if (ni < g.nxgs || ni > g.nxge || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
-          atomicOr(a.err, ERR_DOMAIN);
-          ni = ni < g.nxgs ? g.nxgs : (ni > g.nxge ? g.nxge : ni);
-          dj = dj < -1 ? -1 : (dj > 1 ? 1 : dj);
-          dk = dk < -1 ? -1 : (dk > 1 ? 1 : dk);
-        }
-        dir = (dk + 1) * 3 + (dj + 1);
-        a.code[p] = ((ni - g.nxgs) << CODE_SHIFT3) | dir;
+        a.code[p] = code;
+        dir = code & 15;
       } else {
         dir = a.code[p] & 15;
       }

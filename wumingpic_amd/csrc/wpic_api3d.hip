@@ -219,14 +219,16 @@ int push3(wpic_ctx3d* c) {
     HIPCHK(launch_push3d(c));
   }
   c->b_valid = true;
+  c->b_coded = false;
   c->b_accl = false;
   c->b_aligned = true;
   c->exch_pending = false;
   return 0;
 }
 
-// fast and mixed modes: particle__solv and ele_cur in one fused pass (the
-// mixed one also writes the exchange codes with the x cell wrapped)
+// fast and mixed modes: particle__solv and ele_cur in one fused pass, which
+// also writes the exchange codes with the x cell wrapped (and, FP64, the x,
+// y and z wraps of gp: particle_x and particle_yz's position changes)
 int push_deposit3(wpic_ctx3d* c) {
   c->cb = c->ca;
   if (c->cn == c->ca) c->cn = pick_free3(c->ca, c->cb);
@@ -242,13 +244,14 @@ int push_deposit3(wpic_ctx3d* c) {
   c->b_valid = true;
   c->b_accl = false;
   c->b_aligned = true;
+  c->b_coded = true;
   c->exch_pending = false;
   return 0;
 }
 
 int exchange3(wpic_ctx3d* c) {
   Timed3 t(c, "exchange");
-  HIPCHK(launch_colscan3d(c, c->mixed ? 0 : 1));
+  HIPCHK(launch_colscan3d(c, c->b_coded ? 0 : 1));
   HIPCHK(launch_colcount3d(c));
   HIPCHK(launch_colbuild3d(c));
   c->h_cols_ok[c->cn] = false;
@@ -614,6 +617,7 @@ int wpic_upload_gp3d(wpic_ctx3d* c, const double* gp, const int* np2, int np) {
   int rc = upload_cols(c, c->cb, gp, np2, np, 1);
   if (rc) return rc;
   c->b_valid = true;
+  c->b_coded = false;
   c->b_accl = false;
   c->exch_pending = false;
   rc = fetch_cols(c, c->ca);
@@ -776,7 +780,7 @@ int wpic_step3d(wpic_ctx3d* c, int nsteps) {
     HIPCHK(hipStreamWaitEvent(c->st2, c->ev_push, 0));
     {
       OnSecondStream3 on2(c);
-      if (!c->mixed) {
+      if (!c->b_coded) {
         Timed3 t(c, "exchange");
         HIPCHK(launch_wrapx3d(c));
       }
@@ -974,6 +978,7 @@ int wpic_mom_accl3d(wpic_ctx3d* c) {
     HIPCHK(launch_mom_accl3d(c));
   }
   c->b_valid = true;
+  c->b_coded = false;
   c->b_accl = true;  // gp holds up's positions: sorted by cell like up
   c->b_aligned = true;
   c->exch_pending = false;

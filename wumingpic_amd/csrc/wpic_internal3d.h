@@ -49,6 +49,45 @@ struct PartsM3 {
 // (new x cell - nxgs) << 4 | dir.
 constexpr int DIR_STAY = 4;
 constexpr int CODE_SHIFT3 = 4;
+
+// The boundaries a pushed FP64 particle of column (j, k) meets before the
+// sort, and its exchange code: (WRAPX) boundary_periodic__particle_x
+// (3d/common/boundary_periodic.f90:80-95, ipos = int(x*d_delx)), then
+// particle_yz's y and z wraps (:147-165) and the destination.  A step beyond
+// one column or an x cell outside the box flags ERR_DOMAIN and is clamped.
+// Returns the code; `moved` says whether y or z left the column.
+template <bool WRAPX>
+__device__ inline int wrap3_code(const Grid3D& g, double& x, double& y, double& z, int j, int k, int* err,
+                                 bool& moved) {
+  if (WRAPX) {
+    const int ipos = (int)(x * g.d_delx);
+    if (ipos < g.nxgs)
+      x = x + g.spanx;
+    else if (ipos >= g.nxge + 1)
+      x = x - g.spanx;
+  }
+  const int jpos = (int)(y * g.d_delx), kpos = (int)(z * g.d_delx);
+  int dj = jpos - j, dk = kpos - k;
+  moved = !(jpos == j && kpos == k);
+  if (moved) {
+    if (jpos <= g.nygs - 1)
+      y = y + g.spany;
+    else if (jpos >= g.nyge + 1)
+      y = y - g.spany;
+    if (kpos <= g.nzgs - 1)
+      z = z + g.spanz;
+    else if (kpos >= g.nzge + 1)
+      z = z - g.spanz;
+  }
+  int ni = (int)x;
+  if (ni < g.nxgs || ni > g.nxge || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
+    atomicOr(err, ERR_DOMAIN);
+    ni = ni < g.nxgs ? g.nxgs : (ni > g.nxge ? g.nxge : ni);
+    dj = dj < -1 ? -1 : (dj > 1 ? 1 : dj);
+    dk = dk < -1 ? -1 : (dk > 1 ? 1 : dk);
+  }
+  return ((ni - g.nxgs) << CODE_SHIFT3) | ((dk + 1) * 3 + (dj + 1));
+}
 // CG partial-sum blocks in 3D: enough blocks to cover 256^3 cells with
 // few cells per thread; the partials are summed in a fixed order
 constexpr int NPB3 = 2048;
@@ -90,6 +129,7 @@ struct wpic_ctx3d {
   int* seg;     // [ncol][8] arrival segments (source column << 4 | dir), in order
   int* nseg;    // [ncol]
   bool b_valid, b_aligned, exch_pending;
+  bool b_coded;  // gp holds its exchange codes and its x, y, z wraps (the fused push wrote them)
   int pusher;   // WPIC_PUSHER_BORIS or WPIC_PUSHER_VAY
   bool b_accl;  // B is mom_calc__accl's gp: A's positions, A's cell order
   std::vector<int> h_cols[3];
