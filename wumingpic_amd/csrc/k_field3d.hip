@@ -322,8 +322,19 @@ __global__ __launch_bounds__(256) void k_cg3_fin1(Cg3Args a) {
 
 // p = r + bv p (the previous iteration's update, fused); ap = -p(k-1) - p(j-1)
 // - p(i-1) + f4 p - p(i+1) - p(j+1) - p(k+1); sum r^2 and p.ap (:483-505)
+// p = r + bv p (the previous iteration's update, fused here), then ap = f4 p -
+// sum of the six neighbours; sum r^2 and sum p.ap (3d/common/field.f90's cgm).
+// A block marches a tile of 32 x 8 columns through a slab of z levels: p at
+// levels k-1, k, k+1 of a column stay in registers, level k of the tile (with
+// its one-column halo) goes through LDS, so r and the old p are read from HBM
+// once per node.  The work units (tile, slab) are dealt to the NPB3 blocks;
+// a block with none writes zero partials.  Every value is the per-node
+// expression of the reference's order, so pn and ap do not depend on the
+// tiling (the fast mode's partial sums do, in a fixed order).
 __global__ __launch_bounds__(256) void k_cg3_a(Cg3Args a) {
+  constexpr int TXC = 32, TYC = 8, SW3 = TXC + 2;
   __shared__ double sh[4];
+  __shared__ double sm[(TYC + 2) * SW3];
   const int l = blockIdx.y;
   if (!a.sc[l].active) return;
   const Grid3D& g = a.g;
@@ -332,21 +343,49 @@ __global__ __launch_bounds__(256) void k_cg3_a(Cg3Args a) {
   const double *r = a.r + (size_t)l * n, *po = a.pold + (size_t)l * n;
   double* pn = a.pnew + (size_t)l * n;
   double* ap = a.ap + (size_t)l * n;
+  const int ntx = (g.nx + TXC - 1) / TXC, nty = (g.ny + TYC - 1) / TYC, ntile = ntx * nty;
+  int nsl = NPB3 / ntile;
+  if (nsl < 1) nsl = 1;
+  if (nsl > g.nz) nsl = g.nz;
+  const int len = (g.nz + nsl - 1) / nsl;
+  const int units = ntile * nsl;
+  const int tx = threadIdx.x % TXC, ty = threadIdx.x / TXC;
   double s_r = 0.0, s_2 = 0.0;
-  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)NPB3 * 256) {
-    int i, j, k;
-    decode3(g, q, i, j, k);
+  for (int u = blockIdx.x; u < units; u += NPB3) {
+    const int sl = u / ntile, t = u - sl * ntile;
+    const int ks = g.nzs + sl * len;
+    const int ke = ks + len - 1 < g.nze ? ks + len - 1 : g.nze;
+    if (ks > g.nze) continue;  // uniform in the block
+    const int i = g.nxgs + (t % ntx) * TXC + tx, j = g.nys + (t / ntx) * TYC + ty;
+    const bool act = i <= g.nxge && j <= g.nye;
     size_t m;
 #define P(ii, jj, kk) (m = nbr3(g, ii, jj, kk), r[m] + bv * po[m])
-    const double pB = P(i, j, k - 1), pS = P(i, j - 1, k), pW = P(i - 1, j, k);
-    const double pE = P(i + 1, j, k), pN = P(i, j + 1, k), pT = P(i, j, k + 1);
+    double pB = act ? P(i, j, ks - 1) : 0.0, pC = act ? P(i, j, ks) : 0.0;
+    for (int k = ks; k <= ke; k++) {
+      const double pT = act ? P(i, j, k + 1) : 0.0;
+      if (act) {
+        sm[(ty + 1) * SW3 + tx + 1] = pC;
+        if (tx == 0) sm[(ty + 1) * SW3] = P(i - 1, j, k);
+        if (tx == TXC - 1 || i == g.nxge) sm[(ty + 1) * SW3 + tx + 2] = P(i + 1, j, k);
+        if (ty == 0) sm[tx + 1] = P(i, j - 1, k);
+        if (ty == TYC - 1 || j == g.nye) sm[(ty + 2) * SW3 + tx + 1] = P(i, j + 1, k);
+      }
+      __syncthreads();
+      if (act) {
+        const double pS = sm[ty * SW3 + tx + 1], pW = sm[(ty + 1) * SW3 + tx];
+        const double pE = sm[(ty + 1) * SW3 + tx + 2], pN = sm[(ty + 2) * SW3 + tx + 1];
+        const double av = -pB - pS - pW + f4 * pC - pE - pN - pT;
+        const size_t q = cidx3(g, i, j, k);
+        pn[q] = pC;
+        ap[q] = av;
+        s_r = s_r + r[q] * r[q];
+        s_2 = s_2 + pC * av;
+      }
+      __syncthreads();
+      pB = pC;
+      pC = pT;
+    }
 #undef P
-    const double pC = r[q] + bv * po[q];
-    const double av = -pB - pS - pW + f4 * pC - pE - pN - pT;
-    pn[q] = pC;
-    ap[q] = av;
-    s_r = s_r + r[q] * r[q];
-    s_2 = s_2 + pC * av;
   }
   if (!a.parity) {
     const double t1 = bsum256(s_r, sh);
