@@ -338,8 +338,8 @@ __global__ __launch_bounds__(m3::NT, 3) void k_push_deposit_mixed3d(Mix3Args a) 
 #pragma unroll
     for (int q = 0; q < 54; q++) {
       float v = q < 18 ? jx[q] : (q < 36 ? jy[q - 18] : jz[q - 36]);
-#pragma unroll
-      for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, G);
+      static_assert(G == 8, "the group is one half of a DPP row");
+      v = row8_sum(v);
       if ((q % G) == lg && v != 0.0f) {
         const int comp = q / 18, rr = q % 18, m = rr / 9, u = (rr / 3) % 3, w = rr % 3;
         int xo, yo, zo;

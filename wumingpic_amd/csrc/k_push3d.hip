@@ -545,12 +545,44 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
     const int so = (lk + 1) * (SW * SH) + (lj + 1) * SW + (li + 1);
     const int bx = li + 2, by = lj + 2, bz = lk + 2;  // the cell in the current tile
 
+    // two-deep load pipeline: the record of element t + G and the
+    // permutation entry of element t + 2G are in flight while element t is
+    // pushed
+    int ia_nn = 0;
+    double px = 0.0, py = 0.0, pz = 0.0, pux = 0.0, puy = 0.0, puz = 0.0;
+    uint64_t pid = 0;
+    {
+      int s_, p_, ia_n = 0;
+      if (lg < ntot) {
+        cl.slot(lg, s_, p_);
+        ia_n = a.ap ? a.ap[p_] : p_;
+      }
+      if (lg + GF < ntot) {
+        cl.slot(lg + GF, s_, p_);
+        ia_nn = a.ap ? a.ap[p_] : p_;
+      }
+      if (lg < ntot) {
+        px = a.A.x[ia_n]; py = a.A.y[ia_n]; pz = a.A.z[ia_n];
+        pux = a.A.ux[ia_n]; puy = a.A.uy[ia_n]; puz = a.A.uz[ia_n];
+        pid = a.A.id[ia_n];
+      }
+    }
     for (int t = lg; t < ntot; t += GF) {
       int sc, pc;
       cl.slot(t, sc, pc);
-      const int ia = a.ap ? a.ap[pc] : pc;
-      const double x = a.A.x[ia], y = a.A.y[ia], z = a.A.z[ia];
-      const double ux = a.A.ux[ia], uy = a.A.uy[ia], uz = a.A.uz[ia];
+      const double x = px, y = py, z = pz, ux = pux, uy = puy, uz = puz;
+      const uint64_t id = pid;
+      if (t + GF < ntot) {
+        const int ia = ia_nn;
+        px = a.A.x[ia]; py = a.A.y[ia]; pz = a.A.z[ia];
+        pux = a.A.ux[ia]; puy = a.A.uy[ia]; puz = a.A.uz[ia];
+        pid = a.A.id[ia];
+      }
+      if (t + 2 * GF < ntot) {
+        int s_, p_;
+        cl.slot(t + 2 * GF, s_, p_);
+        ia_nn = a.ap ? a.ap[p_] : p_;
+      }
       const double* kc = spc[sc];
       double nw[3], g4, g5, g6, sh[9];
       if (VAY)
@@ -562,7 +594,7 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
       a.B.ux[pc] = g4;
       a.B.uy[pc] = g5;
       a.B.uz[pc] = g6;
-      a.B.id[pc] = a.A.id[ia];
+      a.B.id[pc] = id;
 
       // new shape and cell increment per axis (field.f90:253-329)
       const double qd = kc[3];
@@ -643,8 +675,8 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
 #pragma unroll
     for (int q = 0; q < 54; q++) {
       double v = q < 18 ? jx[q] : (q < 36 ? jy[q - 18] : jz[q - 36]);
-#pragma unroll
-      for (int off = GF / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, GF);
+      static_assert(GF == 8, "the group is one half of a DPP row");
+      v = row8_sum(v);
       if ((q % GF) == lg && v != 0.0) {
         const int comp = q / 18, r = q % 18, m = r / 9, u = (r / 3) % 3, w = r % 3;
         int xo, yo, zo;

@@ -218,6 +218,24 @@ __device__ inline T row16_sum(T v) {
   return v;
 }
 
+// Sum over the 8 lanes of a half DPP row, every lane of the half receiving
+// it: quad permutes for the partners 1 and 2 apart, then the half-row mirror
+// (lane l and 7 - l), which pairs each quad with the other.
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;
+template <class T>
+__device__ inline T row8_sum(T v) {
+  if constexpr (sizeof(T) == 8) {
+    v += dpp_f64<DPP_QUAD_XOR1>(v);
+    v += dpp_f64<DPP_QUAD_XOR2>(v);
+    v += dpp_f64<DPP_ROW_HALF_MIRROR>(v);
+  } else {
+    v += dpp_f32<DPP_QUAD_XOR1>(v);
+    v += dpp_f32<DPP_QUAD_XOR2>(v);
+    v += dpp_f32<DPP_ROW_HALF_MIRROR>(v);
+  }
+  return v;
+}
+
 // Slots in an LDS queue for the active lanes of a wave with one atomic on the
 // counter *n: the active lanes get consecutive slots in lane order.
 __device__ inline int wave_enqueue(int* n) {
