@@ -10,8 +10,8 @@
 // the FP64 uj with FP64 atomics and the field solve is the FP64 one.  The work mapping is the FP64
 // fused kernel's (k_push_deposit3d_fast): 8x4x4-cell tiles, 8 lanes per cell,
 // the 54 current values of a particle that stays in its cell in registers, a
-// particle that changes cell queued with its offsets and added with its
-// 3 x 4 x 4 window per component after each round of cells.  The push writes the exchange code
+// particle that changes cell queued (per wave) with its offsets and added with
+// its 3 x 4 x 4 window per component after the wave's round of cells.  The push writes the exchange code
 // ((new x cell - nxgs) << 4 | y-z direction) itself: the x wrap is a cell
 // wrap and the y-z offsets are already relative to the destination column.
 #include "wpic_internal3d.h"
@@ -25,7 +25,7 @@ namespace m3 {
 constexpr int TX = 8, TY = 4, TZ = 4, NT = 256, G = 8;
 constexpr int SW = TX + 2, SH = TY + 2, SD = TZ + 2, SN = SW * SH * SD;  // staged tmpf
 constexpr int AW = TX + 4, AH = TY + 4, AD = TZ + 4, AN = AW * AH * AD;  // current tile
-constexpr int NXQ = 512;   // queued cell-changing particles per round (beyond: added at once)
+constexpr int QW = 128;    // queued cell-changing particles per wave and round (beyond: added at once)
 }  // namespace m3
 
 struct Mix3Args {
@@ -200,11 +200,13 @@ __global__ __launch_bounds__(m3::NT, 3) void k_push_deposit_mixed3d(Mix3Args a) 
   __shared__ float sf[6][SN];
   __shared__ double acc[3][AN];
   __shared__ float spc[WPIC_MAXSP][4];  // fac1, txxx, fac2, q delx / delt
-  // this round's cell-changing particles: old and new in-cell offsets, and
-  // cell in the tile | species << 7 | (inc + 1) per axis << 9, 11, 13
-  __shared__ float xqo[NXQ][6];
-  __shared__ int xqc[NXQ];
-  __shared__ int nxq[2];
+  // each wave's cell-changing particles of the round: old and new in-cell
+  // offsets, and cell in the tile | species << 7 | (inc + 1) per axis << 9,
+  // 11, 13
+  constexpr int NW = NT / 64;
+  __shared__ float xqo[NW][QW][6];
+  __shared__ int xqc[NW][QW];
+  __shared__ int nxq[NW];
   const Grid3D& g = a.g;
   const int ntiles = a.ntx * a.nty * a.ntz;
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(m3::NT, 3) void k_push_deposit_mixed3d(Mix3Args a) 
     if (i <= g.nxge + 1 && j <= g.nye + 1 && k <= g.nze + 1) sf[c][rem] = (float)tmpf3m(g, a.uf, c, i, j, k);
   }
   for (int q = tid; q < 3 * AN; q += NT) (&acc[0][0])[q] = 0.0f;
-  if (tid == 0) nxq[0] = nxq[1] = 0;
+  if (tid < NW) nxq[tid] = 0;
 #pragma unroll
   for (int s = 0; s < WPIC_MAXSP; s++) {
     if (tid == s && s < g.nsp) {
@@ -237,9 +239,11 @@ __global__ __launch_bounds__(m3::NT, 3) void k_push_deposit_mixed3d(Mix3Args a) 
   mc.delt = (float)g.delt;
   mc.ic2 = (float)(1.0 / (g.c * g.c));
   mc.cc2 = (float)(g.c * g.c);
-  // rounds of NGRP cells (one per group), each followed by the queue's drain
+  const int w = tid >> 6, lane = tid & 63;
+  int* const nq_r = &nxq[w];
+  // rounds of NGRP cells (one per group, 64 / G per wave), each wave
+  // draining its own queue after its cells of the round
   for (int rnd = 0; rnd < (TX * TY * TZ) / NGRP; rnd++) {
-    int* const nq_r = &nxq[rnd & 1];
     const int cc = rnd * NGRP + grp;
     const int li = cc % TX, lj = (cc / TX) % TY, lk = cc / (TX * TY);
     const int i = i0 + li, j = j0 + lj, k = k0 + lk;
@@ -306,13 +310,13 @@ __global__ __launch_bounds__(m3::NT, 3) void k_push_deposit_mixed3d(Mix3Args a) 
             }
         } else {
           const int slot = wave_enqueue(nq_r);
-          if (slot < NXQ) {
+          if (slot < QW) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-              xqo[slot][d] = o[d];
-              xqo[slot][3 + d] = r.f[d];
+              xqo[w][slot][d] = o[d];
+              xqo[w][slot][3 + d] = r.f[d];
             }
-            xqc[slot] = cc | (s << 7) | ((r.inc[0] + 1) << 9) | ((r.inc[1] + 1) << 11) | ((r.inc[2] + 1) << 13);
+            xqc[w][slot] = cc | (s << 7) | ((r.inc[0] + 1) << 9) | ((r.inc[1] + 1) << 11) | ((r.inc[2] + 1) << 13);
           } else {
             deposit3_window_m(r, qd, [&](int comp, int xo, int yo, int zo, float v) {
               tile_add(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
@@ -341,40 +345,49 @@ __global__ __launch_bounds__(m3::NT, 3) void k_push_deposit_mixed3d(Mix3Args a) 
       static_assert(G == 8, "the group is one half of a DPP row");
       v = row8_sum(v);
       if ((q % G) == lg && v != 0.0f) {
-        const int comp = q / 18, rr = q % 18, m = rr / 9, u = (rr / 3) % 3, w = rr % 3;
+        const int comp = q / 18, rr = q % 18, m = rr / 9, u = (rr / 3) % 3, b = rr % 3;
         int xo, yo, zo;
         if (comp == 0) {
-          xo = m; yo = u - 1; zo = w - 1;
+          xo = m; yo = u - 1; zo = b - 1;
         } else if (comp == 1) {
-          xo = u - 1; yo = m; zo = w - 1;
+          xo = u - 1; yo = m; zo = b - 1;
         } else {
-          xo = u - 1; yo = w - 1; zo = m;
+          xo = u - 1; yo = b - 1; zo = m;
         }
         tile_add(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
       }
     }
     }  // the cell
 
-    __syncthreads();
-    // this round's queued particles that changed cell, from their offsets
-    const int nq = *nq_r < NXQ ? *nq_r : NXQ;
-    for (int q = tid; q < nq; q += NT) {
-      const int e = xqc[q];
+    // the wave's queued particles of the round, from their offsets: the queue
+    // was written by this wave's own lanes, and LDS operations of one wave
+    // complete in order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int nq = *nq_r < QW ? *nq_r : QW;
+    for (int q = lane; q < nq; q += 64) {
+      const int e = xqc[w][q];
       const int qcc = e & 127, s = (e >> 7) & 3;
       const int qbx = qcc % TX + 2, qby = (qcc / TX) % TY + 2, qbz = qcc / (TX * TY) + 2;
       Mix3Push r;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        shape3m(xqo[q][d] - 0.5f, r.sh[3 * d], r.sh[3 * d + 1], r.sh[3 * d + 2]);
-        r.f[d] = xqo[q][3 + d];
+        shape3m(xqo[w][q][d] - 0.5f, r.sh[3 * d], r.sh[3 * d + 1], r.sh[3 * d + 2]);
+        r.f[d] = xqo[w][q][3 + d];
         r.inc[d] = ((e >> (9 + 2 * d)) & 3) - 1;
       }
       deposit3_window_m(r, spc[s][3], [&](int comp, int xo, int yo, int zo, float v) {
         tile_add(&acc[comp][((qbz + zo) * AH + (qby + yo)) * AW + (qbx + xo)], v);
       });
     }
-    if (tid == 0) nxq[(rnd + 1) & 1] = 0;
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) *nq_r = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 
   __syncthreads();

@@ -463,12 +463,13 @@ __device__ inline void deposit3_window_dh(const double dh0[3], const double dh1[
 // current lives on 2 x 3 x 3 nodes per component: every lane accumulates
 // those 54 values in registers over the particles of the cell it handles and
 // the lane group reduces them once per cell.  A particle that changes cell is
-// queued in LDS; after the tile's cells, all lanes add the queued particles'
-// 3 x 4 x 4 windows per component (deposit3_window) to the LDS tile.  The
-// tile is added to uj with one HBM atomic per node.
+// queued in LDS, private to the wave; after the wave's cells of a round, its
+// lanes add the queued particles' 3 x 4 x 4 windows per component
+// (deposit3_window) to the LDS tile.  The tile is added to uj with one HBM
+// atomic per node.
 namespace p3f {
-constexpr int G = 8;       // lanes per cell
-constexpr int NXQ = 512;   // queued cell-changing particles per round (beyond: a global list)
+constexpr int G = 8;     // lanes per cell
+constexpr int QW = 128;  // queued cell-changing particles per wave and round (beyond: a global list)
 }
 
 template <bool VAY>
@@ -478,12 +479,14 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
   __shared__ double sf[6][SN];
   __shared__ double acc[3][AN];
   __shared__ double spc[WPIC_MAXSP][4];  // fac1, txxx, fac2, q delx / delt
-  // this round's cell-changing particles: old and new shape arguments per
-  // axis, and cell in the tile | species << 7 | (inc + 1) per axis << 9, 11, 13
-  __shared__ double xqd[p3f::NXQ][6];
-  __shared__ int xqc[p3f::NXQ];
-  __shared__ int nxq[2];
-  using p3f::NXQ;
+  // each wave's cell-changing particles of the round: old and new shape
+  // arguments per axis, and cell in the tile | species << 7 | (inc + 1) per
+  // axis << 9, 11, 13
+  constexpr int NW = NT / 64;
+  using p3f::QW;
+  __shared__ double xqd[NW][QW][6];
+  __shared__ int xqc[NW][QW];
+  __shared__ int nxq[NW];
   const Grid3D& g = a.g;
   const int ntiles = a.ntx * a.nty * a.ntz;
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -497,7 +500,7 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
     if (i <= g.nxge + 1 && j <= g.nye + 1 && k <= g.nze + 1) sf[c][rem] = tmpf3(g, a.uf, c, i, j, k);
   }
   for (int q = tid; q < 3 * AN; q += NT) (&acc[0][0])[q] = 0.0;
-  if (tid == 0) nxq[0] = nxq[1] = 0;
+  if (tid < NW) nxq[tid] = 0;
 #pragma unroll
   for (int s = 0; s < WPIC_MAXSP; s++) {
     if (tid == s && s < g.nsp) {
@@ -514,9 +517,11 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
   const int grp = tid / GF, lg = tid % GF;
   constexpr int NGRP = NT / GF;
   const double d_delx = g.d_delx, delt = g.delt, c = g.c;
-  // rounds of NGRP cells (one per group), each followed by the queue's drain
+  const int w = tid >> 6, lane = tid & 63;
+  int* const nq_r = &nxq[w];
+  // rounds of NGRP cells (one per group, 64 / G per wave), each wave
+  // draining its own queue after its cells of the round
   for (int rnd = 0; rnd < (TX * TY * TZ) / NGRP; rnd++) {
-    int* const nq_r = &nxq[rnd & 1];
     const int cc = rnd * NGRP + grp;
     const int li = cc % TX, lj = (cc / TX) % TY, lk = cc / (TX * TY);
     const int i = i0 + li, j = j0 + lj, k = k0 + lk;
@@ -642,14 +647,14 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
       } else {
         // a particle that changes cell: its window is added after the cell loop
         const int slot = wave_enqueue(nq_r);
-        if (slot < NXQ) {
+        if (slot < QW) {
           const double o[3] = {x, y, z};
 #pragma unroll
           for (int d = 0; d < 3; d++) {
-            xqd[slot][d] = o[d] * d_delx - 5e-1 - cell[d];
-            xqd[slot][3 + d] = dh1[d];
+            xqd[w][slot][d] = o[d] * d_delx - 5e-1 - cell[d];
+            xqd[w][slot][3 + d] = dh1[d];
           }
-          xqc[slot] = cc | (sc << 7) | ((inc[0] + 1) << 9) | ((inc[1] + 1) << 11) | ((inc[2] + 1) << 13);
+          xqc[w][slot] = cc | (sc << 7) | ((inc[0] + 1) << 9) | ((inc[1] + 1) << 11) | ((inc[2] + 1) << 13);
         } else {
           // beyond the LDS queue: listed for k_deposit3d_overflow, which
           // reads its new position unwrapped and wraps it after the deposit
@@ -678,37 +683,46 @@ __global__ __launch_bounds__(p3::NT, 2) void k_push_deposit3d_fast(Push3Args a) 
       static_assert(GF == 8, "the group is one half of a DPP row");
       v = row8_sum(v);
       if ((q % GF) == lg && v != 0.0) {
-        const int comp = q / 18, r = q % 18, m = r / 9, u = (r / 3) % 3, w = r % 3;
+        const int comp = q / 18, r = q % 18, m = r / 9, u = (r / 3) % 3, b = r % 3;
         int xo, yo, zo;
         if (comp == 0) {
-          xo = m; yo = u - 1; zo = w - 1;
+          xo = m; yo = u - 1; zo = b - 1;
         } else if (comp == 1) {
-          xo = u - 1; yo = m; zo = w - 1;
+          xo = u - 1; yo = m; zo = b - 1;
         } else {
-          xo = u - 1; yo = w - 1; zo = m;
+          xo = u - 1; yo = b - 1; zo = m;
         }
         atomicAdd(&acc[comp][((bz + zo) * AH + (by + yo)) * AW + (bx + xo)], v);
       }
     }
     }  // the cell
 
-    __syncthreads();
-    // this round's queued particles that changed cell; thread t drains entries
-    // t, t + NT, ...
-    const int nq = *nq_r < NXQ ? *nq_r : NXQ;
-    for (int q = tid; q < nq; q += NT) {
-      const int e = xqc[q];
+    // the wave's queued particles of the round: the queue was written by this
+    // wave's own lanes, and LDS operations of one wave complete in order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int nq = *nq_r < QW ? *nq_r : QW;
+    for (int q = lane; q < nq; q += 64) {
+      const int e = xqc[w][q];
       const int qcc = e & 127, s = (e >> 7) & 3;
       const int inc[3] = {((e >> 9) & 3) - 1, ((e >> 11) & 3) - 1, ((e >> 13) & 3) - 1};
       const int qbx = qcc % TX + 2, qby = (qcc / TX) % TY + 2, qbz = qcc / (TX * TY) + 2;
-      const double dh0[3] = {xqd[q][0], xqd[q][1], xqd[q][2]}, dh1[3] = {xqd[q][3], xqd[q][4], xqd[q][5]};
+      const double dh0[3] = {xqd[w][q][0], xqd[w][q][1], xqd[w][q][2]};
+      const double dh1[3] = {xqd[w][q][3], xqd[w][q][4], xqd[w][q][5]};
       deposit3_window_dh(dh0, dh1, inc, spc[s][3], [&](int comp, int xo, int yo, int zo, double v) {
         atomicAdd(&acc[comp][((qbz + zo) * AH + (qby + yo)) * AW + (qbx + xo)], v);
       });
     }
-    if (tid == 0) nxq[(rnd + 1) & 1] = 0;
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) *nq_r = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  __syncthreads();
   const size_t pl = plane3(g);
   for (int q = tid; q < 3 * AN; q += NT) {
     const int comp = q / AN, rem = q - comp * AN;
