@@ -18,6 +18,8 @@
 //   k_colsort3d   stable counting sort of R by x cell: for every sorted slot
 //                 the gp slot its particle comes from (the next push reads
 //                 the record and its id through it).
+// The fused fast and mixed steps skip k_colbuild3d: k_colsort3d_fast sorts
+// each column's stayers and arrivals straight from the mover lists.
 #include "wpic_internal3d.h"
 
 namespace wpic {
@@ -452,6 +454,172 @@ __global__ __launch_bounds__(SB) void k_colsort3d(Sort3Args a, int nbits) {
   if (bad) atomicOr(a.err, ERR_ROW);
 }
 
+// The fused step's exchange and sort in the fast and mixed modes, in one
+// pass per destination column and without the reference's hole filling: the
+// column's particles are its own stayers (direction DIR_STAY) in slot order
+// and the arrivals of its segments (k_colscan3d's per-direction mover lists),
+// each wave taking a quarter of the stayer slots followed by a quarter of the
+// arrivals; a stable counting sort by x cell of that sequence, written as a
+// permutation.  Each cell receives the particles particle_yz and sort__bucket
+// put there (np2 and cumcnt are the reference's), in an order of its own, so
+// only the summation order of the deposit changes.
+__global__ __launch_bounds__(SB) void k_colsort3d_fast(Sort3Args a, int nbits) {
+  extern __shared__ int hist[];  // [SB/64][nx]
+  __shared__ int wsum[SB / 64];
+  __shared__ int carry;
+  __shared__ int scnt[8], sptr[8];
+  __shared__ int nsg;
+  constexpr int SF = 8, NW = SB / 64;
+  const Grid3D& g = a.g;
+  const int nx = g.nx;
+  const int rw = blockIdx.x;
+  const int s = rw / g.ncol, col = rw % g.ncol;
+  const int base = a.colbaseB[rw], n = a.np2B[rw];
+  const int L = a.np2N[rw], nb = a.colbaseN[rw];
+  const int m = a.colkm[rw * 2 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const unsigned ntot = (unsigned)(g.ncap * g.nsp);
+  const uint64_t lt = lt_mask(lane);
+  if (tid == 0) {
+    nsg = a.nseg[col];
+    for (int e = 0; e < nsg; e++) {
+      const int sg = a.seg[col * 8 + e];
+      const int src = s * g.ncol + (sg >> 4), d = sg & 15;
+      scnt[e] = a.dcnt[src * 9 + d];
+      sptr[e] = a.colbaseB[src] + a.dstart[src * 9 + d];
+    }
+  }
+  for (int q = tid; q < NW * nx; q += SB) hist[q] = 0;
+  __syncthreads();
+  // this wave's stayer slots [lo, hi) of the column and arrivals [alo, ahi)
+  const int lo = (int)(((long long)n * w) / NW), hi = (int)(((long long)n * (w + 1)) / NW);
+  const int alo = (int)(((long long)m * w) / NW), ahi = (int)(((long long)m * (w + 1)) / NW);
+  const int nsl = hi - lo;  // chunk index space: stayer slots then arrivals
+  const int tot = nsl + (ahi - alo);
+  // the segments in registers: arrival t of segment q (first arrival sb[q])
+  // is list entry sp[q] + t; an empty tail of segments starts at infinity
+  int sb[8], sp[8];
+  {
+    const int nseg = nsg;
+    int acc = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const bool in = q < nseg;
+      const int cnt = in ? scnt[q] : 0;
+      sb[q] = in ? acc : 0x7fffffff;
+      sp[q] = in ? sptr[q] - acc : 0;
+      acc += cnt;
+    }
+  }
+  int* hw = hist + w * nx;
+  int* cc = a.cumcnt + (size_t)rw * (nx + 1);
+  bool bad = false;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int e0 = 0; e0 < tot; e0 += 64 * SF) {
+      // a stayer slot's code (direction and key in one word), or an
+      // arrival's list entry (its source slot); every load of the chunk is
+      // issued before the first is used
+      int v[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        const int e = e0 + u * 64 + lane;
+        const int t = alo + (e - nsl);
+        int d = sp[0];
+#pragma unroll
+        for (int q = 1; q < 8; q++) d = t >= sb[q] ? sp[q] : d;
+        const int* ptr = e < nsl ? a.code + (base + lo + e) : (e < tot ? a.mlist + (d + t) : a.code);
+        v[u] = *ptr;
+      }
+      int src[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        const int e = e0 + u * 64 + lane;
+        const bool arr = e >= nsl && e < tot;
+        const bool okarr = arr && (unsigned)v[u] < ntot;
+        bad |= arr && !okarr;
+        src[u] = e < nsl ? (((v[u] & 15) == DIR_STAY) ? base + lo + e : -1) : (okarr ? v[u] : -1);
+      }
+      // an arrival's code (the stayers already hold theirs)
+      int c2[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        const int e = e0 + u * 64 + lane;
+        c2[u] = a.code[(e >= nsl && src[u] >= 0) ? src[u] : 0];
+      }
+      int key[SF];
+      uint64_t peers[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        const int e = e0 + u * 64 + lane;
+        key[u] = src[u] < 0 ? 0 : ((e < nsl ? v[u] : c2[u]) >> CODE_SHIFT3);
+        peers[u] = peers_of(key[u], src[u] >= 0, nbits);
+      }
+      if (pass == 0) {
+#pragma unroll
+        for (int u = 0; u < SF; u++)
+          if (src[u] >= 0 && __popcll(peers[u] & lt) == 0) atomicAdd(&hw[key[u]], __popcll(peers[u]));
+      } else {
+        // the lowest lane of each key takes the key's cursor for all of them
+        int old[SF];
+#pragma unroll
+        for (int u = 0; u < SF; u++) {
+          old[u] = 0;
+          if (src[u] >= 0 && __popcll(peers[u] & lt) == 0) old[u] = atomicAdd(&hw[key[u]], __popcll(peers[u]));
+        }
+        int pos[SF];
+#pragma unroll
+        for (int u = 0; u < SF; u++) {
+          const int lead = src[u] >= 0 ? __builtin_ctzll(peers[u]) : lane;
+          pos[u] = __shfl(old[u], lead, 64) + __popcll(peers[u] & lt);
+        }
+#pragma unroll
+        for (int u = 0; u < SF; u++) {
+          const bool ok = src[u] >= 0 && pos[u] < L;
+          bad |= src[u] >= 0 && !ok;
+          if (ok) a.perm[nb + pos[u]] = src[u];
+        }
+      }
+    }
+    __syncthreads();
+    if (pass == 1) break;
+    // exclusive scan over keys of the column totals -> cumcnt; per-wave cursors
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int k0 = 0; k0 < nx; k0 += SB) {
+      const int key = k0 + tid;
+      int kt = 0;
+      if (key < nx)
+        for (int q = 0; q < NW; q++) kt += hist[q * nx + key];
+      int v = kt;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+      }
+      if (lane == 63) wsum[w] = v;
+      __syncthreads();
+      int pre = carry;
+      for (int q = 0; q < w; q++) pre += wsum[q];
+      const int excl = pre + v - kt;
+      if (key < nx) {
+        cc[key] = excl;
+        int st = excl;
+        for (int q = 0; q < NW; q++) {
+          const int h = hist[q * nx + key];
+          hist[q * nx + key] = st;
+          st += h;
+        }
+      }
+      __syncthreads();
+      if (tid == SB - 1) carry = excl + kt;
+      __syncthreads();
+    }
+    if (tid == 0) cc[nx] = L;
+    __syncthreads();
+  }
+  if (bad) atomicOr(a.err, ERR_ROW);
+}
+
 // sort__bucket on a gp that did not go through particle_yz: identity R
 __global__ __launch_bounds__(SB) void k_codes_identity3d(Sort3Args a) {
   const Grid3D& g = a.g;
@@ -520,6 +688,14 @@ hipError_t launch_colsort3d(wpic_ctx3d* c) {
   while ((1 << nbits) < c->g.nx) nbits++;
   const size_t shm = sizeof(int) * (SB / 64) * (size_t)c->g.nx;
   hipLaunchKernelGGL(k_colsort3d, ncols(c), dim3(SB), shm, c->st, sort3_args(c), nbits);
+  return hipGetLastError();
+}
+
+hipError_t launch_colsort3d_fast(wpic_ctx3d* c) {
+  int nbits = 0;
+  while ((1 << nbits) < c->g.nx) nbits++;
+  const size_t shm = sizeof(int) * (SB / 64) * (size_t)c->g.nx;
+  hipLaunchKernelGGL(k_colsort3d_fast, ncols(c), dim3(SB), shm, c->st, sort3_args(c), nbits);
   return hipGetLastError();
 }
 

@@ -249,20 +249,25 @@ int push_deposit3(wpic_ctx3d* c) {
   return 0;
 }
 
-int exchange3(wpic_ctx3d* c) {
+// particle_yz on gp (columns B -> columns N, permutation R); with
+// `lists_only` the column counts and mover lists alone, for the fused fast
+// sort (k_colsort3d_fast builds each new column from them)
+int exchange3(wpic_ctx3d* c, bool lists_only = false) {
   Timed3 t(c, "exchange");
   HIPCHK(launch_colscan3d(c, c->b_coded ? 0 : 1));
   HIPCHK(launch_colcount3d(c));
-  HIPCHK(launch_colbuild3d(c));
+  if (!lists_only) HIPCHK(launch_colbuild3d(c));
   c->h_cols_ok[c->cn] = false;
   c->exch_pending = true;
   return 0;
 }
 
-int sort_into_a3(wpic_ctx3d* c) {
+// sort__bucket: gp through R into A, by x cell within columns (fast: the
+// fused step's one-pass exchange and sort, k_colsort3d_fast)
+int sort_into_a3(wpic_ctx3d* c, bool fast = false) {
   {
     Timed3 t(c, "sort");
-    HIPCHK(launch_colsort3d(c));
+    HIPCHK(fast ? launch_colsort3d_fast(c) : launch_colsort3d(c));
   }
   const int old_a = c->ca;
   c->ca = c->cn;
@@ -784,9 +789,10 @@ int wpic_step3d(wpic_ctx3d* c, int nsteps) {
         Timed3 t(c, "exchange");
         HIPCHK(launch_wrapx3d(c));
       }
-      rc = exchange3(c);
+      const bool fast = c->mode == WPIC_MODE_FAST || c->mixed;
+      rc = exchange3(c, fast);
       if (rc) return rc;
-      rc = sort_into_a3(c);
+      rc = sort_into_a3(c, fast);
       if (rc) return rc;
       HIPCHK(hipEventRecord(c->ev_sort, c->st));
     }

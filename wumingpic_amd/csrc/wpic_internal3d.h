@@ -190,6 +190,7 @@ hipError_t launch_colscan3d(wpic_ctx3d* c, int wrap = 1);
 hipError_t launch_colcount3d(wpic_ctx3d* c);
 hipError_t launch_colbuild3d(wpic_ctx3d* c);
 hipError_t launch_colsort3d(wpic_ctx3d* c);
+hipError_t launch_colsort3d_fast(wpic_ctx3d* c);
 hipError_t launch_codes_identity3d(wpic_ctx3d* c);
 // transfers and initial state
 hipError_t launch_up_to_soa3d(wpic_ctx3d* c, const double* aos, long long n, int which);
