@@ -8,14 +8,16 @@
 // of 48 B.  Push and deposit run in FP32 (FMA contraction on) with the tile's
 // current accumulated in FP32 in LDS; the tile is added to the FP64 uj with
 // FP64 atomics (the global current reduction stays FP64), and the field solve
-// is the FP64 one.  Work mapping as the FP64 fast kernel: one workgroup per
-// 32x4-cell tile, 16-lane groups per cell.
+// is the FP64 one.  Work mapping as the FP64 fast kernel (one workgroup per
+// 32x4-cell tile) with 32-lane groups per cell, so that a group's loads and
+// stores of each 4-byte record array cover 128 contiguous bytes per pass as
+// the FP64 kernel's 16-lane groups do.
 #include "wpic_internal.h"
 
 namespace wpic {
 
 namespace mx {
-constexpr int TX = 32, TY = 4, NT = 256, G = 16;
+constexpr int TX = 32, TY = 4, NT = 256, G = 32;
 constexpr int SFW = TX + 2, SFH = TY + 2;
 constexpr int ACW = TX + 4, ACH = TY + 4;
 }  // namespace mx
@@ -77,8 +79,8 @@ __device__ inline float rsqf(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Every lane keeps the full 5x5x3 FP32 block of the cell it walks (no LDS
 // traffic per particle), the group reduces it once per cell.  Work mapping:
-// wave w walks row w of the tile four x-adjacent cells at a time (one per
-// 16-lane group), so the records of particles that crossed a cell or row
+// wave w walks row w of the tile two x-adjacent cells at a time (one per
+// 32-lane group), so the records of particles that crossed a cell or row
 // boundary are read while the lines they share with their old neighbours
 // are in L2.
 template <int BC>
@@ -296,8 +298,9 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
 #pragma unroll
     for (int k = 0; k < 65; k++) {
       float v = k < 20 ? ax[k] : (k < 40 ? ay[k - 20] : az[k - 40]);
-      static_assert(G == 16, "the group is one DPP row");
+      static_assert(G == 32, "the group is two DPP rows");
       v = row16_sum(v);
+      v += __shfl_xor(v, 16, 32);
       if ((k % G) == lg) {
         int comp, ip, jp;
         if (k < 20) {
