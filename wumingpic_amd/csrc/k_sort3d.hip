@@ -12,7 +12,8 @@
 // computes it once per context (seg / nseg).  Then, in parallel:
 //   k_colscan3d   per column: y/z wraps, direction, exchange code; leaver slots
 //                 (holes) and the movers of each direction in slot order;
-//   k_colcount3d  arrivals, new column lengths and bases;
+//   k_colcount3d  arrivals, new column lengths and bases (block-wise
+//                 prefix sums, then the blocks' offsets);
 //   k_colbuild3d  R = for each slot of the post-exchange column the particle
 //                 that lands there (hole filling as :418-455 does it);
 //   k_colsort3d   stable counting sort of R by x cell: for every sorted slot
@@ -36,6 +37,7 @@ struct Sort3Args {
   int *code, *R, *holes, *mlist;
   int *colbaseB, *np2B, *colbaseN, *np2N, *cumcnt;
   int *dcnt, *dstart, *colkm;
+  int* csum;  // [nsp * ceil(ncol / 1024)] column-count workgroup totals
   const int *seg, *nseg;
   int* err;
 };
@@ -57,10 +59,15 @@ __global__ __launch_bounds__(SB) void k_wrapx3d(Sort3Args a) {
 }
 
 // per column: the y/z wraps of particle_yz (:147-165) and the exchange code;
-// pass 2 lists the holes and the movers of each direction in slot order
+// pass 2 lists the holes and the movers of each direction in slot order.
+// Each wave owns a contiguous quarter of the column: pass 1 counts its
+// quarter's directions with ballots (SF chunks of 64 per round, their loads
+// issued before the first is used), one barrier turns the four waves' counts
+// into offsets, and pass 2 writes the wave's holes and movers from them, so
+// the block waits at three barriers whatever the column's length.
 __global__ __launch_bounds__(SB) void k_colscan3d(Sort3Args a, int wrap) {
-  __shared__ int wt[SB / 64][10];
-  __shared__ int run[10];
+  constexpr int SF = 4, NW = SB / 64;
+  __shared__ int wt[NW][10];
   __shared__ int dst[9];
   const Grid3D& g = a.g;
   const int rw = blockIdx.x;
@@ -69,136 +76,163 @@ __global__ __launch_bounds__(SB) void k_colscan3d(Sort3Args a, int wrap) {
   const int base = a.colbaseB[rw], n = a.np2B[rw];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint64_t lt = lt_mask(lane);
-  if (tid < 10) run[tid] = 0;
-  __syncthreads();
-  // pass 1: codes and per-direction counts
-  for (int c0 = 0; c0 < n; c0 += SB) {
-    const int q = c0 + tid;
-    int dir = -1;
-    if (q < n) {
-      const int p = base + q;
-      if (wrap) {
-        double x = a.B.x[p], y = a.B.y[p], z = a.B.z[p];
-        bool moved;
-        const int code = wrap3_code<false>(g, x, y, z, j, k, a.err, moved);
-        if (moved) {
-          a.B.y[p] = y;
-          a.B.z[p] = z;
+  const int lo = (int)(((long long)n * w) / NW), hi = (int)(((long long)n * (w + 1)) / NW);
+  // pass 1: codes and the quarter's per-direction counts (9) and leavers
+  int cnt[10];
+#pragma unroll
+  for (int d = 0; d < 10; d++) cnt[d] = 0;
+  for (int k0 = lo; k0 < hi; k0 += 64 * SF) {
+    int dir[SF];
+    if (wrap) {
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        const int q = k0 + u * 64 + lane;
+        dir[u] = -1;
+        if (q < hi) {
+          const int p = base + q;
+          double x = a.B.x[p], y = a.B.y[p], z = a.B.z[p];
+          bool moved;
+          const int code = wrap3_code<false>(g, x, y, z, j, k, a.err, moved);
+          if (moved) {
+            a.B.y[p] = y;
+            a.B.z[p] = z;
+          }
+          a.code[p] = code;
+          dir[u] = code & 15;
         }
-        a.code[p] = code;
-        dir = code & 15;
-      } else {
-        dir = a.code[p] & 15;
       }
+    } else {
+      int v[SF];
+#pragma unroll
+      for (int u = 0; u < SF; u++) {
+        const int q = k0 + u * 64 + lane;
+        v[u] = a.code[base + (q < hi ? q : lo)];
+      }
+#pragma unroll
+      for (int u = 0; u < SF; u++) dir[u] = k0 + u * 64 + lane < hi ? (v[u] & 15) : -1;
     }
 #pragma unroll
-    for (int d = 0; d < 9; d++) {
-      const uint64_t b = __ballot(dir == d);
-      if (lane == 0) wt[w][d] = __popcll(b);
+    for (int u = 0; u < SF; u++) {
+#pragma unroll
+      for (int d = 0; d < 9; d++) cnt[d] += __popcll(__ballot(dir[u] == d));
+      cnt[9] += __popcll(__ballot(dir[u] >= 0 && dir[u] != DIR_STAY));
     }
-    __syncthreads();
-    if (tid < 9) {
-      int t = 0;
-      for (int v = 0; v < SB / 64; v++) t += wt[v][tid];
-      run[tid] += t;
-    }
-    __syncthreads();
   }
+  if (lane == 0) {
+#pragma unroll
+    for (int d = 0; d < 10; d++) wt[w][d] = cnt[d];
+  }
+  __syncthreads();
   if (tid == 0) {
     int acc = 0;
     for (int d = 0; d < 9; d++) {
+      int t = 0;
+      for (int v = 0; v < NW; v++) t += wt[v][d];
       dst[d] = acc;
-      a.dcnt[rw * 9 + d] = run[d];
+      a.dcnt[rw * 9 + d] = t;
       a.dstart[rw * 9 + d] = acc;
-      if (d != DIR_STAY) acc += run[d];
+      if (d != DIR_STAY) acc += t;
     }
     a.colkm[rw * 2] = acc;  // leavers
   }
   __syncthreads();
-  if (tid < 10) run[tid] = 0;
-  __syncthreads();
-  // pass 2: holes (1-based slots of the leavers) and per-direction mover lists
-  for (int c0 = 0; c0 < n; c0 += SB) {
-    const int q = c0 + tid;
-    const int dir = q < n ? (a.code[base + q] & 15) : -1;
-    const bool lv = q < n && dir != DIR_STAY;
-    const uint64_t bl = __ballot(lv);
-    if (lane == 0) wt[w][9] = __popcll(bl);
-    uint64_t bd = 0;
+  // this wave's running offsets: the earlier waves' counts, then its own
+  int oh = 0, od[9];
 #pragma unroll
-    for (int d = 0; d < 9; d++) {
-      const uint64_t b = __ballot(dir == d);
-      if (lane == 0) wt[w][d] = __popcll(b);
-      if (dir == d) bd = b;
+  for (int d = 0; d < 9; d++) od[d] = dst[d];
+  for (int v = 0; v < w; v++) {
+    oh += wt[v][9];
+#pragma unroll
+    for (int d = 0; d < 9; d++) od[d] += wt[v][d];
+  }
+  // pass 2: holes (1-based slots of the leavers) and per-direction mover lists
+  for (int k0 = lo; k0 < hi; k0 += 64 * SF) {
+    int v[SF];
+#pragma unroll
+    for (int u = 0; u < SF; u++) {
+      const int q = k0 + u * 64 + lane;
+      v[u] = a.code[base + (q < hi ? q : lo)];
     }
-    __syncthreads();
-    if (lv) {
-      int oh = run[9], od = run[dir];
-      for (int v = 0; v < w; v++) {
-        oh += wt[v][9];
-        od += wt[v][dir];
+#pragma unroll
+    for (int u = 0; u < SF; u++) {
+      const int q = k0 + u * 64 + lane;
+      const int dir = q < hi ? (v[u] & 15) : -1;
+      const bool lv = dir >= 0 && dir != DIR_STAY;
+      const uint64_t bl = __ballot(lv);
+      if (lv) a.holes[base + oh + __popcll(bl & lt)] = q + 1;
+      oh += __popcll(bl);
+#pragma unroll
+      for (int d = 0; d < 9; d++) {
+        if (d == DIR_STAY) continue;
+        const uint64_t bd = __ballot(dir == d);
+        if (dir == d) a.mlist[base + od[d] + __popcll(bd & lt)] = base + q;
+        od[d] += __popcll(bd);
       }
-      a.holes[base + oh + __popcll(bl & lt)] = q + 1;
-      a.mlist[base + dst[dir] + od + __popcll(bd & lt)] = base + q;
     }
-    __syncthreads();
-    if (tid < 10) {
-      int t = 0;
-      for (int v = 0; v < SB / 64; v++) t += wt[v][tid];
-      run[tid] += t;
-    }
-    __syncthreads();
   }
 }
 
-// arrivals m of every column, new length L = np2 - leavers + m, bases
-__global__ __launch_bounds__(1024) void k_colcount3d(Sort3Args a) {
-  __shared__ int wsum[16];
-  __shared__ int carry;
+// arrivals m of every column, new length L = np2 - leavers + m, and its base:
+// one workgroup per 1024 columns of a species takes the exclusive prefix of L
+// within its columns (written as colbaseN for now) and its total; a single
+// workgroup then turns the totals into offsets, and a last pass adds them.
+constexpr int CC = 1024;  // columns per counting workgroup
+__global__ __launch_bounds__(CC) void k_colcount3d(Sort3Args a) {
+  __shared__ int wsum[CC / 64];
   const Grid3D& g = a.g;
   const int ncol = g.ncol, tid = threadIdx.x;
-  for (int s = 0; s < g.nsp; s++) {
-    if (tid == 0) carry = 0;
-    __syncthreads();
-    for (int r0 = 0; r0 < ncol; r0 += 1024) {
-      const int r = r0 + tid;
-      int L = 0;
-      if (r < ncol) {
-        const int rw = s * ncol + r;
-        int m = 0;
-        for (int e = 0; e < a.nseg[r]; e++) {
-          const int sg = a.seg[r * 8 + e];
-          m += a.dcnt[(s * ncol + (sg >> 4)) * 9 + (sg & 15)];
-        }
-        a.colkm[rw * 2 + 1] = m;
-        L = a.np2B[rw] - a.colkm[rw * 2] + m;
-        a.np2N[rw] = L;
-      }
-      const int lane = tid & 63, w = tid >> 6;
-      int v = L;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int t = __shfl_up(v, off, 64);
-        if (lane >= off) v += t;
-      }
-      if (lane == 63) wsum[w] = v;
-      __syncthreads();
-      if (tid == 0) {
-        int acc = 0;
-        for (int q = 0; q < 16; q++) {
-          const int t = wsum[q];
-          wsum[q] = acc;
-          acc += t;
-        }
-      }
-      __syncthreads();
-      const int excl = carry + wsum[w] + v - L;
-      if (r < ncol) a.colbaseN[s * ncol + r] = (int)(s * g.ncap) + excl;
-      __syncthreads();
-      if (tid == 1023) carry = excl + L;
-      __syncthreads();
+  const int nbs = (ncol + CC - 1) / CC;
+  const int s = blockIdx.x / nbs, r = (blockIdx.x % nbs) * CC + tid;
+  int L = 0;
+  if (r < ncol) {
+    const int rw = s * ncol + r;
+    int m = 0;
+    for (int e = 0; e < a.nseg[r]; e++) {
+      const int sg = a.seg[r * 8 + e];
+      m += a.dcnt[(s * ncol + (sg >> 4)) * 9 + (sg & 15)];
     }
+    a.colkm[rw * 2 + 1] = m;
+    L = a.np2B[rw] - a.colkm[rw * 2] + m;
+    a.np2N[rw] = L;
   }
+  const int lane = tid & 63, w = tid >> 6;
+  int v = L;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  if (lane == 63) wsum[w] = v;
+  __syncthreads();
+  int pre = 0;
+  for (int q = 0; q < w; q++) pre += wsum[q];
+  const int excl = pre + v - L;
+  if (r < ncol) a.colbaseN[s * ncol + r] = excl;
+  if (tid == CC - 1) a.csum[blockIdx.x] = excl + L;
+}
+
+// exclusive scan of the counting workgroups' totals, per species
+__global__ void k_colcount3d_scan(Sort3Args a) {
+  const Grid3D& g = a.g;
+  const int nbs = (g.ncol + CC - 1) / CC;
+  const int s = threadIdx.x;
+  if (s >= g.nsp) return;
+  int acc = 0;
+  for (int b = 0; b < nbs; b++) {
+    const int t = a.csum[s * nbs + b];
+    a.csum[s * nbs + b] = acc;
+    acc += t;
+  }
+}
+
+// column bases: the species' block plus the counting workgroup's offset
+__global__ __launch_bounds__(256) void k_colcount3d_base(Sort3Args a) {
+  const Grid3D& g = a.g;
+  const int nbs = (g.ncol + CC - 1) / CC;
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= g.nsp * g.ncol) return;
+  const int s = q / g.ncol, r = q - s * g.ncol;
+  a.colbaseN[q] += (int)(s * g.ncap) + a.csum[s * nbs + r / CC];
 }
 
 // R of one column: slot -> source particle (boundary_periodic.f90:418-455)
@@ -654,6 +688,7 @@ static Sort3Args sort3_args(wpic_ctx3d* c) {
   a.dcnt = c->dcnt;
   a.dstart = c->dstart;
   a.colkm = c->colkm;
+  a.csum = c->csum;
   a.seg = c->seg;
   a.nseg = c->nseg;
   a.err = c->err;
@@ -674,7 +709,11 @@ hipError_t launch_colscan3d(wpic_ctx3d* c, int wrap) {
 }
 
 hipError_t launch_colcount3d(wpic_ctx3d* c) {
-  hipLaunchKernelGGL(k_colcount3d, dim3(1), dim3(1024), 0, c->st, sort3_args(c));
+  const Sort3Args a = sort3_args(c);
+  const int nbs = (c->g.ncol + CC - 1) / CC, n = c->g.nsp * c->g.ncol;
+  hipLaunchKernelGGL(k_colcount3d, dim3((unsigned)(c->g.nsp * nbs)), dim3(CC), 0, c->st, a);
+  hipLaunchKernelGGL(k_colcount3d_scan, dim3(1), dim3(64), 0, c->st, a);
+  hipLaunchKernelGGL(k_colcount3d_base, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }
 

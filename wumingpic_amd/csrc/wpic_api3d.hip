@@ -502,6 +502,7 @@ int wpic_create3d(wpic_ctx3d** out, const wpic_params3d* p) {
   if (e == hipSuccess) e = dalloc(&c->dcnt, (size_t)9 * nr);
   if (e == hipSuccess) e = dalloc(&c->dstart, (size_t)9 * nr);
   if (e == hipSuccess) e = dalloc(&c->colkm, (size_t)2 * nr);
+  if (e == hipSuccess) e = dalloc(&c->csum, (size_t)c->g.nsp * ((c->g.ncol + 1023) / 1024));
   if (e == hipSuccess) e = dalloc(&c->seg, hseg.size());
   if (e == hipSuccess) e = dalloc(&c->nseg, hnseg.size());
   if (e == hipSuccess) e = hipMemcpy(c->seg, hseg.data(), sizeof(int) * hseg.size(), hipMemcpyHostToDevice);
@@ -551,7 +552,7 @@ int wpic_destroy3d(wpic_ctx3d* c) {
   void* ps[] = {c->A.x,    c->A.y,   c->A.z,      c->A.ux,    c->A.uy,   c->A.uz,    c->B.x,    c->B.y,
                 c->B.z,    c->B.ux,  c->B.uy,     c->B.uz,    c->id[0],  c->id[1],   c->code,   c->R,
                 c->holes,  c->mlist, c->cols[0],  c->cols[1], c->cols[2], c->cumcnt, c->dcnt,   c->dstart,
-                c->colkm,  c->seg,   c->nseg,     c->aosbase, c->uf,     c->df,      c->uj,     c->pjbuf,
+                c->colkm,  c->csum,  c->seg,   c->nseg,     c->aosbase, c->uf,     c->df,      c->uj,     c->pjbuf,
                 c->phi,    c->pa,    c->pb,       c->rr,      c->bb,     c->ap,      c->part,   c->scal,
                 c->err,    c->staging, c->gbuf,   c->red,     c->permbuf, c->nov,    c->mom,
                 c->MA.ix,  c->MA.ox, c->MA.oy,    c->MA.oz,   c->MA.ux,  c->MA.uy,  c->MA.uz,

@@ -126,6 +126,7 @@ struct wpic_ctx3d {
   int* dcnt;    // [nsp*ncol][9] movers per direction (DIR_STAY: stayers)
   int* dstart;  // [nsp*ncol][9] exclusive start of each direction in mlist
   int* colkm;   // [nsp*ncol][2] leavers, arrivals
+  int* csum;    // [nsp * ceil(ncol / 1024)] column-count workgroup totals
   int* seg;     // [ncol][8] arrival segments (source column << 4 | dir), in order
   int* nseg;    // [ncol]
   bool b_valid, b_aligned, exch_pending;
