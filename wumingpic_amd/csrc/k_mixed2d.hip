@@ -299,9 +299,11 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
     for (int k = 0; k < 65; k++) {
       float v = k < 20 ? ax[k] : (k < 40 ? ay[k - 20] : az[k - 40]);
       static_assert(G == 32, "the group is two DPP rows");
+      // each row's sum, then row_bcast:15 adds lane 15 of the group's lower
+      // row to the lanes of its upper row, which then hold the group's total
       v = row16_sum(v);
-      v += __shfl_xor(v, 16, 32);
-      if ((k % G) == lg) {
+      v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), DPP_ROW_BCAST15, 0xa, 0xf, false));
+      if ((k % 16) + 16 == lg) {
         int comp, ip, jp;
         if (k < 20) {
           comp = 0; jp = k / 4 - 2; ip = k % 4 - 1;

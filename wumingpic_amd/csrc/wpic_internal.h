@@ -202,6 +202,7 @@ constexpr int DPP_QUAD_XOR1 = 0xB1;  // quad_perm [1, 0, 3, 2]
 constexpr int DPP_QUAD_XOR2 = 0x4E;  // quad_perm [2, 3, 0, 1]
 constexpr int DPP_ROW_ROR4 = 0x124;
 constexpr int DPP_ROW_ROR8 = 0x128;
+constexpr int DPP_ROW_BCAST15 = 0x142;  // lane 15 of each row to the next row's lanes
 template <class T>
 __device__ inline T row16_sum(T v) {
   if constexpr (sizeof(T) == 8) {
