@@ -274,6 +274,56 @@ def test_fast3d_cell_changers_beyond_the_tile_queue():
     assert ok, info
 
 
+def test_fast3d_steps_sparse_columns_match_oracle():
+    """Fast mode's one-pass exchange and sort on a ragged state: cells with 0 to
+    7 particles, two empty columns, and y-z momenta large enough that many
+    particles change column every step (arrivals into empty columns, columns
+    that lose most of theirs, movers across every face, edge and corner of the
+    periodic box).  After one step each cell holds the oracle's particles bit
+    for bit as a set, with the oracle's np2 and cumcnt; two more steps (the
+    sort now reading through the previous step's permutation, the push on
+    fields that carry the deposit's rounding) keep every particle id exactly
+    once, cumcnt consistent with np2 and the fields within 1e-9 of the
+    oracle's."""
+    from wumingpic_amd import MODE_FAST
+    rng = np.random.default_rng(53)
+    q, r = S.weibel_consts(4)
+    p = po.make_params3d(6, 5, 4, np_cap=3 * 8 * 6, delt=0.5, q=q, r=r)
+    counts = rng.integers(0, 8, size=(6, 5, 4))
+    counts[:, 1, 2] = 0
+    counts[:, 3, 0] = 0
+    up, cc, np2 = S.sorted_particles3d(p, 4, rng, u_scale=(0.3, 0.6), drift=(0.0, 0.0), colocate=True,
+                                       counts=[counts, counts])
+    uf = S.periodic_fill_fields3d(p, S.random_fields3d(p, rng, scale=0.02))
+    assert (np2 == 0).sum() == 4
+    ids0 = [np.sort(np.concatenate([up[6, :np2[j, k, s], j, k, s].view(np.int64) for k in range(p.nz)
+                                    for j in range(p.ny)])) for s in range(2)]
+    print("fast3d sparse columns, 1 + 2 steps", flush=True)
+    with _pic(p, MODE_FAST) as sim:
+        sim.upload_up(up, np2, cc)
+        sim.upload_fields(uf)
+        sim.step(1)
+        u1, n21, cc1 = sim.download_up(p.np)
+        sim.step(2)
+        u3, n23, cc3 = sim.download_up(p.np)
+        uf3 = sim.download_fields()
+    o = po.Oracle3D(p)
+    gp = np.zeros_like(up, order="F")
+    o.step(up, gp, uf, cc, np2, 1)
+    assert np.array_equal(n21, np2)
+    assert np.array_equal(cc1, cc)
+    ok, info = _cells_equal(u1, up, np2, cc)
+    assert ok, info
+    o.step(up, gp, uf, cc, np2, 2)
+    for s in range(2):
+        ids = np.sort(np.concatenate([u3[6, :n23[j, k, s], j, k, s].view(np.int64) for k in range(p.nz)
+                                      for j in range(p.ny)]))
+        assert np.array_equal(ids, ids0[s]), s
+    assert np.array_equal(cc3[-1], n23)
+    assert np.all(np.diff(cc3, axis=0) >= 0)
+    assert np.abs(uf3 - uf).max() <= 1e-9 * np.abs(uf).max()
+
+
 def test_weibel3d_fast_mode_close_and_conserving():
     from wumingpic_amd import MODE_FAST
     sim, p, up, uf, cc, np2 = _weibel3d_device_state(12, 8, 6, 4, MODE_FAST)
