@@ -33,6 +33,7 @@ struct MixArgs {
   int* code;
   int* err;
   int ntx, nty;
+  int* rcnt;  // particles per (species, row, x cell) after the step, or null
 };
 
 // cell-centred field component c at (i, j) (particle.f90:74-80), FP64
@@ -90,6 +91,10 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
   __shared__ float sf[6][SFW * SFH];
   __shared__ float acc[3][ACW * ACH];
   __shared__ float spc[WPIC_MAXSP][5];  // fac1, txxx, fac2, qk, q
+  // particles per (species, row, cell) after the step for the tile's cells
+  // and their one-cell halo (rcnt)
+  __shared__ int hcn[WPIC_MAXSP][TY + 2][TX + 2];
+  const bool counting = a.rcnt != nullptr;
 
   const Grid2D& g = a.g;
   const int ntiles = a.ntx * a.nty;
@@ -105,6 +110,8 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
     if (i <= g.nxge + 1 && j <= g.nye + 1) sf[c][rem] = (float)cell_field_m(g, a.uf, c, i, j);
   }
   for (int k = tid; k < 3 * ACW * ACH; k += NT) (&acc[0][0])[k] = 0.0f;
+  if (counting)
+    for (int k = tid; k < WPIC_MAXSP * (TY + 2) * (TX + 2); k += NT) (&hcn[0][0][0])[k] = 0;
   // per-species constants (particle.f90:92-94, field.f90:274), formed in FP64
   // and rounded once
   if (tid < g.nsp && tid < WPIC_MAXSP) {
@@ -291,7 +298,9 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
         a.B.ix[pi] = cx;
         a.B.ox[pi] = fx;
         a.B.oy[pi] = fy;
-        a.code[pi] = ((cx - g.nxgs) << 2) | (incy + 1);
+        const int code = ((cx - g.nxgs) << 2) | (incy + 1);
+        a.code[pi] = code;
+        if (counting && code != (((i - g.nxgs) << 2) | 1)) count_leaver<TX, TY>(g, hcn, a.rcnt, i0, j0, li, lj, s, code);
       }
     }
     // reduce the cell's 65 weights over the group, spread the LDS atomics
@@ -318,6 +327,7 @@ __global__ __launch_bounds__(mx::NT, 2) void k_push_deposit_mixed(MixArgs a) {
   }
 
   __syncthreads();
+  if (counting) flush_counts<TX, TY>(g, hcn, a.rcnt, a.cumcnt, i0, j0);
   const size_t pl = (size_t)g.fw * g.fh;
   for (int k = tid; k < 3 * ACW * ACH; k += NT) {
     const int comp = k / (ACW * ACH), rem = k - comp * (ACW * ACH);
@@ -477,6 +487,7 @@ hipError_t launch_push_deposit_mixed(wpic_ctx* c) {
   a.err = c->err;
   a.ntx = (c->g.nx + mx::TX - 1) / mx::TX;
   a.nty = (c->g.ny + mx::TY - 1) / mx::TY;
+  a.rcnt = c->rcnt_fill ? c->rcnt : nullptr;
   const int nblk = xcd_grid(a.ntx * a.nty);
   if (c->g.bc == WPIC_BC_WALLX)
     hipLaunchKernelGGL(k_push_deposit_mixed<WPIC_BC_WALLX>, dim3(nblk), dim3(mx::NT), 0, c->st, a);

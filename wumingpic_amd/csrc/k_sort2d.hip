@@ -27,6 +27,7 @@ struct SortArgs {
   int *code, *R, *holes, *ulist, *dlist;
   int *rowbaseB, *np2B, *rowbaseN, *np2N, *rowbaseA, *np2A, *cumcnt, *rowcnt;
   int* err;
+  const int* rcnt;  // particles per (species, row, x cell) from the fused push, or null
 };
 
 __device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
@@ -469,16 +470,54 @@ __global__ __launch_bounds__(SB) void k_rowsort_fast(SortArgs a, int* perm, int 
   const int alo = (int)(((long long)m * w) / NW), ahi = (int)(((long long)m * (w + 1)) / NW);
   const int nsl = hi - lo;  // chunk index space: stayer slots then arrivals
   const int tot = nsl + (ahi - alo);
-  int* hw = hist + w * nx;
+  // with the push's per-cell counts (rcnt) the cursors come from them and
+  // one pass places the particles, the four waves sharing one cursor per key
+  // (the order within a cell is then the LDS's, not the slots'); otherwise a
+  // counting pass builds per-wave histograms first
+  const bool counted = a.rcnt != nullptr;
+  int* hw = counted ? hist : hist + w * nx;
   const unsigned ntot = (unsigned)(g.ncap * g.nsp);
   const int* codes = a.code;
+  if (counted) {
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    int* cc = a.cumcnt + (size_t)rw * (nx + 1);
+    const int* rc = a.rcnt + (size_t)rw * nx;
+    for (int k0 = 0; k0 < nx; k0 += SB) {
+      const int key = k0 + tid;
+      const int t = key < nx ? rc[key] : 0;
+      int v = t;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
+      }
+      if (lane == 63) wsum[w] = v;
+      __syncthreads();
+      int pre = carry;
+      for (int q = 0; q < w; q++) pre += wsum[q];
+      const int excl = pre + v - t;
+      if (key < nx) {
+        cc[key] = excl;
+        hist[key] = excl;
+      }
+      __syncthreads();
+      if (tid == SB - 1) carry = excl + t;
+      __syncthreads();
+    }
+    if (tid == 0) {
+      cc[nx] = L;
+      if (carry != L) atomicOr(a.err, ERR_ROW);
+    }
+    __syncthreads();
+  }
   // SF chunks of 64 per wave and round.  Every load of a round is issued
   // before the first is used (addresses are selected, not branched on), and
   // the per-key cursors move by LDS atomics, whose returns the LDS serves in
   // issue order: a round waits for memory twice and for the LDS once.
   constexpr int SF = 8;
   bool bad = false;
-  for (int pass = 0; pass < 2; pass++) {
+  for (int pass = counted ? 1 : 0; pass < 2; pass++) {
     for (int e0 = 0; e0 < tot; e0 += 64 * SF) {
       // a stayer slot's code (row step and key in one word), or an arrival's
       // list entry (its source slot)
@@ -620,6 +659,7 @@ static SortArgs sort_args(wpic_ctx* c) {
   a.cumcnt = c->cumcnt;
   a.rowcnt = c->rowcnt;
   a.err = c->err;
+  a.rcnt = c->rcnt_fill ? c->rcnt : nullptr;
   return a;
 }
 

@@ -203,6 +203,7 @@ int field_after_deposit(wpic_ctx* c) {
 }
 
 int deposit(wpic_ctx* c, bool push, bool fused) {
+  if (c->rcnt_fill) HIPCHK(hipMemsetAsync(c->rcnt, 0, sizeof(int) * (size_t)c->g.nsp * c->g.ny * c->g.nx, c->st));
   if (c->mixed) {
     HIPCHK(launch_zero(c, c->uj, (size_t)3 * c->g.fw * c->g.fh));
     Timed t(c, "push_deposit");
@@ -283,8 +284,14 @@ int step_once(wpic_ctx* c) {
   c->rb = c->ra;
   c->rn = (c->ra + 1) % 3;
   set_rows(c);
+  // the mixed push counts each row's particles per x cell for the sort (in the
+  // FP64 fast push the same counting costs more than the sort's counting pass)
+  c->rcnt_fill = c->mixed;
   int rc = deposit(c, true, true);
-  if (rc) return rc;
+  if (rc) {
+    c->rcnt_fill = false;
+    return rc;
+  }
   c->b_valid = true;
   c->b_accl = false;
   c->b_aligned = true;
@@ -300,8 +307,12 @@ int step_once(wpic_ctx* c) {
     // fast and mixed modes put each cell's particles in an order of their own
     const bool fast = c->mode != WPIC_MODE_PARITY;
     rc = exchange(c, false, fast);
-    if (rc) return rc;
+    if (rc) {
+      c->rcnt_fill = false;
+      return rc;
+    }
     rc = sort_into_a(c, fast);
+    c->rcnt_fill = false;
     if (rc) return rc;
     HIPCHK(hipEventRecord(c->ev_sort, c->st));
   }
@@ -421,6 +432,7 @@ int wpic_create2d(wpic_ctx** out, const wpic_params2d* p) {
   if (e == hipSuccess) e = dalloc(&c->cumcnt, (size_t)nr * (nx + 1));
   if (e == hipSuccess) e = hipMemset(c->cumcnt, 0, sizeof(int) * (size_t)nr * (nx + 1));
   if (e == hipSuccess) e = dalloc(&c->rowcnt, (size_t)4 * nr);
+  if (e == hipSuccess) e = dalloc(&c->rcnt, (size_t)nr * c->g.nx);
   if (e == hipSuccess) e = dalloc(&c->aosbase, (size_t)nr);
   if (e == hipSuccess) e = dalloc(&c->uf, 6 * pl);
   if (e == hipSuccess) e = dalloc(&c->df, 6 * pl);
@@ -465,7 +477,7 @@ int wpic_destroy(wpic_ctx* c) {
   }
   void* ps[] = {c->A.x, c->A.y, c->A.ux, c->A.uy, c->A.uz, c->B.x, c->B.y, c->B.ux, c->B.uy, c->B.uz,
                 c->id[0], c->id[1], c->code, c->R, c->holes, c->ulist, c->dlist, c->rows[0], c->rows[1],
-                c->rows[2], c->cumcnt, c->rowcnt, c->aosbase, c->uf, c->df, c->uj, c->pjbuf, c->phi, c->pa,
+                c->rows[2], c->cumcnt, c->rowcnt, c->rcnt, c->aosbase, c->uf, c->df, c->uj, c->pjbuf, c->phi, c->pa,
                 c->pb, c->rr, c->bb, c->ap, c->part, c->scal, c->err, c->staging, c->gbuf, c->red, c->mom,
                 c->MA.ix, c->MA.ox, c->MA.oy, c->MA.ux, c->MA.uy, c->MA.uz,
                 c->MB.ix, c->MB.ox, c->MB.oy, c->MB.ux, c->MB.uy, c->MB.uz, c->permbuf};

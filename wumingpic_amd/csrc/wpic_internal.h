@@ -237,6 +237,51 @@ __device__ inline T row8_sum(T v) {
   return v;
 }
 
+// The sort's per-cell counts kept by a fused push: every cell of the tile
+// keeps its particles (cumcnt, added when the tile's counts are flushed)
+// except those that leave it, each of which moves one count from its own cell
+// to the cell and row of its exchange code ((new x cell - nxgs) << 2 | row
+// step + 1): in the tile's LDS counts (one-cell halo), or straight to rcnt
+// when the destination lies beyond (a wrap across the box).  rcnt is
+// [nsp*ny][nx].
+template <int TXC, int TYC>
+__device__ inline void count_leaver(const Grid2D& g, int (*hcn)[TYC + 2][TXC + 2], int* rcnt, int i0, int j0, int li,
+                                    int lj, int s, int code) {
+  atomicAdd(&hcn[s][lj + 1][li + 1], -1);
+  const int cn = (code >> 2) + g.nxgs, dy = (code & 3) - 1;
+  const int hx = cn - i0 + 1;
+  if (hx >= 0 && hx <= TXC + 1) {
+    atomicAdd(&hcn[s][lj + 1 + dy][hx], 1);
+  } else {
+    int jn = j0 + lj + dy;
+    if (jn < g.nys) jn += g.ny;
+    else if (jn > g.nye) jn -= g.ny;
+    atomicAdd(&rcnt[((size_t)s * g.ny + (jn - g.nys)) * g.nx + (cn - g.nxgs)], 1);
+  }
+}
+// the tile's counts, with its cells' own particles, added to rcnt (rows wrap
+// periodically in y)
+template <int TXC, int TYC>
+__device__ inline void flush_counts(const Grid2D& g, int (*hcn)[TYC + 2][TXC + 2], int* rcnt, const int* cumcnt,
+                                    int i0, int j0) {
+  constexpr int N = (TYC + 2) * (TXC + 2);
+  for (int k = threadIdx.x; k < g.nsp * N; k += blockDim.x) {
+    const int s = k / N, rem = k - s * N;
+    const int hr = rem / (TXC + 2), hx = rem % (TXC + 2);
+    int v = hcn[s][hr][hx];
+    const int cx = i0 - 1 + hx;
+    int jn = j0 - 1 + hr;
+    if (hr >= 1 && hr <= TYC && hx >= 1 && hx <= TXC && cx <= g.nxge && jn <= g.nye) {
+      const int* cr = cumcnt + ((size_t)s * g.ny + (jn - g.nys)) * (g.nx + 1) + (cx - g.nxgs);
+      v += cr[1] - cr[0];
+    }
+    if (v == 0) continue;
+    if (jn < g.nys) jn += g.ny;
+    else if (jn > g.nye) jn -= g.ny;
+    atomicAdd(&rcnt[((size_t)s * g.ny + (jn - g.nys)) * g.nx + (cx - g.nxgs)], v);
+  }
+}
+
 // Slots in an LDS queue for the active lanes of a wave with one atomic on the
 // counter *n: the active lanes get consecutive slots in lane order.
 __device__ inline int wave_enqueue(int* n) {
@@ -303,6 +348,11 @@ struct wpic_ctx {
   int *rowbaseB, *np2B;           // rows of B (gp)
   int *rowbaseN, *np2N;           // rows after the y exchange
   int* rowcnt;                    // [nsp*ny][4] leavers, up, down, incoming
+  // [nsp*ny][nx] particles per (species, row, x cell) after the step's push:
+  // the mixed push counts them (rcnt_fill) and the fast sort takes the rows'
+  // cell counts from them instead of counting the codes again
+  int* rcnt;
+  bool rcnt_fill;
   bool b_valid;       // B holds a gp array
   bool b_aligned;     // B's slots are A's slots (after particle__solv)
   int pusher;         // WPIC_PUSHER_BORIS or WPIC_PUSHER_VAY
