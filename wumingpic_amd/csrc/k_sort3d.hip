@@ -497,7 +497,7 @@ __global__ __launch_bounds__(SB) void k_colsort3d(Sort3Args a, int nbits) {
 // permutation.  Each cell receives the particles particle_yz and sort__bucket
 // put there (np2 and cumcnt are the reference's), in an order of its own, so
 // only the summation order of the deposit changes.
-__global__ __launch_bounds__(SB) void k_colsort3d_fast(Sort3Args a, int nbits) {
+__global__ __launch_bounds__(SB) void k_colsort3d_fast(Sort3Args a) {
   extern __shared__ int hist[];  // [SB/64][nx]
   __shared__ int wsum[SB / 64];
   __shared__ int carry;
@@ -513,7 +513,6 @@ __global__ __launch_bounds__(SB) void k_colsort3d_fast(Sort3Args a, int nbits) {
   const int m = a.colkm[rw * 2 + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const unsigned ntot = (unsigned)(g.ncap * g.nsp);
-  const uint64_t lt = lt_mask(lane);
   if (tid == 0) {
     nsg = a.nseg[col];
     for (int e = 0; e < nsg; e++) {
@@ -581,31 +580,22 @@ __global__ __launch_bounds__(SB) void k_colsort3d_fast(Sort3Args a, int nbits) {
         c2[u] = a.code[(e >= nsl && src[u] >= 0) ? src[u] : 0];
       }
       int key[SF];
-      uint64_t peers[SF];
 #pragma unroll
       for (int u = 0; u < SF; u++) {
         const int e = e0 + u * 64 + lane;
         key[u] = src[u] < 0 ? 0 : ((e < nsl ? v[u] : c2[u]) >> CODE_SHIFT3);
-        peers[u] = peers_of(key[u], src[u] >= 0, nbits);
       }
+      // one LDS atomic per particle: lanes of a key meet on its counter and
+      // the LDS serves them one after another (the order within a cell is
+      // the fast modes' own)
       if (pass == 0) {
 #pragma unroll
         for (int u = 0; u < SF; u++)
-          if (src[u] >= 0 && __popcll(peers[u] & lt) == 0) atomicAdd(&hw[key[u]], __popcll(peers[u]));
+          if (src[u] >= 0) atomicAdd(&hw[key[u]], 1);
       } else {
-        // the lowest lane of each key takes the key's cursor for all of them
-        int old[SF];
-#pragma unroll
-        for (int u = 0; u < SF; u++) {
-          old[u] = 0;
-          if (src[u] >= 0 && __popcll(peers[u] & lt) == 0) old[u] = atomicAdd(&hw[key[u]], __popcll(peers[u]));
-        }
         int pos[SF];
 #pragma unroll
-        for (int u = 0; u < SF; u++) {
-          const int lead = src[u] >= 0 ? __builtin_ctzll(peers[u]) : lane;
-          pos[u] = __shfl(old[u], lead, 64) + __popcll(peers[u] & lt);
-        }
+        for (int u = 0; u < SF; u++) pos[u] = src[u] >= 0 ? atomicAdd(&hw[key[u]], 1) : 0;
 #pragma unroll
         for (int u = 0; u < SF; u++) {
           const bool ok = src[u] >= 0 && pos[u] < L;
@@ -731,10 +721,8 @@ hipError_t launch_colsort3d(wpic_ctx3d* c) {
 }
 
 hipError_t launch_colsort3d_fast(wpic_ctx3d* c) {
-  int nbits = 0;
-  while ((1 << nbits) < c->g.nx) nbits++;
   const size_t shm = sizeof(int) * (SB / 64) * (size_t)c->g.nx;
-  hipLaunchKernelGGL(k_colsort3d_fast, ncols(c), dim3(SB), shm, c->st, sort3_args(c), nbits);
+  hipLaunchKernelGGL(k_colsort3d_fast, ncols(c), dim3(SB), shm, c->st, sort3_args(c));
   return hipGetLastError();
 }
 
