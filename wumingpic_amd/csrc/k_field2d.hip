@@ -172,7 +172,9 @@ __device__ inline double nbr(const Grid2D& g, const double* v, int i, int j, int
   return signed_val(s, v[k]);
 }
 
-// gkl (field.f90:125-146), b = f5*gkl, warm start phi = df(l), sum b^2
+// gkl (field.f90:125-146), b = f5*gkl, warm start phi = df(l), sum b^2.
+// Component l = blockIdx.y: three blocks per cell range, each thread's sum
+// over its cells in the same order as with one block for all three.
 __global__ __launch_bounds__(256) void k_cg_init(CgArgs a) {
   __shared__ double sh[4];
   const Grid2D& g = a.g;
@@ -182,34 +184,33 @@ __global__ __launch_bounds__(256) void k_cg_init(CgArgs a) {
                *U6 = a.uf + 5 * pl;
   const double *J1 = a.uj, *J2 = a.uj + pl, *J3 = a.uj + 2 * pl;
   const double f1 = g.f1, f2 = g.f2, f3 = g.f3, f5 = g.f5;
-  double s[3] = {0.0, 0.0, 0.0};
+  const int l = blockIdx.y;
+  double s = 0.0;
   for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += NPB * 256) {
     const int i = g.nxgs + k % g.nx, j = g.nys + k / g.nx;
 #define F(P, ii, jj) P[fidx(g, ii, jj)]
-    const double gk1 = +f2 * (+F(U1, i, j - 1) + F(U1, i - 1, j) - 4. * F(U1, i, j) + F(U1, i + 1, j) + F(U1, i, j + 1) +
-                              f3 * (-F(J3, i, j - 1) + F(J3, i, j))) -
-                       f1 * (-F(U6, i, j - 1) + F(U6, i, j));
-    const double gk2 = +f2 * (+F(U2, i, j - 1) + F(U2, i - 1, j) - 4. * F(U2, i, j) + F(U2, i + 1, j) + F(U2, i, j + 1) -
-                              f3 * (-F(J3, i - 1, j) + F(J3, i, j))) +
-                       f1 * (-F(U6, i - 1, j) + F(U6, i, j));
-    const double gk3 = +f2 * (+F(U3, i, j - 1) + F(U3, i - 1, j) - 4. * F(U3, i, j) + F(U3, i + 1, j) + F(U3, i, j + 1) +
-                              f3 * (-F(J2, i - 1, j) + F(J2, i, j) + F(J1, i, j - 1) - F(J1, i, j))) -
-                       f1 * (-F(U5, i - 1, j) + F(U5, i, j) + F(U4, i, j - 1) - F(U4, i, j));
+    double gk;
+    if (l == 0)
+      gk = +f2 * (+F(U1, i, j - 1) + F(U1, i - 1, j) - 4. * F(U1, i, j) + F(U1, i + 1, j) + F(U1, i, j + 1) +
+                  f3 * (-F(J3, i, j - 1) + F(J3, i, j))) -
+           f1 * (-F(U6, i, j - 1) + F(U6, i, j));
+    else if (l == 1)
+      gk = +f2 * (+F(U2, i, j - 1) + F(U2, i - 1, j) - 4. * F(U2, i, j) + F(U2, i + 1, j) + F(U2, i, j + 1) -
+                  f3 * (-F(J3, i - 1, j) + F(J3, i, j))) +
+           f1 * (-F(U6, i - 1, j) + F(U6, i, j));
+    else
+      gk = +f2 * (+F(U3, i, j - 1) + F(U3, i - 1, j) - 4. * F(U3, i, j) + F(U3, i + 1, j) + F(U3, i, j + 1) +
+                  f3 * (-F(J2, i - 1, j) + F(J2, i, j) + F(J1, i, j - 1) - F(J1, i, j))) -
+           f1 * (-F(U5, i - 1, j) + F(U5, i, j) + F(U4, i, j - 1) - F(U4, i, j));
 #undef F
-    const double gk[3] = {gk1, gk2, gk3};
-#pragma unroll
-    for (int l = 0; l < 3; l++) {
-      const double bv = f5 * gk[l];
-      a.b[(size_t)l * n + k] = bv;
-      a.phi[(size_t)l * n + k] = a.df[pl * l + fidx(g, i, j)];
-      s[l] = s[l] + bv * bv;
-    }
+    const double bv = f5 * gk;
+    a.b[(size_t)l * n + k] = bv;
+    a.phi[(size_t)l * n + k] = a.df[pl * l + fidx(g, i, j)];
+    s = s + bv * bv;
   }
   if (!a.parity) {
-    for (int l = 0; l < 3; l++) {
-      const double t = block_sum256(s[l], sh);
-      if (threadIdx.x == 0) a.part[((size_t)l * NPB + blockIdx.x) * 2] = t;
-    }
+    const double t = block_sum256(s, sh);
+    if (threadIdx.x == 0) a.part[((size_t)l * NPB + blockIdx.x) * 2] = t;
   }
 }
 
@@ -241,30 +242,27 @@ __global__ __launch_bounds__(256) void k_cg_fin0(CgArgs a) {
   }
 }
 
-// r = b + (sum of neighbours) - f4 phi; p = r; sum r^2 (field.f90:366-381)
+// r = b + (sum of neighbours) - f4 phi; p = r; sum r^2 (field.f90:366-381);
+// component l = blockIdx.y, as in k_cg_init
 __global__ __launch_bounds__(256) void k_cg_r0(CgArgs a) {
   __shared__ double sh[4];
   const Grid2D& g = a.g;
   const int n = g.nx * g.ny;
   const double f4 = g.f4;
-  double s[3] = {0.0, 0.0, 0.0};
+  const int l = blockIdx.y;
+  const double* ph = a.phi + (size_t)l * n;
+  double s = 0.0;
   for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += NPB * 256) {
     const int i = g.nxgs + k % g.nx, j = g.nys + k / g.nx;
-#pragma unroll
-    for (int l = 0; l < 3; l++) {
-      const double* ph = a.phi + (size_t)l * n;
-      const double rv = a.b[(size_t)l * n + k] + nbr(g, ph, i, j - 1, l) + nbr(g, ph, i - 1, j, l) -
-                        f4 * ph[k] + nbr(g, ph, i + 1, j, l) + nbr(g, ph, i, j + 1, l);
-      a.r[(size_t)l * n + k] = rv;
-      a.pold[(size_t)l * n + k] = rv;
-      s[l] = s[l] + rv * rv;
-    }
+    const double rv = a.b[(size_t)l * n + k] + nbr(g, ph, i, j - 1, l) + nbr(g, ph, i - 1, j, l) - f4 * ph[k] +
+                      nbr(g, ph, i + 1, j, l) + nbr(g, ph, i, j + 1, l);
+    a.r[(size_t)l * n + k] = rv;
+    a.pold[(size_t)l * n + k] = rv;
+    s = s + rv * rv;
   }
   if (!a.parity) {
-    for (int l = 0; l < 3; l++) {
-      const double t = block_sum256(s[l], sh);
-      if (threadIdx.x == 0) a.part[((size_t)l * NPB + blockIdx.x) * 2] = t;
-    }
+    const double t = block_sum256(s, sh);
+    if (threadIdx.x == 0) a.part[((size_t)l * NPB + blockIdx.x) * 2] = t;
   }
 }
 
@@ -523,9 +521,9 @@ hipError_t launch_dfield_bc(wpic_ctx* c) {
 
 hipError_t launch_cg_init(wpic_ctx* c) {
   CgArgs a = cg_args(c);
-  hipLaunchKernelGGL(k_cg_init, dim3(NPB), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg_init, dim3(NPB, 3), dim3(256), 0, c->st, a);
   hipLaunchKernelGGL(k_cg_fin0, dim3(3), dim3(256), 0, c->st, a);
-  hipLaunchKernelGGL(k_cg_r0, dim3(NPB), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg_r0, dim3(NPB, 3), dim3(256), 0, c->st, a);
   hipLaunchKernelGGL(k_cg_fin1, dim3(3), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }
@@ -534,7 +532,7 @@ hipError_t launch_cg_init(wpic_ctx* c) {
 hipError_t launch_cg_start(wpic_ctx* c) {
   CgArgs a = cg_args(c);
   hipLaunchKernelGGL(k_cg_fin0, dim3(3), dim3(256), 0, c->st, a);
-  hipLaunchKernelGGL(k_cg_r0, dim3(NPB), dim3(256), 0, c->st, a);
+  hipLaunchKernelGGL(k_cg_r0, dim3(NPB, 3), dim3(256), 0, c->st, a);
   hipLaunchKernelGGL(k_cg_fin1, dim3(3), dim3(256), 0, c->st, a);
   return hipGetLastError();
 }
